@@ -1,0 +1,191 @@
+/*
+ * tsphere.h — C ABI of the MI355X (gfx950) spectral IMEX library.
+ *
+ * This is the drop-in boundary for the reference's hot path: the spherical-
+ * harmonic transform plan of src/sht/sht.ts and the element-wise-kernel /
+ * step-recording backend of src/mgpu/{plan,wgsl,model}.ts.  Everything is
+ * plain C: integer status codes (TSPHERE_OK = 0, never abort), opaque
+ * handles, caller-owned device buffers, plan-owned tables and scratch, and
+ * no implicit synchronisation except in the functions documented as blocking
+ * (the host<->device reads, as in the reference, whose only await is the
+ * readback, src/mgpu/model.ts:261).
+ *
+ * Threading: one host thread drives a plan; all work of a plan is ordered on
+ * the stream passed to each call.  Calls that overlap on one plan from
+ * different streams race on its scratch (the reference's shared fmBuf has the
+ * same rule, src/sht/sht.ts:291-298).
+ *
+ * Spectral layout (bit-exact with src/sht/layout.ts:1-32): complex Q_lm for
+ * m >= 0 stored as interleaved float pairs [re, im], m-major (for m = 0..mmax,
+ * for l = m..lmax), index lm = m(lmax+1) - m(m-1)/2 + (l-m).  Spatial layout:
+ * float spat[ilat*nphi + iphi], ilat north -> south (Gauss nodes decreasing),
+ * phi_j = 2 pi j / nphi.  Orthonormal harmonics with Condon-Shortley phase.
+ */
+#ifndef TSPHERE_H
+#define TSPHERE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define TSPHERE_OK 0
+#define TSPHERE_ERR_INVALID 1     /* bad argument or configuration            */
+#define TSPHERE_ERR_HIP 2         /* HIP runtime error (message has details)  */
+#define TSPHERE_ERR_COMPILE 3     /* hipRTC compile failure (log in message)  */
+#define TSPHERE_ERR_UNSUPPORTED 4 /* valid request this build cannot serve     */
+#define TSPHERE_ERR_NODEVICE 5    /* no GPU visible                            */
+
+/* Library version, as 10000*major + 100*minor + patch. */
+int tsphere_version(void);
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *tsphere_last_error(void);
+/* Number of visible GPUs (0 without a device; never an error). */
+int tsphere_device_count(int *count);
+
+/* ------------------------------------------------------- device plumbing */
+typedef void *tsphere_stream; /* a hipStream_t; NULL = the null stream */
+typedef struct tsphere_event_t *tsphere_event;
+
+int tsphere_stream_create(tsphere_stream *out); /* default priority only */
+int tsphere_stream_destroy(tsphere_stream s);
+int tsphere_stream_sync(tsphere_stream s);      /* blocking */
+int tsphere_device_sync(void);                  /* blocking */
+
+int tsphere_malloc(size_t bytes, void **out);
+int tsphere_free(void *ptr);
+/* Host -> device, enqueued on s (the host buffer must stay alive until s
+ * reaches the copy; tsphere_stream_sync to be sure). */
+int tsphere_memcpy_htod(void *dst, const void *src, size_t bytes, tsphere_stream s);
+/* Device -> host: blocking (waits for s, then copies). */
+int tsphere_memcpy_dtoh(void *dst, const void *src, size_t bytes, tsphere_stream s);
+int tsphere_memcpy_dtod(void *dst, const void *src, size_t bytes, tsphere_stream s);
+/* dst[i] = value for i < count, by a kernel (safe inside graph capture). */
+int tsphere_fill_f32(float *dst, float value, long long count, tsphere_stream s);
+
+int tsphere_event_create(tsphere_event *out);
+int tsphere_event_destroy(tsphere_event e);
+int tsphere_event_record(tsphere_event e, tsphere_stream s);
+/* Milliseconds between two recorded events (blocking on `end`). */
+int tsphere_event_elapsed(tsphere_event start, tsphere_event end, float *ms);
+
+/* -------------------------------------------------- the transform plan
+ * Replaces ShtPlan (src/sht/sht.ts:116-469).  The plan computes its tables in
+ * f64 on the host (Gauss nodes/weights gauss.ts:11-51, recurrence
+ * coefficients coeffs.ts:28-56, the seed ytilde_m^m = amm sin^m(theta) with a
+ * scaled exponent, the twiddle table sht.ts:200-205) and keeps them on the
+ * device as f32, plus a scratch Fourier buffer for `max_fields` fields.
+ */
+#define TSPHERE_FOURIER_AUTO 0 /* FFT when nphi is a power of two >= 8, else DFT */
+#define TSPHERE_FOURIER_FFT 1
+#define TSPHERE_FOURIER_DFT 2
+
+#define TSPHERE_LEG_FP32 0   /* Legendre stage on the FP32 VALU                */
+#define TSPHERE_LEG_BF16X3 1 /* Legendre stage on MFMA, split-BF16 x3, FP32 acc */
+
+typedef struct {
+  int lmax, mmax, nlat, nphi; /* validated as layout.ts:34-43               */
+  int fourier;                /* TSPHERE_FOURIER_*  (ShtOptions.fourier)     */
+  int legendre;               /* TSPHERE_LEG_*                               */
+  int max_fields;             /* fields per batched call (>= 1)             */
+} tsphere_sht_config;
+
+typedef struct {
+  int lmax, mmax, nlat, nphi;
+  long long nlm;      /* layout.ts:24-27 */
+  int fourier;        /* resolved: TSPHERE_FOURIER_FFT or _DFT (ShtPlan.fourierMode) */
+  int legendre;
+  int leg_lat;        /* latitudes one Legendre lane walks: ceil(nlat/2) (ShtPlan.legLat) */
+  int max_fields;
+  size_t scratch_bytes; /* plan-owned device memory: tables + Fourier scratch */
+} tsphere_plan_info;
+
+typedef struct tsphere_plan_t *tsphere_plan;
+
+/* ShtPlan.create (sht.ts:166-183): validates (TSPHERE_ERR_INVALID with the
+ * reference's message), resolves the Fourier mode (an explicit FFT request
+ * that cannot be met is TSPHERE_ERR_INVALID, sht.ts:173-178). */
+int tsphere_plan_create(const tsphere_sht_config *cfg, tsphere_plan *out);
+int tsphere_plan_destroy(tsphere_plan p); /* ShtPlan.destroy (sht.ts:463-468) */
+int tsphere_plan_query(tsphere_plan p, tsphere_plan_info *info);
+/* ShtPlan.theta / cosTheta / gaussWeights (sht.ts:122-125): f64, nlat each;
+ * any pointer may be NULL. */
+int tsphere_plan_grid(tsphere_plan p, double *theta, double *cos_theta, double *weights);
+
+/* The plan's f32 device tables, computed on the host without a device (for
+ * inspection and tests).  Any pointer may be NULL.  Sizes: ab 2*nlm,
+ * amm mmax+1, ctstw 3*nlat ([cos | sin | w*2pi/nphi], sht.ts:194-199),
+ * seed_y/seed_ny (mmax+1)*ceil(nlat/2) (ytilde_m^m at latitude pair j as
+ * seed_y * 2^(56*seed_ny), seed_ny <= 0), trig 2*nphi. */
+int tsphere_plan_tables(const tsphere_sht_config *cfg, float *ab, float *amm, float *ctstw,
+                        float *seed_y, int *seed_ny, float *trig);
+
+/* synth / analys on caller-owned device buffers (createSynthBinding +
+ * encodeSynthInto, sht.ts:299-338; createAnalysBinding + encodeAnalysInto,
+ * sht.ts:312-353).  qlm: 2*nlm floats; spat: nlat*nphi floats.  Enqueued on
+ * s, no synchronisation. */
+int tsphere_synth(tsphere_plan p, const float *qlm, float *spat, tsphere_stream s);
+int tsphere_analys(tsphere_plan p, const float *spat, float *qlm, tsphere_stream s);
+
+/* Batched: nfield <= max_fields fields, field f at qlm + f*qlm_stride and
+ * spat + f*spat_stride (strides in floats).  One launch per stage for all
+ * fields, so the Legendre recurrence is shared by every field. */
+int tsphere_synth_batch(tsphere_plan p, int nfield, const float *qlm, long long qlm_stride,
+                        float *spat, long long spat_stride, tsphere_stream s);
+int tsphere_analys_batch(tsphere_plan p, int nfield, const float *spat, long long spat_stride,
+                         float *qlm, long long qlm_stride, tsphere_stream s);
+
+/* Single stages (ShtPlan.encodeStage, sht.ts:368-401; what
+ * scripts/diagnose-sht.ts reads to tell the halves apart).  fm is the
+ * Fourier-space intermediate, complex [(f*(mmax+1) + m)*nlat + ilat] as
+ * interleaved floats (for one field, the reference's fmBuf layout). */
+int tsphere_leg_synth(tsphere_plan p, int nfield, const float *qlm, long long qlm_stride,
+                      float *fm, tsphere_stream s);
+int tsphere_fourier_synth(tsphere_plan p, int nfield, const float *fm, float *spat,
+                          long long spat_stride, tsphere_stream s);
+int tsphere_fourier_analys(tsphere_plan p, int nfield, const float *spat,
+                           long long spat_stride, float *fm, tsphere_stream s);
+int tsphere_leg_analys(tsphere_plan p, int nfield, const float *fm, float *qlm,
+                       long long qlm_stride, tsphere_stream s);
+
+/* ------------------------------------------- generated element-wise kernels
+ * Replaces the WGSL pipeline creation of src/mgpu/plan.ts:145-170 for the
+ * kernels buildKernel (src/mgpu/wgsl.ts:321-372) emits: HIP C++ source is
+ * compiled for gfx950 by hipRTC at model-compile time.  A compile failure is
+ * TSPHERE_ERR_COMPILE with the hipRTC log in tsphere_last_error(). */
+typedef struct tsphere_module_t *tsphere_module;
+typedef struct tsphere_kernel_t *tsphere_kernel;
+
+/* options: extra hipRTC options separated by '\n' (may be NULL). */
+int tsphere_module_compile(const char *source, const char *options, tsphere_module *out);
+int tsphere_module_destroy(tsphere_module m);
+int tsphere_module_kernel(tsphere_module m, const char *name, tsphere_kernel *out);
+int tsphere_kernel_destroy(tsphere_kernel k); /* the handle only; the module owns the code */
+/* Compile-only check (no device needed): 0 if `source` compiles for gfx950. */
+int tsphere_rtc_check(const char *source, const char *options);
+
+/* Launch a generated element-wise kernel whose signature is
+ *   (long long count, float *out, const float *in0.., float s0..)
+ * over `count` elements: nptr device pointers (out first), nscalar floats
+ * passed BY VALUE (changing a parameter never recompiles, as the reference's
+ * params buffer, plan.ts:410-420). */
+int tsphere_kernel_launch(tsphere_kernel k, long long count, int nptr, void *const *ptrs,
+                          int nscalar, const float *scalars, tsphere_stream s);
+
+/* ------------------------------------------------------- step recording
+ * The reference re-records its op list every step (plan.ts:436-478); here a
+ * step is captured once into a HIP graph and replayed.  Capture what the
+ * stream does between begin and end (kernels, transforms, copies). */
+typedef struct tsphere_graph_t *tsphere_graph;
+
+int tsphere_graph_begin(tsphere_stream s);
+int tsphere_graph_end(tsphere_stream s, tsphere_graph *out);
+int tsphere_graph_launch(tsphere_graph g, tsphere_stream s);
+int tsphere_graph_destroy(tsphere_graph g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSPHERE_H */

@@ -1,0 +1,36 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "turing-sphere_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+
+
+def _has_gpu() -> bool:
+    try:
+        import tsphere
+
+        return tsphere.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Skip-free guard: a gpu-marked test on a box without a GPU is an error,
+    not a silent pass."""
+    if not _has_gpu():
+        pytest.fail("no GPU visible to libtsphere (gpu tests need an MI355X)")
+    return True
+
+
+def emit(msg: str) -> None:
+    """Print a progress line immediately (names the case before GPU work)."""
+    print(msg, flush=True)

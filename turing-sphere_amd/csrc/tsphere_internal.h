@@ -1,0 +1,87 @@
+// Internal declarations shared by the C-ABI runtime (tsphere_runtime.hip)
+// and the gfx950 kernels (sht_*.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "tsphere.h"
+
+namespace tsphere {
+
+// ytilde_m^m at one latitude pair: value = y * 2^(56*ny), ny <= 0.  The host
+// computes it in f64 in the log domain, so no underflow at any m.
+struct Seed {
+  float y;
+  int ny;
+};
+
+// fp32 extended-range constants of the recurrence (the SHTNS values the
+// reference injects, src/sht/wgsl/common.ts:17-20): a lane whose running value
+// carries a negative exponent `ny` does not accumulate until the value is back
+// above ACCURACY in true magnitude.
+constexpr float kScale = 72057594037927936.0f;         // 2^56
+constexpr float kInvScale = 1.0f / 72057594037927936.0f;  // 2^-56, exact
+constexpr float kAccuracy = 1e-15f;
+constexpr float kRescaleThr = kAccuracy * kScale + 1.0f;  // ~73.06
+
+// Device tables of one plan (all f32 except the seed exponent).
+struct LegTables {
+  const float2* ab;      // [nlm + pad]: (a_l^m, b_l^m), m-major like Q
+  const float* ct;       // [npair]: cos(theta_j), northern latitude of pair j
+  const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
+  const Seed* seed;      // [(mmax+1) * npair]
+  int lmax, mmax, nlat, npair;
+};
+
+struct Plan {
+  tsphere_sht_config cfg{};
+  int fourier = TSPHERE_FOURIER_FFT;
+  int npair = 0;
+  long long nlm = 0;
+  float2* d_ab = nullptr;
+  float* d_ct = nullptr;
+  float* d_w = nullptr;
+  Seed* d_seed = nullptr;
+  float2* d_trig = nullptr;
+  float2* d_fm = nullptr;  // Fourier scratch, max_fields * (mmax+1) * nlat
+  size_t bytes = 0;
+  std::vector<double> theta, cos_theta, weights;
+  LegTables tables() const {
+    return LegTables{d_ab, d_ct, d_w, d_seed, cfg.lmax, cfg.mmax, cfg.nlat, npair};
+  }
+};
+
+// Host-side table construction (no device needed).
+struct HostTables {
+  std::vector<float> ab;       // 2*(nlm + pad)
+  std::vector<float> amm;      // mmax+1
+  std::vector<float> ct, st, w;  // nlat each
+  std::vector<Seed> seed;      // (mmax+1)*npair
+  std::vector<float> trig;     // 2*nphi
+  std::vector<double> x, wg;   // f64 Gauss nodes / weights
+};
+constexpr int kAbPad = 4;
+void build_host_tables(const tsphere_sht_config& cfg, HostTables& t);
+long long nlm_calc(int lmax, int mmax);
+
+// Kernel launchers (sht_legendre.hip, sht_fourier.hip).  All enqueue on `s`.
+hipError_t launch_leg_synth(const LegTables& T, int nfield, const float2* q, long long qstride,
+                            float2* fm, hipStream_t s);
+hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, float2* q,
+                             long long qstride, hipStream_t s);
+hipError_t launch_leg_synth_mfma(const LegTables& T, int nfield, const float2* q,
+                                 long long qstride, float2* fm, hipStream_t s);
+hipError_t launch_leg_analys_mfma(const LegTables& T, int nfield, const float2* fm, float2* q,
+                                  long long qstride, hipStream_t s);
+hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const float2* trig,
+                                int nfield, const float2* fm, float* spat, long long sstride,
+                                hipStream_t s);
+hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, const float2* trig,
+                                 int nfield, const float* spat, long long sstride, float2* fm,
+                                 hipStream_t s);
+hipError_t launch_fill(float* dst, float v, long long n, hipStream_t s);
+
+}  // namespace tsphere

@@ -1,0 +1,707 @@
+// C-ABI runtime of the gfx950 spectral IMEX library (declared in
+// include/tsphere.h): error plumbing, device memory, streams/events, the
+// transform plan (host f64 precompute -> f32 device tables), hipRTC modules
+// for the generated element-wise kernels, and HIP-graph step capture.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tsphere_internal.h"
+
+using namespace tsphere;
+
+// ---------------------------------------------------------------- errors
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    return fail(TSPHERE_ERR_NODEVICE, "%s: no GPU (%s)", what, hipGetErrorString(e));
+  return fail(TSPHERE_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define TS_HIP(call, what)                       \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int ok() {
+  g_err.clear();
+  return TSPHERE_OK;
+}
+
+hipStream_t S(tsphere_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+}  // namespace
+
+// ------------------------------------------------------------ host tables
+namespace tsphere {
+
+long long nlm_calc(int lmax, int mmax) {
+  return (long long)(mmax + 1) * (lmax + 1) - ((long long)mmax * (mmax + 1)) / 2;
+}
+
+namespace {
+long long lm_index(int lmax, int l, int m) {
+  return (long long)m * (lmax + 1) - ((long long)m * (m - 1)) / 2 + (l - m);
+}
+
+// Gauss-Legendre nodes (decreasing) and weights: Newton on P_n from the same
+// initial guess and stopping rule as src/sht/gauss.ts:11-51.
+void gauss(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const int half = (n + 1) >> 1;
+  for (int i = 0; i < half; i++) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int iter = 0; iter < 100; iter++) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double dz = p1 / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-15 * std::fabs(z) + 1e-300) {
+        double q1 = 1.0, q2 = 0.0;
+        for (int j = 1; j <= n; j++) {
+          const double q3 = q2;
+          q2 = q1;
+          q1 = ((2.0 * j - 1.0) * z * q2 - (j - 1.0) * q3) / j;
+        }
+        pp = n * (z * q1 - q2) / (z * z - 1.0);
+        z -= q1 / pp;
+        break;
+      }
+    }
+    x[i] = z;
+    x[n - 1 - i] = -z;
+    const double wi = 2.0 / ((1.0 - z * z) * pp * pp);
+    w[i] = wi;
+    w[n - 1 - i] = wi;
+  }
+  if (n & 1) x[half - 1] = 0.0;
+}
+}  // namespace
+
+void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
+  const int lmax = c.lmax, mmax = c.mmax, nlat = c.nlat, nphi = c.nphi;
+  const long long nlm = nlm_calc(lmax, mmax);
+  const int npair = (nlat + 1) / 2;
+
+  // Recurrence coefficients, coeffs.ts:28-56 (f64, then f32).
+  std::vector<double> amm(mmax + 1);
+  double tt = 1.0 / (4.0 * M_PI);
+  amm[0] = std::sqrt(tt);
+  for (int m = 1; m <= mmax; m++) {
+    tt *= (2.0 * m + 1.0) / (2.0 * m);
+    amm[m] = -std::sqrt(tt);
+    if (m % 2 == 0) amm[m] = -amm[m];
+  }
+  t.ab.assign(2 * (nlm + kAbPad), 0.0f);
+  for (int m = 0; m <= mmax; m++) {
+    if (m + 1 <= lmax) t.ab[2 * lm_index(lmax, m + 1, m)] = (float)std::sqrt(2.0 * m + 3.0);
+    for (int l = m + 2; l <= lmax; l++) {
+      const long long lm = lm_index(lmax, l, m);
+      const double t1 = (double)(l + m) * (l - m);
+      const double t2 = (double)(l - 1 + m) * (l - 1 - m);
+      t.ab[2 * lm] = (float)std::sqrt(((2.0 * l + 1.0) * (2.0 * l - 1.0)) / t1);
+      t.ab[2 * lm + 1] = (float)-std::sqrt(((2.0 * l + 1.0) / (2.0 * l - 3.0)) * (t2 / t1));
+    }
+  }
+  t.amm.resize(mmax + 1);
+  for (int m = 0; m <= mmax; m++) t.amm[m] = (float)amm[m];
+
+  // Grid tables, sht.ts:194-199.
+  gauss(nlat, t.x, t.wg);
+  t.ct.resize(nlat);
+  t.st.resize(nlat);
+  t.w.resize(nlat);
+  for (int i = 0; i < nlat; i++) {
+    t.ct[i] = (float)t.x[i];
+    t.st[i] = (float)std::sqrt(1.0 - t.x[i] * t.x[i]);
+    t.w[i] = (float)(t.wg[i] * ((2.0 * M_PI) / nphi));
+  }
+
+  // Seed ytilde_m^m(theta_j) = amm[m] sin^m(theta_j), in the log domain:
+  // e = log2|amm| + m log2(sin theta).  A value below ACCURACY is carried as
+  // y * 2^(56 ny) with |y| <= RESCALE_THR, exactly the state the reference's
+  // binary exponentiation (common.ts:27-55) hands the recurrence.
+  const double log2_thr = std::log2((double)kRescaleThr);
+  t.seed.resize((size_t)(mmax + 1) * npair);
+  for (int m = 0; m <= mmax; m++) {
+    const double la = std::log2(std::fabs(amm[m]));
+    const double sg = amm[m] < 0 ? -1.0 : 1.0;
+    for (int j = 0; j < npair; j++) {
+      const double st = std::sqrt(1.0 - t.x[j] * t.x[j]);
+      const double e = la + (m == 0 ? 0.0 : m * std::log2(st));
+      int ny = (int)std::ceil((e - log2_thr) / 56.0);
+      if (ny > 0) ny = 0;
+      const double y = sg * std::exp2(e - 56.0 * ny);
+      t.seed[(size_t)m * npair + j] = Seed{(float)y, ny};
+    }
+  }
+
+  // Twiddles in f64 (sht.ts:200-205).
+  t.trig.resize(2 * (size_t)nphi);
+  for (int k = 0; k < nphi; k++) {
+    t.trig[2 * k] = (float)std::cos((2.0 * M_PI * k) / nphi);
+    t.trig[2 * k + 1] = (float)std::sin((2.0 * M_PI * k) / nphi);
+  }
+}
+
+}  // namespace tsphere
+
+// ================================================================= C ABI
+extern "C" {
+
+int tsphere_version(void) { return 100; }
+
+const char* tsphere_last_error(void) { return g_err.c_str(); }
+
+int tsphere_device_count(int* count) {
+  if (!count) return fail(TSPHERE_ERR_INVALID, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  (void)hipGetLastError();
+  *count = n;
+  return ok();
+}
+
+// ---------------------------------------------------------- plumbing
+int tsphere_stream_create(tsphere_stream* out) {
+  if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
+  hipStream_t s = nullptr;
+  TS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  *out = s;
+  return ok();
+}
+
+int tsphere_stream_destroy(tsphere_stream s) {
+  if (s) TS_HIP(hipStreamDestroy(S(s)), "hipStreamDestroy");
+  return ok();
+}
+
+int tsphere_stream_sync(tsphere_stream s) {
+  TS_HIP(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
+  return ok();
+}
+
+int tsphere_device_sync(void) {
+  TS_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return ok();
+}
+
+int tsphere_malloc(size_t bytes, void** out) {
+  if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) return ok();
+  TS_HIP(hipMalloc(out, bytes), "hipMalloc");
+  return ok();
+}
+
+int tsphere_free(void* ptr) {
+  if (ptr) TS_HIP(hipFree(ptr), "hipFree");
+  return ok();
+}
+
+int tsphere_memcpy_htod(void* dst, const void* src, size_t bytes, tsphere_stream s) {
+  if (bytes == 0) return ok();
+  if (!dst || !src) return fail(TSPHERE_ERR_INVALID, "memcpy_htod: NULL pointer");
+  TS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)), "hipMemcpyAsync(H2D)");
+  return ok();
+}
+
+int tsphere_memcpy_dtoh(void* dst, const void* src, size_t bytes, tsphere_stream s) {
+  if (bytes == 0) return ok();
+  if (!dst || !src) return fail(TSPHERE_ERR_INVALID, "memcpy_dtoh: NULL pointer");
+  TS_HIP(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
+  TS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy(D2H)");
+  return ok();
+}
+
+int tsphere_memcpy_dtod(void* dst, const void* src, size_t bytes, tsphere_stream s) {
+  if (bytes == 0) return ok();
+  if (!dst || !src) return fail(TSPHERE_ERR_INVALID, "memcpy_dtod: NULL pointer");
+  TS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s)), "hipMemcpyAsync(D2D)");
+  return ok();
+}
+
+int tsphere_fill_f32(float* dst, float value, long long count, tsphere_stream s) {
+  if (count < 0 || (count > 0 && !dst)) return fail(TSPHERE_ERR_INVALID, "fill: bad arguments");
+  TS_HIP(launch_fill(dst, value, count, S(s)), "fill kernel");
+  return ok();
+}
+
+struct tsphere_event_t {
+  hipEvent_t e;
+};
+
+int tsphere_event_create(tsphere_event* out) {
+  if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
+  hipEvent_t e;
+  TS_HIP(hipEventCreate(&e), "hipEventCreate");
+  *out = new tsphere_event_t{e};
+  return ok();
+}
+
+int tsphere_event_destroy(tsphere_event e) {
+  if (e) {
+    hipError_t r = hipEventDestroy(e->e);
+    delete e;
+    if (r != hipSuccess) return hip_fail(r, "hipEventDestroy");
+  }
+  return ok();
+}
+
+int tsphere_event_record(tsphere_event e, tsphere_stream s) {
+  if (!e) return fail(TSPHERE_ERR_INVALID, "event is NULL");
+  TS_HIP(hipEventRecord(e->e, S(s)), "hipEventRecord");
+  return ok();
+}
+
+int tsphere_event_elapsed(tsphere_event a, tsphere_event b, float* ms) {
+  if (!a || !b || !ms) return fail(TSPHERE_ERR_INVALID, "event_elapsed: NULL argument");
+  TS_HIP(hipEventSynchronize(b->e), "hipEventSynchronize");
+  TS_HIP(hipEventElapsedTime(ms, a->e, b->e), "hipEventElapsedTime");
+  return ok();
+}
+
+// ------------------------------------------------------------- the plan
+struct tsphere_plan_t : Plan {};
+
+static int validate(const tsphere_sht_config* c) {
+  if (!c) return fail(TSPHERE_ERR_INVALID, "config is NULL");
+  // Messages of layout.ts:34-43.
+  if (c->lmax < 1) return fail(TSPHERE_ERR_INVALID, "lmax must be an integer >= 1 (got %d)", c->lmax);
+  if (c->mmax < 0 || c->mmax > c->lmax)
+    return fail(TSPHERE_ERR_INVALID, "mmax must be an integer in [0, lmax] (got %d)", c->mmax);
+  if (c->nlat <= c->lmax)
+    return fail(TSPHERE_ERR_INVALID,
+                "nlat must be an integer > lmax for exact Gauss quadrature (got nlat=%d, lmax=%d)",
+                c->nlat, c->lmax);
+  if (c->nphi < 2 * c->mmax + 1)
+    return fail(TSPHERE_ERR_INVALID,
+                "nphi must be an integer >= 2*mmax+1 to avoid aliasing (got nphi=%d, mmax=%d)",
+                c->nphi, c->mmax);
+  if (c->nlat > 4096)
+    return fail(TSPHERE_ERR_UNSUPPORTED, "nlat=%d exceeds this build's limit of 4096", c->nlat);
+  if (c->fourier < TSPHERE_FOURIER_AUTO || c->fourier > TSPHERE_FOURIER_DFT)
+    return fail(TSPHERE_ERR_INVALID, "unknown fourier mode %d", c->fourier);
+  if (c->legendre != TSPHERE_LEG_FP32 && c->legendre != TSPHERE_LEG_BF16X3)
+    return fail(TSPHERE_ERR_INVALID, "unknown legendre mode %d", c->legendre);
+  if (c->max_fields < 1) return fail(TSPHERE_ERR_INVALID, "max_fields must be >= 1");
+  return TSPHERE_OK;
+}
+
+// The FFT path needs a power-of-two nphi whose half fits the per-wave LDS
+// slice (the reference's fftFits test, sht.ts:169-172, with gfx950 limits).
+static bool fft_fits(int nphi) { return is_pow2(nphi) && nphi >= 8 && nphi <= 4096; }
+
+int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, float* ctstw,
+                        float* seed_y, int* seed_ny, float* trig) {
+  if (int r = validate(cfg)) return r;
+  HostTables t;
+  build_host_tables(*cfg, t);
+  const long long nlm = nlm_calc(cfg->lmax, cfg->mmax);
+  const int nlat = cfg->nlat, npair = (nlat + 1) / 2;
+  if (ab) std::memcpy(ab, t.ab.data(), sizeof(float) * 2 * nlm);
+  if (amm) std::memcpy(amm, t.amm.data(), sizeof(float) * (cfg->mmax + 1));
+  if (ctstw) {
+    for (int i = 0; i < nlat; i++) {
+      ctstw[i] = t.ct[i];
+      ctstw[nlat + i] = t.st[i];
+      ctstw[2 * nlat + i] = t.w[i];
+    }
+  }
+  const size_t ns = (size_t)(cfg->mmax + 1) * npair;
+  if (seed_y)
+    for (size_t i = 0; i < ns; i++) seed_y[i] = t.seed[i].y;
+  if (seed_ny)
+    for (size_t i = 0; i < ns; i++) seed_ny[i] = t.seed[i].ny;
+  if (trig) std::memcpy(trig, t.trig.data(), sizeof(float) * 2 * cfg->nphi);
+  return ok();
+}
+
+static void plan_free(Plan* p) {
+  if (!p) return;
+  for (void* ptr : {(void*)p->d_ab, (void*)p->d_ct, (void*)p->d_w, (void*)p->d_seed,
+                    (void*)p->d_trig, (void*)p->d_fm})
+    if (ptr) (void)hipFree(ptr);
+}
+
+}  // extern "C"
+
+template <typename T>
+static hipError_t upload(T** dst, const T* src, size_t n) {
+  hipError_t e = hipMalloc((void**)dst, sizeof(T) * n);
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
+}
+
+extern "C" {
+
+int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
+  if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (int r = validate(cfg)) return r;
+  const bool fits = fft_fits(cfg->nphi);
+  if (cfg->fourier == TSPHERE_FOURIER_FFT && !fits)
+    return fail(TSPHERE_ERR_INVALID,
+                "fourier:'fft' requires power-of-two nphi in [8, 4096] (nphi=%d)", cfg->nphi);
+  if (cfg->legendre == TSPHERE_LEG_BF16X3)
+    return fail(TSPHERE_ERR_UNSUPPORTED, "split-BF16 MFMA Legendre stage not built");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(TSPHERE_ERR_NODEVICE, "tsphere_plan_create: no GPU visible");
+  }
+
+  auto* p = new (std::nothrow) tsphere_plan_t();
+  if (!p) return fail(TSPHERE_ERR_INVALID, "out of host memory");
+  p->cfg = *cfg;
+  p->fourier = (cfg->fourier == TSPHERE_FOURIER_DFT || !fits) ? TSPHERE_FOURIER_DFT
+                                                               : TSPHERE_FOURIER_FFT;
+  p->npair = (cfg->nlat + 1) / 2;
+  p->nlm = nlm_calc(cfg->lmax, cfg->mmax);
+
+  HostTables t;
+  build_host_tables(*cfg, t);
+  p->theta.resize(cfg->nlat);
+  p->cos_theta = t.x;
+  p->weights = t.wg;
+  for (int i = 0; i < cfg->nlat; i++) p->theta[i] = std::acos(t.x[i]);
+
+  hipError_t e = hipSuccess;
+  const size_t nfm = (size_t)cfg->max_fields * (cfg->mmax + 1) * cfg->nlat;
+  if (e == hipSuccess) e = upload(&p->d_ab, reinterpret_cast<const float2*>(t.ab.data()),
+                                  (size_t)(p->nlm + kAbPad));
+  if (e == hipSuccess) e = upload(&p->d_ct, t.ct.data(), (size_t)p->npair);
+  if (e == hipSuccess) e = upload(&p->d_w, t.w.data(), (size_t)cfg->nlat);
+  if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
+  if (e == hipSuccess)
+    e = upload(&p->d_trig, reinterpret_cast<const float2*>(t.trig.data()), (size_t)cfg->nphi);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_fm, sizeof(float2) * nfm);
+  if (e != hipSuccess) {
+    plan_free(p);
+    delete p;
+    return hip_fail(e, "tsphere_plan_create");
+  }
+  p->bytes = sizeof(float2) * (p->nlm + kAbPad) + sizeof(float) * (p->npair + cfg->nlat) +
+             sizeof(Seed) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm;
+  *out = p;
+  return ok();
+}
+
+int tsphere_plan_destroy(tsphere_plan p) {
+  if (p) {
+    plan_free(p);
+    delete p;
+  }
+  return ok();
+}
+
+int tsphere_plan_query(tsphere_plan p, tsphere_plan_info* info) {
+  if (!p || !info) return fail(TSPHERE_ERR_INVALID, "plan_query: NULL argument");
+  info->lmax = p->cfg.lmax;
+  info->mmax = p->cfg.mmax;
+  info->nlat = p->cfg.nlat;
+  info->nphi = p->cfg.nphi;
+  info->nlm = p->nlm;
+  info->fourier = p->fourier;
+  info->legendre = p->cfg.legendre;
+  info->leg_lat = p->npair;
+  info->max_fields = p->cfg.max_fields;
+  info->scratch_bytes = p->bytes;
+  return ok();
+}
+
+int tsphere_plan_grid(tsphere_plan p, double* theta, double* cos_theta, double* weights) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  const size_t n = sizeof(double) * p->cfg.nlat;
+  if (theta) std::memcpy(theta, p->theta.data(), n);
+  if (cos_theta) std::memcpy(cos_theta, p->cos_theta.data(), n);
+  if (weights) std::memcpy(weights, p->weights.data(), n);
+  return ok();
+}
+
+// ------------------------------------------------------------ transforms
+static int check_batch(tsphere_plan p, int nfield, const void* a, const void* b) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1 || nfield > p->cfg.max_fields)
+    return fail(TSPHERE_ERR_INVALID, "nfield=%d outside [1, max_fields=%d]", nfield,
+                p->cfg.max_fields);
+  if (!a || !b) return fail(TSPHERE_ERR_INVALID, "NULL buffer");
+  return TSPHERE_OK;
+}
+
+static hipError_t leg_synth(Plan* p, int nfield, const float* qlm, long long qstride, float2* fm,
+                            hipStream_t s) {
+  return launch_leg_synth(p->tables(), nfield, reinterpret_cast<const float2*>(qlm), qstride / 2,
+                          fm, s);
+}
+
+static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, float* qlm, long long qstride,
+                             hipStream_t s) {
+  return launch_leg_analys(p->tables(), nfield, fm, reinterpret_cast<float2*>(qlm), qstride / 2,
+                           s);
+}
+
+int tsphere_leg_synth(tsphere_plan p, int nfield, const float* qlm, long long qlm_stride,
+                      float* fm, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, qlm, fm)) return r;
+  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
+  TS_HIP(leg_synth(p, nfield, qlm, qlm_stride, reinterpret_cast<float2*>(fm), S(s)), "leg_synth");
+  return ok();
+}
+
+int tsphere_leg_analys(tsphere_plan p, int nfield, const float* fm, float* qlm,
+                       long long qlm_stride, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, fm, qlm)) return r;
+  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
+  TS_HIP(leg_analys(p, nfield, reinterpret_cast<const float2*>(fm), qlm, qlm_stride, S(s)),
+         "leg_analys");
+  return ok();
+}
+
+int tsphere_fourier_synth(tsphere_plan p, int nfield, const float* fm, float* spat,
+                          long long spat_stride, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, fm, spat)) return r;
+  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                              nfield, reinterpret_cast<const float2*>(fm), spat, spat_stride,
+                              S(s)),
+         "fourier_synth");
+  return ok();
+}
+
+int tsphere_fourier_analys(tsphere_plan p, int nfield, const float* spat, long long spat_stride,
+                           float* fm, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, spat, fm)) return r;
+  TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                               nfield, spat, spat_stride, reinterpret_cast<float2*>(fm), S(s)),
+         "fourier_analys");
+  return ok();
+}
+
+int tsphere_synth_batch(tsphere_plan p, int nfield, const float* qlm, long long qlm_stride,
+                        float* spat, long long spat_stride, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, qlm, spat)) return r;
+  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
+  TS_HIP(leg_synth(p, nfield, qlm, qlm_stride, p->d_fm, S(s)), "leg_synth");
+  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                              nfield, p->d_fm, spat, spat_stride, S(s)),
+         "fourier_synth");
+  return ok();
+}
+
+int tsphere_analys_batch(tsphere_plan p, int nfield, const float* spat, long long spat_stride,
+                         float* qlm, long long qlm_stride, tsphere_stream s) {
+  if (int r = check_batch(p, nfield, spat, qlm)) return r;
+  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
+  TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                               nfield, spat, spat_stride, p->d_fm, S(s)),
+         "fourier_analys");
+  TS_HIP(leg_analys(p, nfield, p->d_fm, qlm, qlm_stride, S(s)), "leg_analys");
+  return ok();
+}
+
+int tsphere_synth(tsphere_plan p, const float* qlm, float* spat, tsphere_stream s) {
+  return tsphere_synth_batch(p, 1, qlm, 0, spat, 0, s);
+}
+
+int tsphere_analys(tsphere_plan p, const float* spat, float* qlm, tsphere_stream s) {
+  return tsphere_analys_batch(p, 1, spat, 0, qlm, 0, s);
+}
+
+// ------------------------------------------------------ hipRTC modules
+struct tsphere_module_t {
+  hipModule_t mod = nullptr;
+};
+struct tsphere_kernel_t {
+  hipFunction_t fn = nullptr;
+};
+
+static int rtc_compile(const char* source, const char* options, std::vector<char>& code) {
+  if (!source) return fail(TSPHERE_ERR_INVALID, "source is NULL");
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, source, "tsphere_generated.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS)
+    return fail(TSPHERE_ERR_COMPILE, "hiprtcCreateProgram failed");
+  // Element-wise kernels are HBM-bound, so contraction buys nothing; keeping
+  // every f32 operation separately rounded makes them reproducible against
+  // an unfused fp32 evaluation of the same expression.
+  std::vector<std::string> opts = {"--offload-arch=gfx950",
+                                   "-O3",
+                                   "-ffp-contract=off",
+                                   "-fhip-fp32-correctly-rounded-divide-sqrt",
+                                   "-Xclang",
+                                   "-target-feature",
+                                   "-Xclang",
+                                   "-packed-fp32-ops"};
+  if (options) {
+    std::string o(options);
+    size_t pos = 0;
+    while (pos <= o.size()) {
+      size_t nl = o.find('\n', pos);
+      if (nl == std::string::npos) nl = o.size();
+      if (nl > pos) opts.push_back(o.substr(pos, nl - pos));
+      pos = nl + 1;
+    }
+  }
+  std::vector<const char*> argv;
+  for (auto& s : opts) argv.push_back(s.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)argv.size(), argv.data());
+  if (r != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    return fail(TSPHERE_ERR_COMPILE, "hipRTC compile failed (%s):\n%s", hiprtcGetErrorString(r),
+                log.c_str());
+  }
+  size_t n = 0;
+  hiprtcGetCodeSize(prog, &n);
+  code.resize(n);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return TSPHERE_OK;
+}
+
+int tsphere_rtc_check(const char* source, const char* options) {
+  std::vector<char> code;
+  if (int r = rtc_compile(source, options, code)) return r;
+  return ok();
+}
+
+int tsphere_module_compile(const char* source, const char* options, tsphere_module* out) {
+  if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  std::vector<char> code;
+  if (int r = rtc_compile(source, options, code)) return r;
+  hipModule_t mod;
+  TS_HIP(hipModuleLoadData(&mod, code.data()), "hipModuleLoadData");
+  *out = new tsphere_module_t{mod};
+  return ok();
+}
+
+int tsphere_module_destroy(tsphere_module m) {
+  if (m) {
+    hipError_t e = m->mod ? hipModuleUnload(m->mod) : hipSuccess;
+    delete m;
+    if (e != hipSuccess) return hip_fail(e, "hipModuleUnload");
+  }
+  return ok();
+}
+
+int tsphere_module_kernel(tsphere_module m, const char* name, tsphere_kernel* out) {
+  if (!m || !name || !out) return fail(TSPHERE_ERR_INVALID, "module_kernel: NULL argument");
+  hipFunction_t fn;
+  hipError_t e = hipModuleGetFunction(&fn, m->mod, name);
+  if (e != hipSuccess) return fail(TSPHERE_ERR_INVALID, "no kernel named '%s' in module", name);
+  *out = new tsphere_kernel_t{fn};
+  return ok();
+}
+
+int tsphere_kernel_destroy(tsphere_kernel k) {
+  delete k;
+  return ok();
+}
+
+int tsphere_kernel_launch(tsphere_kernel k, long long count, int nptr, void* const* ptrs,
+                          int nscalar, const float* scalars, tsphere_stream s) {
+  if (!k) return fail(TSPHERE_ERR_INVALID, "kernel is NULL");
+  if (count < 0 || nptr < 1 || nptr > 64 || nscalar < 0 || nscalar > 64)
+    return fail(TSPHERE_ERR_INVALID, "kernel_launch: bad count/nptr/nscalar");
+  if (!ptrs || (nscalar && !scalars)) return fail(TSPHERE_ERR_INVALID, "kernel_launch: NULL array");
+  if (count == 0) return ok();
+  // Pack (long long count, pointers..., floats...) with natural alignment.
+  alignas(8) unsigned char args[8 + 8 * 64 + 4 * 64];
+  size_t off = 0;
+  std::memcpy(args + off, &count, 8);
+  off += 8;
+  for (int i = 0; i < nptr; i++) {
+    std::memcpy(args + off, &ptrs[i], 8);
+    off += 8;
+  }
+  for (int i = 0; i < nscalar; i++) {
+    std::memcpy(args + off, &scalars[i], 4);
+    off += 4;
+  }
+  size_t size = off;
+  void* config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                    HIP_LAUNCH_PARAM_END};
+  // Each thread handles 4 consecutive elements (float4 traffic).
+  const long long threads = (count + 3) / 4;
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  TS_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, 256, 1, 1, 0, S(s), nullptr, config),
+         "hipModuleLaunchKernel");
+  return ok();
+}
+
+// ---------------------------------------------------------------- graphs
+struct tsphere_graph_t {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int tsphere_graph_begin(tsphere_stream s) {
+  if (!s) return fail(TSPHERE_ERR_INVALID, "graph capture needs a created stream, not NULL");
+  TS_HIP(hipStreamBeginCapture(S(s), hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+  return ok();
+}
+
+int tsphere_graph_end(tsphere_stream s, tsphere_graph* out) {
+  if (!s || !out) return fail(TSPHERE_ERR_INVALID, "graph_end: NULL argument");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  TS_HIP(hipStreamEndCapture(S(s), &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *out = new tsphere_graph_t{g, x};
+  return ok();
+}
+
+int tsphere_graph_launch(tsphere_graph g, tsphere_stream s) {
+  if (!g) return fail(TSPHERE_ERR_INVALID, "graph is NULL");
+  TS_HIP(hipGraphLaunch(g->exec, S(s)), "hipGraphLaunch");
+  return ok();
+}
+
+int tsphere_graph_destroy(tsphere_graph g) {
+  if (g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+  }
+  return ok();
+}
+
+}  // extern "C"
