@@ -137,6 +137,17 @@ int tsphere_synth_batch(tsphere_plan p, int nfield, const float *qlm, long long 
 int tsphere_analys_batch(tsphere_plan p, int nfield, const float *spat, long long spat_stride,
                          float *qlm, long long qlm_stride, tsphere_stream s);
 
+/* Batched over a pointer list: field f reads qlm[f] / spat[f], nfield <= 8
+ * (and <= max_fields).  This is how a step that transforms several separate
+ * buffers -- synth(U) and synth(V) of a two-species model, whose two
+ * encodeSynthInto calls the reference records back to back (plan.ts:464-469)
+ * -- runs as one launch per stage. */
+#define TSPHERE_MAX_MULTI 8
+int tsphere_synth_multi(tsphere_plan p, int nfield, const float *const *qlm, float *const *spat,
+                        tsphere_stream s);
+int tsphere_analys_multi(tsphere_plan p, int nfield, const float *const *spat, float *const *qlm,
+                         tsphere_stream s);
+
 /* Single stages (ShtPlan.encodeStage, sht.ts:368-401; what
  * scripts/diagnose-sht.ts reads to tell the halves apart).  fm is the
  * Fourier-space intermediate, complex [(f*(mmax+1) + m)*nlat + ilat] as

@@ -1,0 +1,219 @@
+"""The .m models through front end -> HIP emitter -> planner -> GPU, checked
+against closed forms (test/analyticChecks.ts A/B/C), the model checks of
+test/modelChecks.ts, the f64 oracle trajectory, and eager-vs-graph replay."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import emit
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MFILES = os.path.join(ROOT, "turing-sphere_amd", "mfiles")
+
+MODELS = {
+    "schnakenberg": (["a", "b", "D1", "D2", "dt"], ["U", "V"], ["u", "v"]),
+    "brusselator": (["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"]),
+    "allencahn": (["eps2", "dt"], ["U"], ["u"]),
+    "linear": (["c", "D", "dt"], ["U"], ["u"]),
+    "logistic": (["r", "D", "dt"], ["U"], ["u"]),
+}
+# src/mgpu/registry.ts:51-102 defaults
+DEFAULTS = {
+    "schnakenberg": {"a": 0.1, "b": 0.9, "D1": 4e-4, "D2": 8e-3, "dt": 0.05},
+    "brusselator": {"A": 3.0, "B": 9.0, "D1": 3.33e-3, "D2": 1.67e-2, "dt": 0.02},
+    "allencahn": {"eps2": 1e-3, "dt": 0.02},
+}
+
+
+def make(key, lmax, nlat, nphi, params, batch=True):
+    import tsphere
+    from tsphere.model import GpuModel
+
+    cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
+    names, state, view = MODELS[key]
+    sht = tsphere.ShtPlan.create(cfg, max_fields=len(state))
+    src = open(os.path.join(MFILES, key + ".m")).read()
+    gm = GpuModel.create(sht, cfg, src, names, state, view, batch=batch)
+    gm.set_params(params)
+    return sht, gm
+
+
+def test_A_linear_mode_recurrence(gpu):
+    """analyticChecks.ts:73-122"""
+    lmax = 15
+    nlat, nphi = O.grid_for_lmax(lmax, 1)
+    c, D, dt, nsteps = -0.3, 0.01, 0.1, 20
+    sht, gm = make("linear", lmax, nlat, nphi, {"c": c, "D": D, "dt": dt})
+    nlm = O.nlm_calc(lmax, lmax)
+    l, m = 5, 2
+    idx = O.lm_index(lmax, l, m)
+    U0 = np.zeros(2 * nlm, np.float32)
+    U0[2 * idx], U0[2 * idx + 1] = 0.8, -0.35
+    gm.upload("U", U0)
+    gm.step(nsteps)
+    U = gm.read("U")
+    g = (1 + dt * c) / (1 + dt * D * l * (l + 1))
+    f = g ** nsteps
+    err = max(abs(U[2 * idx] - 0.8 * f), abs(U[2 * idx + 1] + 0.35 * f))
+    mask = np.ones(2 * nlm, bool)
+    mask[2 * idx:2 * idx + 2] = False
+    leak = np.max(np.abs(U[mask]))
+    emit(f"A: err {err:.2e} leak {leak:.2e}")
+    assert err < 2e-6 and leak < 2e-6
+    gm.destroy()
+    sht.destroy()
+
+
+def test_B_logistic_uniform_map(gpu):
+    """analyticChecks.ts:124-181"""
+    lmax = 15
+    nlat, nphi = O.grid_for_lmax(lmax, 3)
+    r, D, dt, nsteps, u0 = 0.7, 0.01, 0.05, 25, 0.3
+    sht, gm = make("logistic", lmax, nlat, nphi, {"r": r, "D": D, "dt": dt})
+    gm.init(np.full(nlat * nphi, u0, np.float32))
+    Ustart = gm.read("U")
+    gm.step(nsteps)
+    Uend = gm.read("U")
+    got = u0 * (Uend[0] / Ustart[0])
+    want = u0
+    for _ in range(nsteps):
+        want += dt * r * want * (1 - want)
+    u = gm.read("u")
+    spread = float(u.max() - u.min())
+    emit(f"B: err {abs(got - want):.2e} spread {spread:.2e}")
+    assert abs(got - want) < 5e-6 and spread < 1e-6
+    gm.destroy()
+    sht.destroy()
+
+
+def test_C_schnakenberg_linearised(gpu):
+    """analyticChecks.ts:183-256"""
+    p = DEFAULTS["schnakenberg"]
+    lmax = 31
+    nlat, nphi = O.grid_for_lmax(lmax, 3)
+    sht, gm = make("schnakenberg", lmax, nlat, nphi, p)
+    gm.init(np.zeros(nlat * nphi, np.float32))
+    l, m, eps = 24, 7, 1e-6
+    idx = O.lm_index(lmax, l, m)
+    U0, V0 = gm.read("U"), gm.read("V")
+    Up = U0.copy()
+    Up[2 * idx] += eps
+    gm.upload("U", Up)
+    gm.upload("V", V0)
+    nsteps = 40
+    gm.step(nsteps)
+    U, V = gm.read("U"), gm.read("V")
+    us = p["a"] + p["b"]
+    vs = p["b"] / (us * us)
+    J = [[-1 + 2 * us * vs, us * us], [-2 * us * vs, -us * us]]
+    lam = l * (l + 1)
+    du = 1 / (1 + p["dt"] * p["D1"] * lam)
+    dv = 1 / (1 + p["dt"] * p["D2"] * lam)
+    cu, cv = eps, 0.0
+    for _ in range(nsteps):
+        cu, cv = ((cu + p["dt"] * (J[0][0] * cu + J[0][1] * cv)) * du,
+                  (cv + p["dt"] * (J[1][0] * cu + J[1][1] * cv)) * dv)
+    gu = U[2 * idx] - U0[2 * idx]
+    gv = V[2 * idx] - V0[2 * idx]
+    ru = abs(gu - cu) / max(abs(cu), 1e-30)
+    rv = abs(gv - cv) / max(abs(cv), 1e-30)
+    emit(f"C: rel ({ru:.2e}, {rv:.2e}) growth {abs(cu) / eps:.3f}x")
+    assert ru < 5e-3 and rv < 5e-3
+    assert abs(cu) > eps and abs(gu) > eps
+    gm.destroy()
+    sht.destroy()
+
+
+@pytest.mark.parametrize("key", ["schnakenberg", "brusselator", "allencahn"])
+def test_model_checks(gpu, key):
+    """modelChecks.ts:26-77: compiles, fusion guard, 40 stable steps with
+    contrast, at lmax 31 on the dealiased grid."""
+    lmax = 31
+    nlat, nphi = O.grid_for_lmax(lmax, 3)
+    sht, gm = make(key, lmax, nlat, nphi, DEFAULTS[key])
+    kernels = [l for l in gm.describe()["step"] if l.startswith("kernel")]
+    assert len(kernels) == {"schnakenberg": 5, "brusselator": 5, "allencahn": 2}[key]
+    noise = (0.01 * O.randn(1, nlat * nphi)).astype(np.float32)
+    gm.init(noise)
+    gm.step(40)
+    for field in MODELS[key][2]:
+        v = gm.read(field)
+        contrast = float(v.max() - v.min())
+        emit(f"{key}.{field}: finite {np.all(np.isfinite(v))} contrast {contrast:.3e}")
+        assert np.all(np.isfinite(v)) and contrast > 1e-6
+    gm.destroy()
+    sht.destroy()
+
+
+def test_graph_replay_matches_eager(gpu):
+    """A captured step replayed several times (with eager work and syncs in
+    between) gives exactly the eagerly recorded result every time."""
+    p = DEFAULTS["brusselator"]
+    lmax = 31
+    nlat, nphi = O.grid_for_lmax(lmax, 3)
+    sht, gm = make("brusselator", lmax, nlat, nphi, p)
+    gm.init((0.01 * O.randn(5, nlat * nphi)).astype(np.float32))
+    gm.snapshot_state()
+    gm._step.encode_steps(gm.stream, 7)
+    eager = gm.read("U")
+    for rep in range(3):
+        gm.restore_state()
+        gm.graph_steps = 1 if rep == 0 else 3
+        gm.step(7)
+        got = gm.read("U")
+        emit(f"replay {rep}: max diff {np.max(np.abs(got - eager)):.1e}")
+        assert np.array_equal(got, eager)
+    gm.destroy()
+    sht.destroy()
+
+
+def test_parameter_change_without_recompile(gpu):
+    """A parameter change re-records the launches only; results follow it."""
+    lmax = 15
+    nlat, nphi = O.grid_for_lmax(lmax, 1)
+    sht, gm = make("linear", lmax, nlat, nphi, {"c": -0.3, "D": 0.01, "dt": 0.1})
+    nlm = O.nlm_calc(lmax, lmax)
+    U0 = np.zeros(2 * nlm, np.float32)
+    U0[0] = 1.0
+    module = gm._step._module
+    for c in (-0.3, 0.5):
+        gm.set_params({"c": c, "D": 0.01, "dt": 0.1})
+        gm.upload("U", U0)
+        gm.step(3)
+        U = gm.read("U")
+        assert abs(U[0] - (1 + 0.1 * c) ** 3) < 2e-6
+    assert gm._step._module is module
+    gm.destroy()
+    sht.destroy()
+
+
+def test_plain_method_trajectory_vs_f64(gpu):
+    """Plain method: Schnakenberg, lmax 63 on 64x128, 1000 IMEX steps, against
+    the f64 oracle's trajectory from the same f32 initial state."""
+    lmax, nlat, nphi = 63, 64, 128
+    p = {"a": 0.1, "b": 0.9, "D1": 6.5e-4, "D2": 40 * 6.5e-4, "dt": 0.01}
+    npts = nlat * nphi
+    r = O.mulberry32(0, 2 * npts)
+    u0 = (1.0 + (0.02 * r[:npts] - 0.01)).astype(np.float32)
+    v0 = (0.9 + (0.02 * r[npts:] - 0.01)).astype(np.float32)
+    cfg = O.Cfg(lmax, lmax, nlat, nphi)
+    ref = O.ShtReference(cfg)
+    U0 = ref.analys_fast(u0.astype(np.float64)).astype(np.float32)
+    V0 = ref.analys_fast(v0.astype(np.float64)).astype(np.float32)
+    sht, gm = make("schnakenberg", lmax, nlat, nphi, p)
+    gm.upload("U", U0)
+    gm.upload("V", V0)
+    gm.graph_steps = 10
+    gm.step(1000)
+    U, V = gm.read("U"), gm.read("V")
+    emit("f64 trajectory (1000 steps)...")
+    Ur, Vr = O.imex_trajectory(cfg, O.SCHNAKENBERG, p, U0, V0, 1000, ref)
+    eu, ev = O.rel_l2(U, Ur), O.rel_l2(V, Vr)
+    emit(f"plain method: relL2 U {eu:.2e} V {ev:.2e}")
+    assert eu < 1e-4 and ev < 1e-4
+    gm.destroy()
+    sht.destroy()

@@ -15,6 +15,13 @@ def _plan(cfg, **kw):
     return tsphere.ShtPlan.create(tsphere.ShtConfig(cfg.lmax, cfg.mmax, cfg.nlat, cfg.nphi), **kw)
 
 
+def leg_tol(lmax):
+    """Stated fp32 tolerance of one Legendre stage (and of a round trip) vs f64:
+    2e-6 up to lmax 63 (the plain-method target), then growing linearly with
+    lmax like the forward recurrence's rounding error."""
+    return max(2e-6, 3.2e-8 * (lmax + 1))
+
+
 def _random_field_spectrum(cfg, seed):
     """sigma_l = 1/(l+1) spectrum (the SHT config's law), m=0 real."""
     rng = np.random.default_rng(seed)
@@ -88,12 +95,13 @@ def test_stages_vs_oracle(gpu, lmax, nlat, nphi, fourier):
     qo_ref = ref.legendre_analys(gm.astype(np.float64) * (2 * np.pi / nphi))
     e4 = O.rel_l2(qo, qo_ref)
     emit(f"  legSynth {e1:.2e} fourSynth {e2:.2e} fourAnalys {e3:.2e} legAnalys {e4:.2e}")
-    assert max(e1, e2, e3, e4) < 2e-6
+    assert max(e2, e3) < 2e-7          # the FFT stages: a few ulps
+    assert max(e1, e4) < leg_tol(lmax)
     # end to end, and the round trip is the identity for band-limited input
     rt = plan.analys(plan.synth(q))
     e5 = O.rel_l2(rt, q)
     emit(f"  round trip {e5:.2e}")
-    assert e5 < 2e-6
+    assert e5 < leg_tol(lmax)
     plan.destroy()
 
 
@@ -170,5 +178,5 @@ def test_round_trip_large(gpu, lmax):
     e_s = O.rel_l2(spat, ref.synth_fast(q.astype(np.float64)))
     emit(f"  round trip {e_rt:.2e} synth vs f64 {e_s:.2e}")
     assert np.all(np.isfinite(back))
-    assert e_rt < 1e-4 and e_s < 1e-4
+    assert e_rt < leg_tol(lmax) and e_s < leg_tol(lmax)
     plan.destroy()

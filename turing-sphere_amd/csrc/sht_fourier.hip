@@ -117,7 +117,7 @@ template <int LOG2H>
 __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat,
                                                    const float2* __restrict__ trig,
                                                    const float2* __restrict__ fm,
-                                                   float* __restrict__ spat, long long sstride) {
+                                                   Fields Fs) {
   constexpr int H = 1 << LOG2H;
   constexpr int nphi = 2 * H;
   __shared__ float2 lds[4 * H];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat,
   __syncthreads();
   fft_wave<+1, LOG2H>(buf, trig, lane);
   if (row) {
-    float2* out = reinterpret_cast<float2*>(spat + f * sstride + (long long)ilat * nphi);
+    float2* out = reinterpret_cast<float2*>(const_cast<char*>(Fs.at(f))) + (long long)ilat * H;
     for (int m = lane; m < H; m += 64) out[m] = buf[m];
   }
 }
@@ -153,8 +153,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat,
 template <int LOG2H>
 __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat,
                                                     const float2* __restrict__ trig,
-                                                    const float* __restrict__ spat,
-                                                    long long sstride, float2* __restrict__ fm) {
+                                                    Fields Fs, float2* __restrict__ fm) {
   constexpr int H = 1 << LOG2H;
   constexpr int nphi = 2 * H;
   __shared__ float2 lds[4 * H];
@@ -164,7 +163,7 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat,
   const bool row = ilat < nlat;
   const int il = row ? ilat : nlat - 1;
   float2* buf = lds + wv * H;
-  const float2* in = reinterpret_cast<const float2*>(spat + f * sstride + (long long)il * nphi);
+  const float2* in = reinterpret_cast<const float2*>(Fs.at(f)) + (long long)il * H;
   for (int m = lane; m < H; m += 64) buf[m] = in[m];
   __syncthreads();
   fft_wave<-1, LOG2H>(buf, trig, lane);
@@ -187,7 +186,7 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat,
 __global__ __launch_bounds__(256) void k_dft_synth(int mmax, int nlat, int nphi,
                                                    const float2* __restrict__ trig,
                                                    const float2* __restrict__ fm,
-                                                   float* __restrict__ spat, long long sstride) {
+                                                   Fields Fs) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   const int ilat = blockIdx.y;
   const int f = blockIdx.z;
@@ -202,19 +201,18 @@ __global__ __launch_bounds__(256) void k_dft_synth(int mmax, int nlat, int nphi,
     const float2 c = F[(long long)m * nlat + ilat];
     v += 2.0f * (c.x * t.x - c.y * t.y);
   }
-  spat[f * sstride + (long long)ilat * nphi + j] = v;
+  reinterpret_cast<float*>(const_cast<char*>(Fs.at(f)))[(long long)ilat * nphi + j] = v;
 }
 
 // fourier.ts:266-289: G_m = sum_j f_j e^{-i m phi_j}.
 __global__ __launch_bounds__(256) void k_dft_analys(int mmax, int nlat, int nphi,
                                                     const float2* __restrict__ trig,
-                                                    const float* __restrict__ spat,
-                                                    long long sstride, float2* __restrict__ fm) {
+                                                    Fields Fs, float2* __restrict__ fm) {
   const int m = blockIdx.x * 256 + threadIdx.x;
   const int ilat = blockIdx.y;
   const int f = blockIdx.z;
   if (m > mmax) return;
-  const float* row = spat + f * sstride + (long long)ilat * nphi;
+  const float* row = reinterpret_cast<const float*>(Fs.at(f)) + (long long)ilat * nphi;
   float re = 0.f, im = 0.f;
   int idx = 0;
   for (int j = 0; j < nphi; ++j) {
@@ -241,16 +239,15 @@ int ilog2(int n) {
 }  // namespace
 
 hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                int nfield, const float2* fm, float* spat, long long sstride,
-                                hipStream_t s) {
+                                int nfield, const float2* fm, const Fields& spat, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   if (fourier == TSPHERE_FOURIER_FFT) {
     dim3 grid((nlat + 3) / 4, nfield);
     switch (ilog2(nphi / 2)) {
 #define TS_CASE(L)                                                                          \
   case L:                                                                                   \
-    hipLaunchKernelGGL(k_fft_synth<L>, grid, dim3(256), 0, s, mmax, nlat, trig, fm, spat, \
-                       sstride);                                                            \
+    hipLaunchKernelGGL(k_fft_synth<L>, grid, dim3(256), 0, s, mmax, nlat, trig, fm, spat);                \
+                                                                                            \
     break;
       TS_CASE(2) TS_CASE(3) TS_CASE(4) TS_CASE(5) TS_CASE(6) TS_CASE(7) TS_CASE(8) TS_CASE(9)
       TS_CASE(10) TS_CASE(11)
@@ -259,23 +256,21 @@ hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const
     }
   } else {
     dim3 grid((nphi + 255) / 256, nlat, nfield);
-    hipLaunchKernelGGL(k_dft_synth, grid, dim3(256), 0, s, mmax, nlat, nphi, trig, fm, spat,
-                       sstride);
+    hipLaunchKernelGGL(k_dft_synth, grid, dim3(256), 0, s, mmax, nlat, nphi, trig, fm, spat);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                 int nfield, const float* spat, long long sstride, float2* fm,
-                                 hipStream_t s) {
+                                 int nfield, const Fields& spat, float2* fm, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   if (fourier == TSPHERE_FOURIER_FFT) {
     dim3 grid((nlat + 3) / 4, nfield);
     switch (ilog2(nphi / 2)) {
 #define TS_CASE(L)                                                                           \
   case L:                                                                                    \
-    hipLaunchKernelGGL(k_fft_analys<L>, grid, dim3(256), 0, s, mmax, nlat, trig, spat,     \
-                       sstride, fm);                                                         \
+    hipLaunchKernelGGL(k_fft_analys<L>, grid, dim3(256), 0, s, mmax, nlat, trig, spat, fm);\
+                                                                                             \
     break;
       TS_CASE(2) TS_CASE(3) TS_CASE(4) TS_CASE(5) TS_CASE(6) TS_CASE(7) TS_CASE(8) TS_CASE(9)
       TS_CASE(10) TS_CASE(11)
@@ -284,8 +279,7 @@ hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, cons
     }
   } else {
     dim3 grid((mmax + 1 + 255) / 256, nlat, nfield);
-    hipLaunchKernelGGL(k_dft_analys, grid, dim3(256), 0, s, mmax, nlat, nphi, trig, spat,
-                       sstride, fm);
+    hipLaunchKernelGGL(k_dft_analys, grid, dim3(256), 0, s, mmax, nlat, nphi, trig, spat, fm);
   }
   return hipGetLastError();
 }

@@ -38,11 +38,20 @@ __device__ __forceinline__ long long lm_base(int lmax, int m) {
   return (long long)m * (lmax + 1) - ((long long)m * (m - 1)) / 2;
 }
 
+// a * cos(theta) * y with cos(theta) = ct.x + ct.y (f32 hi + lo of the f64
+// node), plus c: rounding the node to f32 perturbs theta and, through it,
+// every ytilde_l by ~ l * 2^-24 / sin(theta) -- the dominant fp32 error of the
+// plain recurrence.  Carrying the low half costs one FMA.
+__device__ __forceinline__ float act_fma(float a, float y, float2 ct, float c) {
+  const float u = a * y;
+  return fmaf(u, ct.x, fmaf(u, ct.y, c));
+}
+
 // One step of the recurrence: (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}), written
 // as ytilde_l = a_l (cos theta) ytilde_{l-1} + b_l ytilde_{l-2}.
-__device__ __forceinline__ void advance(float2 a0, float2 a1, float ct, float& y0, float& y1) {
-  const float t0 = fmaf(a0.x * ct, y1, a0.y * y0);
-  y1 = fmaf(a1.x * ct, t0, a1.y * y1);
+__device__ __forceinline__ void advance(float2 a0, float2 a1, float2 ct, float& y0, float& y1) {
+  const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
+  y1 = act_fma(a1.x, t0, ct, a1.y * y1);
   y0 = t0;
 }
 
@@ -54,8 +63,7 @@ __device__ __forceinline__ void cfma(float2& acc, float y, float2 q) {
 // ------------------------------------------------------------- synthesis
 template <int NF>
 __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, int nchunk,
-                                                   int nwork, const float2* __restrict__ q,
-                                                   long long qstride,
+                                                   int nwork, Fields Fq,
                                                    float2* __restrict__ fm) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -68,20 +76,20 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, int 
   const bool active = j < T.npair;
   const int jc = active ? j : T.npair - 1;
 
-  const float ct = T.ct[jc];
+  const float2 ct = T.ct[jc];
   const Seed sd = T.seed[(long long)m * T.npair + jc];
   const long long base = lm_base(lmax, m);
   const float2* __restrict__ ab = T.ab + base;  // ab[l - m]
   float y0 = sd.y;
   int ny = sd.ny;
-  float y1 = (m < lmax) ? ab[1].x * ct * y0 : 0.0f;
+  float y1 = (m < lmax) ? act_fma(ab[1].x, y0, ct, 0.0f) : 0.0f;
 
   const float2* qf[NF];
   float2 accE[NF], accO[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const int fld = min(f0 + f, nfield - 1);
-    qf[f] = q + (long long)fld * qstride + base;
+    qf[f] = reinterpret_cast<const float2*>(Fq.at(fld)) + base;
     accE[f] = make_float2(0.f, 0.f);
     accO[f] = make_float2(0.f, 0.f);
   }
@@ -129,7 +137,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, int 
     }
     if (l + 2 <= lmax) {
       const float2 a0 = ab[l + 2 - m];
-      const float t0 = fmaf(a0.x * ct, y1, a0.y * y0);
+      const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
 #pragma unroll
       for (int f = 0; f < NF; ++f) cfma(accE[f], t0, qf[f][l + 2 - m]);
     }
@@ -189,7 +197,7 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 // -------------------------------------------------------------- analysis
 template <int NF, int K, bool CHECKED>
 __device__ __forceinline__ void analys_span(float (&acc)[64], const float2* __restrict__ ab,
-                                            int ls, int m, int lmax, const float (&ct)[K],
+                                            int ls, int m, int lmax, const float2 (&ct)[K],
                                             float (&y0)[K], float (&y1)[K], int (&ny)[K],
                                             const float2 (&wp)[K][NF],
                                             const float2 (&wm)[K][NF]) {
@@ -225,7 +233,7 @@ __device__ __forceinline__ void analys_span(float (&acc)[64], const float2* __re
 template <int NF, int K>
 __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int nwaves,
                                                     const float2* __restrict__ fm,
-                                                    float2* __restrict__ q, long long qstride) {
+                                                    Fields Fq) {
   constexpr int SPAN = 32 / NF;
   static_assert(SPAN % 2 == 0 && SPAN * NF * 2 == 64, "64 partials per lane");
   __shared__ float red[2][4][64];
@@ -237,7 +245,8 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
   const float2* __restrict__ ab = T.ab + base;
   const int nthr = nwaves * 64;
 
-  float ct[K], y0[K], y1[K];
+  float2 ct[K];
+  float y0[K], y1[K];
   int ny[K];
   float2 wp[K][NF], wm[K][NF];
 #pragma unroll
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
       const Seed sd = T.seed[(long long)m * npair + j];
       y0[k] = sd.y;
       ny[k] = sd.ny;
-      y1[k] = (m < lmax) ? ab[1].x * ct[k] * y0[k] : 0.0f;
+      y1[k] = (m < lmax) ? act_fma(ab[1].x, y0[k], ct[k], 0.0f) : 0.0f;
       const float w = T.w[j];
       const int js = nlat - 1 - j;
 #pragma unroll
@@ -266,7 +275,7 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
         }
       }
     } else {
-      ct[k] = 0.f;
+      ct[k] = make_float2(0.f, 0.f);
       y0[k] = 0.f;
       y1[k] = 0.f;
       ny[k] = 0;
@@ -301,7 +310,8 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
       const int f = rem >> 1;
       const int l = ls + p;
       if (l <= lmax && f0 + f < nfield) {
-        float* dst = reinterpret_cast<float*>(q + (long long)(f0 + f) * qstride + base + (l - m));
+        float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
+                     2 * (base + (l - m));
         dst[rem & 1] = s;
       }
     }
@@ -310,62 +320,61 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
 }
 
 template <int NF>
-hipError_t leg_synth_nf(const LegTables& T, int nfield, const float2* q, long long qstride,
-                        float2* fm, hipStream_t s) {
+hipError_t leg_synth_nf(const LegTables& T, int nfield, const Fields& q, float2* fm,
+                        hipStream_t s) {
   const int nchunk = (T.npair + 63) / 64;
   const int nwork = (T.mmax + 1) * nchunk;
   dim3 grid((nwork + 3) / 4, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL(k_leg_synth<NF>, grid, dim3(256), 0, s, T, nfield, nchunk, nwork, q,
-                     qstride, fm);
+  hipLaunchKernelGGL(k_leg_synth<NF>, grid, dim3(256), 0, s, T, nfield, nchunk, nwork, q, fm);
   return hipGetLastError();
 }
 
 template <int NF, int K>
 hipError_t leg_analys_nfk(const LegTables& T, int nfield, int nwaves, const float2* fm,
-                          float2* q, long long qstride, hipStream_t s) {
+                          const Fields& q, hipStream_t s) {
   dim3 grid(T.mmax + 1, (nfield + NF - 1) / NF);
   hipLaunchKernelGGL((k_leg_analys<NF, K>), grid, dim3(64 * nwaves), 0, s, T, nfield, nwaves,
-                     fm, q, qstride);
+                     fm, q);
   return hipGetLastError();
 }
 
 template <int NF>
-hipError_t leg_analys_nf(const LegTables& T, int nfield, const float2* fm, float2* q,
-                         long long qstride, hipStream_t s) {
+hipError_t leg_analys_nf(const LegTables& T, int nfield, const float2* fm, const Fields& q,
+                         hipStream_t s) {
   const int nwaves = std::min(4, (T.npair + 63) / 64);
   const int K = (T.npair + 64 * nwaves - 1) / (64 * nwaves);
   switch (K) {
-    case 1: return leg_analys_nfk<NF, 1>(T, nfield, nwaves, fm, q, qstride, s);
-    case 2: return leg_analys_nfk<NF, 2>(T, nfield, nwaves, fm, q, qstride, s);
-    case 3: return leg_analys_nfk<NF, 3>(T, nfield, nwaves, fm, q, qstride, s);
-    case 4: return leg_analys_nfk<NF, 4>(T, nfield, nwaves, fm, q, qstride, s);
+    case 1: return leg_analys_nfk<NF, 1>(T, nfield, nwaves, fm, q, s);
+    case 2: return leg_analys_nfk<NF, 2>(T, nfield, nwaves, fm, q, s);
+    case 3: return leg_analys_nfk<NF, 3>(T, nfield, nwaves, fm, q, s);
+    case 4: return leg_analys_nfk<NF, 4>(T, nfield, nwaves, fm, q, s);
     case 5:
-    case 6: return leg_analys_nfk<NF, 6>(T, nfield, nwaves, fm, q, qstride, s);
+    case 6: return leg_analys_nfk<NF, 6>(T, nfield, nwaves, fm, q, s);
     case 7:
-    case 8: return leg_analys_nfk<NF, 8>(T, nfield, nwaves, fm, q, qstride, s);
+    case 8: return leg_analys_nfk<NF, 8>(T, nfield, nwaves, fm, q, s);
     default: return hipErrorInvalidValue;  // nlat > 4096: rejected at plan creation
   }
 }
 
 }  // namespace
 
-hipError_t launch_leg_synth(const LegTables& T, int nfield, const float2* q, long long qstride,
-                            float2* fm, hipStream_t s) {
+hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, float2* fm,
+                            hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   switch (nfield) {
-    case 1: return leg_synth_nf<1>(T, nfield, q, qstride, fm, s);
-    case 2: return leg_synth_nf<2>(T, nfield, q, qstride, fm, s);
-    case 3: return leg_synth_nf<3>(T, nfield, q, qstride, fm, s);
-    default: return leg_synth_nf<4>(T, nfield, q, qstride, fm, s);
+    case 1: return leg_synth_nf<1>(T, nfield, q, fm, s);
+    case 2: return leg_synth_nf<2>(T, nfield, q, fm, s);
+    case 3: return leg_synth_nf<3>(T, nfield, q, fm, s);
+    default: return leg_synth_nf<4>(T, nfield, q, fm, s);
   }
 }
 
-hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, float2* q,
-                             long long qstride, hipStream_t s) {
+hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
+                             hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  if (nfield == 1) return leg_analys_nf<1>(T, nfield, fm, q, qstride, s);
-  if (nfield == 2) return leg_analys_nf<2>(T, nfield, fm, q, qstride, s);
-  return leg_analys_nf<4>(T, nfield, fm, q, qstride, s);
+  if (nfield == 1) return leg_analys_nf<1>(T, nfield, fm, q, s);
+  if (nfield == 2) return leg_analys_nf<2>(T, nfield, fm, q, s);
+  return leg_analys_nf<4>(T, nfield, fm, q, s);
 }
 
 }  // namespace tsphere

@@ -30,7 +30,8 @@ constexpr float kRescaleThr = kAccuracy * kScale + 1.0f;  // ~73.06
 // Device tables of one plan (all f32 except the seed exponent).
 struct LegTables {
   const float2* ab;      // [nlm + pad]: (a_l^m, b_l^m), m-major like Q
-  const float* ct;       // [npair]: cos(theta_j), northern latitude of pair j
+  const float2* ct;      // [npair]: cos(theta_j) of the northern latitude of pair j,
+                         //   as an unevaluated f32 sum (hi, lo) of the f64 node
   const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
   const Seed* seed;      // [(mmax+1) * npair]
   int lmax, mmax, nlat, npair;
@@ -42,7 +43,7 @@ struct Plan {
   int npair = 0;
   long long nlm = 0;
   float2* d_ab = nullptr;
-  float* d_ct = nullptr;
+  float2* d_ct = nullptr;
   float* d_w = nullptr;
   Seed* d_seed = nullptr;
   float2* d_trig = nullptr;
@@ -54,11 +55,28 @@ struct Plan {
   }
 };
 
+// Where the fields of a batched call live: either up to kMaxList explicit
+// pointers (stride == 0) or base p[0] plus f * stride bytes.  Passed by value
+// in the kernel arguments; `f` is wave-uniform, so the lookup is scalar.
+constexpr int kMaxList = 8;
+struct Fields {
+  const char* p[kMaxList];
+  long long stride;  // bytes; 0 = use the pointer list
+  __host__ __device__ const char* at(int f) const { return stride ? p[0] + f * stride : p[f]; }
+};
+inline Fields strided(const void* base, long long stride_bytes) {
+  Fields F{};
+  F.p[0] = static_cast<const char*>(base);
+  F.stride = stride_bytes;
+  return F;
+}
+
 // Host-side table construction (no device needed).
 struct HostTables {
   std::vector<float> ab;       // 2*(nlm + pad)
   std::vector<float> amm;      // mmax+1
   std::vector<float> ct, st, w;  // nlat each
+  std::vector<float> ct_lo;      // nlat: f32(x - f32(x)), the low half of cos(theta)
   std::vector<Seed> seed;      // (mmax+1)*npair
   std::vector<float> trig;     // 2*nphi
   std::vector<double> x, wg;   // f64 Gauss nodes / weights
@@ -68,20 +86,17 @@ void build_host_tables(const tsphere_sht_config& cfg, HostTables& t);
 long long nlm_calc(int lmax, int mmax);
 
 // Kernel launchers (sht_legendre.hip, sht_fourier.hip).  All enqueue on `s`.
-hipError_t launch_leg_synth(const LegTables& T, int nfield, const float2* q, long long qstride,
-                            float2* fm, hipStream_t s);
-hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, float2* q,
-                             long long qstride, hipStream_t s);
-hipError_t launch_leg_synth_mfma(const LegTables& T, int nfield, const float2* q,
-                                 long long qstride, float2* fm, hipStream_t s);
-hipError_t launch_leg_analys_mfma(const LegTables& T, int nfield, const float2* fm, float2* q,
-                                  long long qstride, hipStream_t s);
+// Spectral fields (Q, 2*nlm floats each) and spatial fields (nlat*nphi
+// floats each) are addressed through Fields; the Fourier scratch fm is the
+// plan's, contiguous per field.
+hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, float2* fm,
+                            hipStream_t s);
+hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
+                             hipStream_t s);
 hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                int nfield, const float2* fm, float* spat, long long sstride,
-                                hipStream_t s);
+                                int nfield, const float2* fm, const Fields& spat, hipStream_t s);
 hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                 int nfield, const float* spat, long long sstride, float2* fm,
-                                 hipStream_t s);
+                                 int nfield, const Fields& spat, float2* fm, hipStream_t s);
 hipError_t launch_fill(float* dst, float v, long long n, hipStream_t s);
 
 }  // namespace tsphere

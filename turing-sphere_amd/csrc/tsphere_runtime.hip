@@ -140,8 +140,10 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
   t.ct.resize(nlat);
   t.st.resize(nlat);
   t.w.resize(nlat);
+  t.ct_lo.resize(nlat);
   for (int i = 0; i < nlat; i++) {
     t.ct[i] = (float)t.x[i];
+    t.ct_lo[i] = (float)(t.x[i] - (double)t.ct[i]);
     t.st[i] = (float)std::sqrt(1.0 - t.x[i] * t.x[i]);
     t.w[i] = (float)(t.wg[i] * ((2.0 * M_PI) / nphi));
   }
@@ -399,7 +401,9 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   const size_t nfm = (size_t)cfg->max_fields * (cfg->mmax + 1) * cfg->nlat;
   if (e == hipSuccess) e = upload(&p->d_ab, reinterpret_cast<const float2*>(t.ab.data()),
                                   (size_t)(p->nlm + kAbPad));
-  if (e == hipSuccess) e = upload(&p->d_ct, t.ct.data(), (size_t)p->npair);
+  std::vector<float2> ct2(p->npair);
+  for (int j = 0; j < p->npair; j++) ct2[j] = make_float2(t.ct[j], t.ct_lo[j]);
+  if (e == hipSuccess) e = upload(&p->d_ct, ct2.data(), (size_t)p->npair);
   if (e == hipSuccess) e = upload(&p->d_w, t.w.data(), (size_t)cfg->nlat);
   if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
   if (e == hipSuccess)
@@ -410,7 +414,7 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     delete p;
     return hip_fail(e, "tsphere_plan_create");
   }
-  p->bytes = sizeof(float2) * (p->nlm + kAbPad) + sizeof(float) * (p->npair + cfg->nlat) +
+  p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + sizeof(float) * cfg->nlat +
              sizeof(Seed) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm;
   *out = p;
   return ok();
@@ -458,31 +462,64 @@ static int check_batch(tsphere_plan p, int nfield, const void* a, const void* b)
   return TSPHERE_OK;
 }
 
-static hipError_t leg_synth(Plan* p, int nfield, const float* qlm, long long qstride, float2* fm,
-                            hipStream_t s) {
-  return launch_leg_synth(p->tables(), nfield, reinterpret_cast<const float2*>(qlm), qstride / 2,
-                          fm, s);
+static int check_stride(long long stride, long long per_field, int nfield, const char* what) {
+  if (nfield > 1 && stride < per_field)
+    return fail(TSPHERE_ERR_INVALID, "%s=%lld smaller than one field (%lld floats)", what, stride,
+                per_field);
+  if (stride < 0) return fail(TSPHERE_ERR_INVALID, "%s must be >= 0", what);
+  return TSPHERE_OK;
 }
 
-static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, float* qlm, long long qstride,
-                             hipStream_t s) {
-  return launch_leg_analys(p->tables(), nfield, fm, reinterpret_cast<float2*>(qlm), qstride / 2,
-                           s);
+static int check_list(tsphere_plan p, int nfield, const void* const* a, const void* const* b) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1 || nfield > p->cfg.max_fields || nfield > kMaxList)
+    return fail(TSPHERE_ERR_INVALID, "nfield=%d outside [1, min(max_fields=%d, %d)]", nfield,
+                p->cfg.max_fields, kMaxList);
+  if (!a || !b) return fail(TSPHERE_ERR_INVALID, "NULL pointer list");
+  for (int f = 0; f < nfield; f++)
+    if (!a[f] || !b[f]) return fail(TSPHERE_ERR_INVALID, "NULL buffer for field %d", f);
+  return TSPHERE_OK;
+}
+
+static Fields listed(const void* const* v, int n) {
+  Fields F{};
+  for (int f = 0; f < n; f++) F.p[f] = static_cast<const char*>(v[f]);
+  F.stride = 0;
+  return F;
+}
+
+static int run_synth(Plan* p, int nfield, const Fields& q, const Fields& spat, hipStream_t s) {
+  TS_HIP(launch_leg_synth(p->tables(), nfield, q, p->d_fm, s), "leg_synth");
+  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                              nfield, p->d_fm, spat, s),
+         "fourier_synth");
+  return ok();
+}
+
+static int run_analys(Plan* p, int nfield, const Fields& spat, const Fields& q, hipStream_t s) {
+  TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                               nfield, spat, p->d_fm, s),
+         "fourier_analys");
+  TS_HIP(launch_leg_analys(p->tables(), nfield, p->d_fm, q, s), "leg_analys");
+  return ok();
 }
 
 int tsphere_leg_synth(tsphere_plan p, int nfield, const float* qlm, long long qlm_stride,
                       float* fm, tsphere_stream s) {
   if (int r = check_batch(p, nfield, qlm, fm)) return r;
-  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
-  TS_HIP(leg_synth(p, nfield, qlm, qlm_stride, reinterpret_cast<float2*>(fm), S(s)), "leg_synth");
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
+  TS_HIP(launch_leg_synth(p->tables(), nfield, strided(qlm, 4 * qlm_stride),
+                          reinterpret_cast<float2*>(fm), S(s)),
+         "leg_synth");
   return ok();
 }
 
 int tsphere_leg_analys(tsphere_plan p, int nfield, const float* fm, float* qlm,
                        long long qlm_stride, tsphere_stream s) {
   if (int r = check_batch(p, nfield, fm, qlm)) return r;
-  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
-  TS_HIP(leg_analys(p, nfield, reinterpret_cast<const float2*>(fm), qlm, qlm_stride, S(s)),
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
+  TS_HIP(launch_leg_analys(p->tables(), nfield, reinterpret_cast<const float2*>(fm),
+                           strided(qlm, 4 * qlm_stride), S(s)),
          "leg_analys");
   return ok();
 }
@@ -490,9 +527,12 @@ int tsphere_leg_analys(tsphere_plan p, int nfield, const float* fm, float* qlm,
 int tsphere_fourier_synth(tsphere_plan p, int nfield, const float* fm, float* spat,
                           long long spat_stride, tsphere_stream s) {
   if (int r = check_batch(p, nfield, fm, spat)) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, nfield,
+                           "spat_stride"))
+    return r;
   TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
-                              nfield, reinterpret_cast<const float2*>(fm), spat, spat_stride,
-                              S(s)),
+                              nfield, reinterpret_cast<const float2*>(fm),
+                              strided(spat, 4 * spat_stride), S(s)),
          "fourier_synth");
   return ok();
 }
@@ -500,8 +540,12 @@ int tsphere_fourier_synth(tsphere_plan p, int nfield, const float* fm, float* sp
 int tsphere_fourier_analys(tsphere_plan p, int nfield, const float* spat, long long spat_stride,
                            float* fm, tsphere_stream s) {
   if (int r = check_batch(p, nfield, spat, fm)) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, nfield,
+                           "spat_stride"))
+    return r;
   TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
-                               nfield, spat, spat_stride, reinterpret_cast<float2*>(fm), S(s)),
+                               nfield, strided(spat, 4 * spat_stride),
+                               reinterpret_cast<float2*>(fm), S(s)),
          "fourier_analys");
   return ok();
 }
@@ -509,23 +553,37 @@ int tsphere_fourier_analys(tsphere_plan p, int nfield, const float* spat, long l
 int tsphere_synth_batch(tsphere_plan p, int nfield, const float* qlm, long long qlm_stride,
                         float* spat, long long spat_stride, tsphere_stream s) {
   if (int r = check_batch(p, nfield, qlm, spat)) return r;
-  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
-  TS_HIP(leg_synth(p, nfield, qlm, qlm_stride, p->d_fm, S(s)), "leg_synth");
-  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
-                              nfield, p->d_fm, spat, spat_stride, S(s)),
-         "fourier_synth");
-  return ok();
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, nfield,
+                           "spat_stride"))
+    return r;
+  return run_synth(p, nfield, strided(qlm, 4 * qlm_stride), strided(spat, 4 * spat_stride),
+                   S(s));
 }
 
 int tsphere_analys_batch(tsphere_plan p, int nfield, const float* spat, long long spat_stride,
                          float* qlm, long long qlm_stride, tsphere_stream s) {
   if (int r = check_batch(p, nfield, spat, qlm)) return r;
-  if (qlm_stride & 1) return fail(TSPHERE_ERR_INVALID, "qlm_stride must be even (complex)");
-  TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
-                               nfield, spat, spat_stride, p->d_fm, S(s)),
-         "fourier_analys");
-  TS_HIP(leg_analys(p, nfield, p->d_fm, qlm, qlm_stride, S(s)), "leg_analys");
-  return ok();
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, nfield,
+                           "spat_stride"))
+    return r;
+  return run_analys(p, nfield, strided(spat, 4 * spat_stride), strided(qlm, 4 * qlm_stride),
+                    S(s));
+}
+
+int tsphere_synth_multi(tsphere_plan p, int nfield, const float* const* qlm, float* const* spat,
+                        tsphere_stream s) {
+  if (int r = check_list(p, nfield, (const void* const*)qlm, (const void* const*)spat)) return r;
+  return run_synth(p, nfield, listed((const void* const*)qlm, nfield),
+                   listed((const void* const*)spat, nfield), S(s));
+}
+
+int tsphere_analys_multi(tsphere_plan p, int nfield, const float* const* spat, float* const* qlm,
+                         tsphere_stream s) {
+  if (int r = check_list(p, nfield, (const void* const*)spat, (const void* const*)qlm)) return r;
+  return run_analys(p, nfield, listed((const void* const*)spat, nfield),
+                    listed((const void* const*)qlm, nfield), S(s));
 }
 
 int tsphere_synth(tsphere_plan p, const float* qlm, float* spat, tsphere_stream s) {

@@ -1,0 +1,367 @@
+"""IR expression tree -> one HIP kernel for gfx950 (the counterpart of
+``buildKernel`` in src/mgpu/wgsl.ts:321-372).
+
+For an ``Assign`` whose right-hand side is element-wise over operands of the
+target's shape, emit one kernel computing ``out[i] = expr`` -- the same
+contract, operator table, refusals and messages as the reference's WGSL
+emitter, with these MI355X-specific choices:
+
+* runtime scalars are kernel arguments (``p<slot>``) rather than a params
+  buffer, so a parameter change only re-records the launch (never
+  recompiles); exact scalars fold to f32 literals; scalars the model derives
+  from parameters are ``const float`` prologue lines (wgsl.ts:107-127);
+* each thread handles four consecutive elements with 16-byte loads and one
+  16-byte store (HBM-bound kernels), falling back to scalar accesses for the
+  tail or an unaligned buffer;
+* the kernel signature is ``(long long n, float* out, const float* in0..,
+  float p0..)``, which ``tsphere_kernel_launch`` packs;
+* element-wise math calls the HIP device library (``expf``, ``sinf``...);
+* hipRTC compiles with ``-ffp-contract=off``, so every f32 operation rounds
+  separately, in the expression tree's order.
+"""
+from __future__ import annotations
+
+import math
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .ir import (Assign, Binary, Call, IRExpr, NumLit, NumericType, Unary, Var, is_multi_element,
+                 is_numeric, is_tensor)
+
+
+class UnsupportedOnGpu(Exception):
+    """wgsl.ts:18-28: a construct the GPU backend cannot express, with the
+    source span of the offending node."""
+
+    def __init__(self, message: str, span=None):
+        super().__init__(message)
+        self.span = span
+
+
+# wgsl.ts:33-43
+BINARY_OPS = {"plus": "+", "minus": "-", "times": "*", "rdivide": "/", "mtimes": "*",
+              "mrdivide": "/"}
+# wgsl.ts:46
+UNARY_OPS = {"uminus": "-", "uplus": "+"}
+# wgsl.ts:49-71 -> HIP device math (float overloads)
+CALL_FNS = {
+    "abs": "fabsf", "acos": "acosf", "asin": "asinf", "atan": "atanf", "atan2": "atan2f",
+    "ceil": "ceilf", "cos": "cosf", "cosh": "coshf", "exp": "expf", "floor": "floorf",
+    "log": "logf", "log2": "log2f", "max": "fmaxf", "min": "fminf",
+    # WGSL round() rounds half to even, and the reference maps MATLAB round onto it
+    "round": "rintf",
+    "sign": "ts_sign", "sin": "sinf", "sinh": "sinhf", "sqrt": "sqrtf", "tan": "tanf",
+    "tanh": "tanhf",
+}
+# Builtins numbl marks element-wise (getBuiltin(..).elementwise, wgsl.ts:196-197).
+ELEMENTWISE_BUILTINS = set(CALL_FNS)
+
+
+def f32_lit(v: float) -> str:
+    """An f32 literal holding exactly the f32 rounding of v (wgsl.ts:75-84).
+    The shortest repr of the double, parsed as float: the same decimal the
+    reference hands its WGSL compiler."""
+    if not math.isfinite(v):
+        raise UnsupportedOnGpu(f"cannot emit non-finite literal {v}")
+    if float(v).is_integer() and abs(v) < 1e21:
+        return f"{int(v)}.0f"
+    r = repr(float(v))
+    return f"{r}f" if ("e" in r or "." in r) else f"{r}.0f"
+
+
+@dataclass
+class KernelInputs:
+    """wgsl.ts:86-96: how operands are read inside the kernel."""
+    tensors: dict            # c_name -> input slot (in0, in1, ...)
+    params: dict             # c_name -> param slot (p0, p1, ...), runtime scalars
+    scalars: dict            # c_name -> (name, expr): scalars derived in the .m
+
+
+@dataclass
+class _Ctx:
+    io: KernelInputs
+    prologue: list = field(default_factory=list)
+    bound: dict = field(default_factory=dict)
+
+
+def _scalar_ident(c_name: str) -> str:
+    return "s_" + re.sub(r"[^A-Za-z0-9_]", "_", c_name)
+
+
+def _bind_scalar(c_name: str, ctx: _Ctx) -> str:
+    """wgsl.ts:116-127: bind a derived scalar once, after its dependencies."""
+    if c_name in ctx.bound:
+        return ctx.bound[c_name]
+    _, expr = ctx.io.scalars[c_name]
+    ident = _scalar_ident(c_name)
+    ctx.bound[c_name] = ident
+    rhs = _emit(expr, ctx)
+    ctx.prologue.append(f"  const float {ident} = {rhs};")
+    return ident
+
+
+def _emit(e: IRExpr, ctx: _Ctx) -> str:
+    """wgsl.ts:133-217"""
+    io = ctx.io
+    k = e.kind
+    if k == "NumLit":
+        return f32_lit(e.value)
+    if k == "Var":
+        if is_tensor(e.ty):
+            slot = io.tensors.get(e.c_name)
+            if slot is None:
+                raise UnsupportedOnGpu(f"no buffer bound for '{e.name}'", e.span)
+            return f"a{slot}"
+        if is_numeric(e.ty) and isinstance(e.ty.exact, (int, float)):
+            return f32_lit(float(e.ty.exact))
+        if e.c_name in io.params:
+            return f"p{io.params[e.c_name]}"
+        if e.c_name in io.scalars:
+            return _bind_scalar(e.c_name, ctx)
+        raise UnsupportedOnGpu(
+            f"scalar '{e.name}' is not a constant, a parameter, or computed in this model",
+            e.span)
+    if k == "Binary":
+        if e.builtin in ("mtimes", "mrdivide") and is_tensor(e.left.ty) and is_tensor(e.right.ty):
+            op = "*" if e.builtin == "mtimes" else "/"
+            raise UnsupportedOnGpu(
+                f"matrix '{op}' is not supported; use the element-wise form ('.{op}')", e.span)
+        if e.builtin in ("power", "mpower"):
+            return _emit_power(e.left, e.right, ctx, e.span)
+        op = BINARY_OPS.get(e.builtin)
+        if op is None:
+            raise UnsupportedOnGpu(f"operator '{e.builtin}' is not supported", e.span)
+        return f"({_emit(e.left, ctx)} {op} {_emit(e.right, ctx)})"
+    if k == "Unary":
+        op = UNARY_OPS.get(e.builtin)
+        if op is None:
+            raise UnsupportedOnGpu(f"unary '{e.builtin}' is not supported", e.span)
+        return f"({op}{_emit(e.operand, ctx)})"
+    if k == "Call":
+        if e.name == "ones":
+            return "1.0f"
+        if e.name == "zeros":
+            return "0.0f"
+        fn = CALL_FNS.get(e.name)
+        if fn is None or e.name not in ELEMENTWISE_BUILTINS:
+            is_user = e.c_name != e.name
+            raise UnsupportedOnGpu(
+                (f"'{e.name}' is a function defined in this model. Only init and step are "
+                 f"compiled — inline its body into the caller.") if is_user else
+                f"'{e.name}' cannot be evaluated element-wise on the GPU", e.span)
+        return f"{fn}({', '.join(_emit(a, ctx) for a in e.args)})"
+    raise UnsupportedOnGpu(f"'{k}' is not supported on the GPU", getattr(e, "span", None))
+
+
+def _emit_power(base: IRExpr, exponent: IRExpr, ctx: _Ctx, span) -> str:
+    """wgsl.ts:224-246: small integer exponents only, as repeated multiply."""
+    kval = None
+    if exponent.kind == "NumLit":
+        kval = exponent.value
+    elif is_numeric(exponent.ty) and isinstance(exponent.ty.exact, (int, float)):
+        kval = float(exponent.ty.exact)
+    b = _emit(base, ctx)
+    if kval is not None and float(kval).is_integer() and 0 <= kval <= 8:
+        kk = int(kval)
+        if kk == 0:
+            return "1.0f"
+        return f"pow_i{kk}({b})"
+    if kval is not None and float(kval).is_integer() and -8 <= kval < 0:
+        return f"(1.0f / pow_i{-int(kval)}({b}))"
+    what = "a runtime value" if kval is None else repr(kval)
+    raise UnsupportedOnGpu(
+        f"'.^' needs a literal integer exponent in [-8, 8] (got {what}); a negative base "
+        f"makes a real pow() undefined", span)
+
+
+def _pow_helpers(used) -> str:
+    """wgsl.ts:249-257: x * x * ... (left-associated)."""
+    out = []
+    for k in sorted(used):
+        body = "x" if k == 1 else "x" + " * x" * (k - 1)
+        out.append(f"__device__ __forceinline__ float pow_i{k}(float x) {{ return {body}; }}")
+    return "\n".join(out)
+
+
+def check_shapes(e: IRExpr, target: NumericType, name: str) -> None:
+    """wgsl.ts:268-304: reject implicit expansion."""
+    want = target.shape
+
+    def same(t):
+        return want is not None and t.shape is not None and tuple(t.shape) == tuple(want)
+
+    def walk(x):
+        if is_numeric(x.ty) and is_multi_element(x.ty) and not same(x.ty):
+            got = "x".join(map(str, x.ty.shape)) if x.ty.shape else "dynamic"
+            res = "x".join(map(str, want)) if want else "dynamic"
+            raise UnsupportedOnGpu(
+                f"'{name}' would need implicit expansion: an operand is {got} but the result "
+                f"is {res}. Expand it explicitly (the GPU kernel walks one index across every "
+                f"operand).", x.span)
+        if x.kind == "Binary":
+            walk(x.left)
+            walk(x.right)
+        elif x.kind == "Unary":
+            walk(x.operand)
+        elif x.kind == "Call" and x.name not in ("ones", "zeros"):
+            for a in x.args:
+                walk(a)
+
+    walk(e)
+
+
+WORKGROUP_SIZE = 256
+ELEMS_PER_THREAD = 4
+
+_SIGN = ("__device__ __forceinline__ float ts_sign(float x) "
+         "{ return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }")
+
+
+@dataclass
+class Kernel:
+    """wgsl.ts:308-313, plus what the launch needs."""
+    code: str        # the kernel's definition (part of a module's source)
+    name: str        # extern "C" symbol
+    count: int       # output elements
+    label: str
+    ninputs: int     # tensor inputs, in slot order
+    nparams: int     # scalar arguments p0.. (the params map's size)
+    body: str = ""   # the per-element expression (for tests / describe)
+
+
+def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: str) -> Kernel:
+    """buildKernel (wgsl.ts:321-372): one element-wise Assign -> one kernel."""
+    if not is_numeric(stmt.ty):
+        raise UnsupportedOnGpu(f"'{stmt.name}' is not a numeric array", stmt.span)
+    if stmt.ty.is_complex:
+        raise UnsupportedOnGpu(
+            f"'{stmt.name}' is complex; the GPU backend is real-only (a spectral field is "
+            f"carried as a real 2 x nlm array)", stmt.span)
+    check_shapes(stmt.expr, stmt.ty, stmt.name)
+    ctx = _Ctx(io)
+    body = _emit(stmt.expr, ctx)
+    emitted = "\n".join(ctx.prologue + [body])
+    used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", emitted)}
+    nin = len(io.tensors)
+    npar = len(io.params)
+    args = ["a" + str(i) for i in range(nin)]
+    params = ["float p" + str(i) for i in range(npar)]
+    elem_params = ", ".join([f"float {a}" for a in args] + params) or "void"
+    call_params = ", ".join(args + [f"p{i}" for i in range(npar)])
+    kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
+    kpar = "".join(f", float p{i}" for i in range(npar))
+    pro = "\n".join("  " + line.strip() for line in ctx.prologue)
+    vec_loads = "\n".join(f"    const float4 v{i} = *reinterpret_cast<const float4*>(in{i} + i0);"
+                          for i in range(nin))
+
+    def lane(c):
+        return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)])
+
+    aligned = " && ".join(["ts_al(out)"] + [f"ts_al(in{i})" for i in range(nin)])
+    scalar_args = ", ".join([f"in{i}[i]" for i in range(nin)] + [f"p{i}" for i in range(npar)])
+    helpers = _pow_helpers(used)
+    code = f"""{helpers}
+// {label}
+__device__ __forceinline__ float {name}_elem({elem_params}) {{
+{pro}
+  return {body};
+}}
+extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}{kpar}) {{
+  const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
+  if (i0 >= n) return;
+  if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
+{vec_loads}
+    float4 r;
+    r.x = {name}_elem({lane('x')});
+    r.y = {name}_elem({lane('y')});
+    r.z = {name}_elem({lane('z')});
+    r.w = {name}_elem({lane('w')});
+    *reinterpret_cast<float4*>(out + i0) = r;
+  }} else {{
+    for (long long i = i0; i < n && i < i0 + {ELEMS_PER_THREAD}; ++i) out[i] = {name}_elem({scalar_args});
+  }}
+}}
+"""
+    return Kernel(code=code, name=name, count=count, label=label, ninputs=nin, nparams=npar,
+                  body=body)
+
+
+MODULE_PRELUDE = f"""// generated element-wise kernels (tsphere.hipgen)
+// (hipRTC provides the HIP device headers implicitly)
+__device__ __forceinline__ bool ts_al(const void* p) {{ return (reinterpret_cast<unsigned long long>(p) & 15ull) == 0ull; }}
+{_SIGN}
+"""
+
+
+def module_source(kernels) -> str:
+    """One hipRTC translation unit for all kernels of a model function: a
+    model compiles with one hipRTC invocation."""
+    seen = set()
+    parts = [MODULE_PRELUDE]
+    for k in kernels:
+        code = k.code
+        # pow helpers are shared: emit each once per module
+        lines = []
+        for line in code.splitlines():
+            m = re.match(r"__device__ __forceinline__ float (pow_i\d+)\(", line)
+            if m:
+                if m.group(1) in seen:
+                    continue
+                seen.add(m.group(1))
+            lines.append(line)
+        parts.append("\n".join(lines))
+    return "\n".join(parts)
+
+
+# --------------------------------------------------------- f32 evaluation
+def eval_f32(e: IRExpr, env: dict, params: dict, scalars: dict | None = None):
+    """Evaluate an element-wise IR tree with numpy float32, one rounding per
+    operation in tree order -- what the generated kernel computes (it is
+    compiled without contraction).  Used by tests as the expected output."""
+    f = np.float32
+    scalars = scalars or {}
+
+    def ev(x):
+        k = x.kind
+        if k == "NumLit":
+            return f(x.value)
+        if k == "Var":
+            if is_tensor(x.ty):
+                return np.asarray(env[x.c_name], np.float32)
+            if is_numeric(x.ty) and isinstance(x.ty.exact, (int, float)):
+                return f(x.ty.exact)
+            if x.c_name in params:
+                return f(params[x.c_name])
+            return ev(scalars[x.c_name][1])
+        if k == "Binary":
+            if x.builtin in ("power", "mpower"):
+                kk = int(x.right.value if x.right.kind == "NumLit" else x.right.ty.exact)
+                b = ev(x.left)
+                if kk == 0:
+                    return f(1)
+                r = b
+                for _ in range(abs(kk) - 1):
+                    r = (r * b).astype(np.float32) if isinstance(r, np.ndarray) else f(r * b)
+                return (f(1) / r) if kk < 0 else r
+            a, b = ev(x.left), ev(x.right)
+            op = BINARY_OPS[x.builtin]
+            with np.errstate(all="ignore"):
+                r = {"+": np.add, "-": np.subtract, "*": np.multiply, "/": np.divide}[op](a, b)
+            return np.float32(r) if np.isscalar(r) else r.astype(np.float32)
+        if k == "Unary":
+            v = ev(x.operand)
+            return -v if x.builtin == "uminus" else v
+        if k == "Call":
+            if x.name == "ones":
+                return f(1)
+            if x.name == "zeros":
+                return f(0)
+            fn = {"abs": np.abs, "sqrt": np.sqrt, "floor": np.floor, "ceil": np.ceil,
+                  "round": np.rint, "sign": np.sign, "max": np.maximum, "min": np.minimum}
+            args = [ev(a) for a in x.args]
+            return fn[x.name](*args).astype(np.float32)
+        raise ValueError(k)
+
+    return ev(e)

@@ -1,0 +1,487 @@
+"""Statement list -> a replayable sequence of GPU operations
+(src/mgpu/plan.ts:1-534, re-designed for one HIP stream and HIP graphs).
+
+As in the reference, everything expensive happens once at plan time:
+kernel compilation (one hipRTC module per function), buffer allocation, and
+the op list.  A step is then pure launch recording -- and here it is recorded
+once into a HIP graph and replayed, instead of re-encoded every step
+(plan.ts:436-478).
+
+Differences from the WebGPU planner, all consequences of the hardware:
+
+* transform batching: ready ``synth`` (or ``analys``) ops with no dependency
+  between them are scheduled together and run as ONE batched transform
+  (``tsphere_synth_multi``), so the Legendre recurrence -- the dominant cost
+  -- is generated once for all species.  Element-wise kernels are scheduled
+  as early as their inputs allow so that transforms meet in batches
+  (Schnakenberg: 2 synths -> 1 launch pair, 2 analyses -> 1);
+* in-place kernels need no scratch + copy-back (plan.ts:371-405): HIP has no
+  binding-aliasing rule and an element-wise kernel reads each element before
+  writing it, in the same thread;
+* feedback without copies: when the value that feeds a state buffer
+  (``Un -> U``) is produced by a kernel after the last read of ``U``, that
+  kernel writes ``U`` directly and the copy op (plan.ts:257-281) disappears.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceBuffer, Stream
+from .hipgen import KernelInputs, UnsupportedOnGpu, build_kernel, module_source
+from .ir import Assign, CompiledFunction, is_numeric, is_tensor, numel, walk
+
+EXTERNAL_OPS = ("synth", "analys")  # externals.ts:95
+
+
+class Placeholder:
+    """A buffer's size without device memory: planning on a machine without a
+    GPU (the fusion and scheduling tests) allocates these instead."""
+
+    def __init__(self, count: int, label: str = ""):
+        self.count, self.label, self.ptr, self.nbytes = int(count), label, None, 4 * int(count)
+
+    def free(self) -> None:
+        pass
+
+
+class HostBuffers:
+    """Buffers for host-bound values, shared by a model's plans (plan.ts:48-92).
+    ``ensure_group`` allocates several same-sized buffers in one allocation
+    (the spectral state), which keeps them adjacent in HBM."""
+
+    def __init__(self, allocate: bool = True):
+        self.allocate = allocate
+        self._slots: dict = {}
+        self._arenas: list = []
+
+    def ensure(self, name: str, count: int) -> DeviceBuffer:
+        b = self._slots.get(name)
+        if b is not None:
+            if b.count != count:
+                raise UnsupportedOnGpu(
+                    f"'{name}' is {b.count} elements in one program and {count} in another")
+            return b
+        b = DeviceBuffer(count, f"mgpu-{name}") if self.allocate else Placeholder(count, name)
+        self._slots[name] = b
+        return b
+
+    def ensure_group(self, names, count: int) -> None:
+        new = [n for n in names if n not in self._slots]
+        if not new:
+            return
+        if not self.allocate:
+            for n in new:
+                self._slots[n] = Placeholder(count, n)
+            return
+        arena = DeviceBuffer(count * len(new), "mgpu-" + "+".join(new))
+        self._arenas.append(arena)
+        for i, n in enumerate(new):
+            self._slots[n] = DeviceBuffer.view(arena, i * count, count, f"mgpu-{n}")
+
+    def get(self, name: str):
+        return self._slots.get(name)
+
+    def names(self):
+        return list(self._slots)
+
+    def upload(self, name: str, data, stream: Stream | None = None) -> None:
+        b = self._slots.get(name)
+        if b is None:
+            raise KeyError(f"upload: no buffer named '{name}'")
+        a = np.asarray(data, np.float32).reshape(-1)
+        if a.size != b.count:
+            raise ValueError(f"upload '{name}': expected {b.count} elements, got {a.size}")
+        b.upload(a, stream)
+
+    def destroy(self) -> None:
+        for b in self._slots.values():
+            b.free()
+        for a in self._arenas:
+            a.free()
+        self._slots.clear()
+        self._arenas.clear()
+
+
+@dataclass
+class PlanSpec:
+    """plan.ts:108-114"""
+    fn: CompiledFunction
+    feedback: list  # output index -> host binding name (or None)
+
+
+# ------------------------------------------------------------------- ops
+@dataclass
+class KernelOp:
+    kind: str
+    label: str
+    kernel: object          # hipgen.Kernel
+    out: str                # c_name of the destination
+    ins: list               # c_names, slot order
+    handle: object = None   # tsphere_kernel
+    count: int = 0
+
+
+@dataclass
+class XformOp:
+    kind: str               # 'synth' | 'analys'
+    label: str
+    srcs: list              # c_names
+    dsts: list
+
+
+@dataclass
+class CopyOp:
+    kind: str
+    label: str
+    src: str
+    dst_host: str
+    dst_keys: tuple = ()    # c_names the host buffer is read under in this function
+
+
+def _reads(op) -> set:
+    if isinstance(op, KernelOp):
+        return set(op.ins)
+    if isinstance(op, XformOp):
+        return set(op.srcs)
+    return {op.src}
+
+
+def _writes(op) -> set:
+    if isinstance(op, KernelOp):
+        return {op.out}
+    if isinstance(op, XformOp):
+        return set(op.dsts)
+    return {"@host:" + op.dst_host, *op.dst_keys}
+
+
+class ModelPlan:
+    """A compiled .m function, ready to run on the GPU (plan.ts:173)."""
+
+    def __init__(self):
+        self.param_names: list[str] = []
+        self.ops: list = []
+        self._bufs: dict[str, DeviceBuffer] = {}     # c_name -> buffer
+        self._by_name: dict[str, DeviceBuffer] = {}  # source name -> buffer
+        self._owned: list[DeviceBuffer] = []
+        self._module = None
+        self._kernels: list = []
+        self._params = np.zeros(1, np.float32)
+        self._sht = None
+        self._graph = None
+        self._graph_steps = 0
+        self._ptr_cache = {}
+
+    # --------------------------------------------------------- creation
+    @staticmethod
+    def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
+               label: str = "step") -> "ModelPlan":
+        """ModelPlan.create (plan.ts:206-407)."""
+        self = ModelPlan()
+        self._sht = sht
+        fn = spec.fn
+        host_of: dict[str, str] = {}       # c_name -> host name, for host-bound args
+        param_slots: dict[str, int] = {}
+        for p in fn.params:
+            if p.binding.kind == "tensor":
+                count = int(np.prod(p.binding.shape))
+                buf = host.ensure(p.name, count)
+                self._bufs[p.c_name] = buf
+                self._by_name[p.name] = buf
+                host_of[p.c_name] = p.name
+            elif p.binding.kind == "param":
+                param_slots[p.c_name] = len(self.param_names)
+                self.param_names.append(p.name)
+            # 'const' arguments are exact in the IR and fold into the kernels
+        self._params = np.zeros(max(1, len(self.param_names)), np.float32)
+
+        derived: dict[str, tuple] = {}
+        counts: dict[str, int] = {}
+        names: dict[str, str] = {}
+        pending_kernels = []
+        ops = []
+        for stmt in fn.body:
+            if stmt.kind == "ReturnFromFunction":
+                continue
+            if stmt.kind != "Assign":
+                raise UnsupportedOnGpu(
+                    f"a model function body may only contain assignments (found '{stmt.kind}')",
+                    stmt.span)
+            if not is_numeric(stmt.ty):
+                raise UnsupportedOnGpu(f"'{stmt.name}' is not a numeric value", stmt.span)
+            if not is_tensor(stmt.ty):
+                derived[stmt.c_name] = (stmt.name, stmt.expr)
+                continue
+            count = numel(stmt.ty)
+            if stmt.c_name in counts and counts[stmt.c_name] != count:
+                raise UnsupportedOnGpu(f"'{stmt.name}' changes size between assignments",
+                                       stmt.span)
+            if stmt.c_name in self._bufs and self._bufs[stmt.c_name].count != count:
+                raise UnsupportedOnGpu(f"'{stmt.name}' changes size between assignments",
+                                       stmt.span)
+            counts[stmt.c_name] = count
+            names[stmt.c_name] = stmt.name
+            ext = _external_call(stmt)
+            if ext is not None:
+                fname, arg_c, arg_name = ext
+                if arg_c not in self._bufs and arg_c not in counts:
+                    raise UnsupportedOnGpu(f"'{fname}' reads '{arg_name}', which has no buffer",
+                                           stmt.span)
+                ops.append(XformOp(fname, f"{stmt.name} = {fname}({arg_name})", [arg_c],
+                                   [stmt.c_name]))
+                continue
+            tensors: dict[str, int] = {}
+            for node in walk(stmt.expr):
+                if node.kind == "Var" and is_tensor(node.ty) and node.c_name not in tensors:
+                    if node.c_name not in self._bufs and node.c_name not in counts:
+                        raise UnsupportedOnGpu(f"'{stmt.name}' reads a value with no buffer",
+                                               stmt.span)
+                    tensors[node.c_name] = len(tensors)
+            kname = f"k{len(pending_kernels)}_{_ident(stmt.name)}"
+            lbl = f"{stmt.name} = <{count} elements, element-wise>"
+            kern = build_kernel(stmt, KernelInputs(tensors, param_slots, derived), count, lbl,
+                                kname)
+            op = KernelOp("kernel", lbl, kern, stmt.c_name, list(tensors), count=count)
+            pending_kernels.append(op)
+            ops.append(op)
+
+        # Feedback (plan.ts:257-281): declared outputs -> the host buffers they replace.
+        for i, out in enumerate(fn.outputs):
+            to = spec.feedback[i] if i < len(spec.feedback) else None
+            if not to:
+                continue
+            if out.c_name not in counts and out.c_name not in self._bufs:
+                raise UnsupportedOnGpu(
+                    f"'{fn.name}' declares the output '{out.name}' but never assigns it")
+            dst = host.get(to)
+            if dst is None:
+                raise UnsupportedOnGpu(f"'{to}' is not a host binding")
+            n_src = counts.get(out.c_name, self._bufs[out.c_name].count if out.c_name in self._bufs else 0)
+            if n_src != dst.count:
+                raise UnsupportedOnGpu(
+                    f"'{out.name}' ({n_src} elements) cannot feed '{to}' ({dst.count})")
+            if self._bufs.get(out.c_name) is dst:
+                continue  # the function updated the host buffer in place
+            keys = tuple(c for c, n in host_of.items() if n == to)
+            ops.append(CopyOp("copy", f"{out.name} -> {to}", out.c_name, to, keys))
+
+        if batch:
+            ops = _schedule(ops, max_batch=min(8, sht.max_fields) if sht is not None else 8)
+        ops = self._fold_feedback(ops, host, host_of)
+
+        # Buffers for every value the .m computes (reused across steps).
+        for c, n in counts.items():
+            if c not in self._bufs:
+                b = (DeviceBuffer(n, f"mgpu-{names[c]}") if host.allocate
+                     else Placeholder(n, names[c]))
+                self._owned.append(b)
+                self._bufs[c] = b
+            self._by_name[names[c]] = self._bufs[c]
+        self.ops = ops
+
+        if pending_kernels and not host.allocate:
+            src = module_source([op.kernel for op in pending_kernels])
+            if _lib.load().tsphere_rtc_check(src.encode(), None) != _lib.OK:
+                raise UnsupportedOnGpu(f"generated HIP failed to compile for '{label}':\n"
+                                       f"{_lib.last_error()}\n--- source ---\n{src}")
+        elif pending_kernels:
+            src = module_source([op.kernel for op in pending_kernels])
+            self._module = _compile(src, label)
+            for op in pending_kernels:
+                h = C.c_void_p()
+                _lib.call("tsphere_module_kernel", self._module, op.kernel.name.encode(),
+                          C.byref(h))
+                op.handle = h.value
+                self._kernels.append(h.value)
+        self._host = host
+        return self
+
+    def _fold_feedback(self, ops, host: HostBuffers, host_of: dict) -> list:
+        """Let the kernel that produces a fed-back value write the host buffer
+        directly when nothing after it reads that buffer's old contents."""
+        out = list(ops)
+        for cp in [o for o in ops if isinstance(o, CopyOp)]:
+            idx = [i for i, o in enumerate(out) if cp.src in _writes(o) and o is not cp]
+            if len(idx) != 1 or cp.src in host_of:
+                continue
+            k = idx[0]
+            prod = out[k]
+            keys = set(cp.dst_keys)
+            # a transform cannot read and write one buffer; an element-wise
+            # kernel may (each element is read before it is written)
+            if isinstance(prod, XformOp) and keys & _reads(prod):
+                continue
+            # nothing after the producer may still want the old contents
+            if any(keys & _reads(o) for o in out[k + 1:] if o is not cp):
+                continue
+            if any(isinstance(o, CopyOp) and o.dst_host == cp.dst_host and o is not cp
+                   for o in out):
+                continue
+            # the producer writes the host buffer; its c_name now names that buffer
+            self._bufs[cp.src] = host.get(cp.dst_host)
+            out.remove(cp)
+        return out
+
+    # --------------------------------------------------------- params
+    def set_params(self, values: dict) -> None:
+        """plan.ts:410-420.  Scalars are launch arguments: a change only
+        invalidates the recorded graph; nothing recompiles."""
+        for i, name in enumerate(self.param_names):
+            v = values.get(name)
+            self._params[i] = v if (v is not None and np.isfinite(v)) else 0.0
+        self.invalidate()
+
+    def invalidate(self) -> None:
+        if self._graph is not None:
+            _lib.call("tsphere_graph_destroy", self._graph)
+        self._graph = None
+        self._graph_steps = 0
+
+    def buffer(self, name: str):
+        return self._by_name.get(name)
+
+    def element_count(self, name: str):
+        b = self._by_name.get(name)
+        return None if b is None else b.count
+
+    # --------------------------------------------------------- recording
+    def _ptrs(self, cnames):
+        key = tuple(cnames)
+        arr = self._ptr_cache.get(key)
+        if arr is None:
+            arr = (C.c_void_p * len(cnames))(*[self._bufs[c].ptr for c in cnames])
+            self._ptr_cache[key] = arr
+        return arr
+
+    def encode_steps(self, stream: Stream, steps: int = 1) -> None:
+        """plan.ts:436-478: enqueue `steps` timesteps on `stream` (no sync)."""
+        L = _lib.load()
+        sh = stream.handle
+        pvals = self._params.ctypes.data_as(C.POINTER(C.c_float))
+        np_ = len(self.param_names)
+        for _ in range(steps):
+            for op in self.ops:
+                if isinstance(op, KernelOp):
+                    ptrs = self._ptrs([op.out] + op.ins)
+                    _lib.check(L.tsphere_kernel_launch(op.handle, op.count, len(ptrs), ptrs, np_,
+                                                       pvals, sh), op.label)
+                elif isinstance(op, XformOp):
+                    srcs = self._ptrs(op.srcs)
+                    dsts = self._ptrs(op.dsts)
+                    fn = L.tsphere_synth_multi if op.kind == "synth" else L.tsphere_analys_multi
+                    _lib.check(fn(self._sht.handle, len(op.srcs), srcs, dsts, sh), op.label)
+                else:
+                    src = self._bufs[op.src]
+                    dst = self._host.get(op.dst_host)
+                    _lib.check(L.tsphere_memcpy_dtod(dst.ptr, src.ptr, src.nbytes, sh), op.label)
+
+    def run(self, stream: Stream, steps: int = 1, graph_steps: int = 1) -> None:
+        """Replay `steps` timesteps through a HIP graph holding `graph_steps`
+        of them (captured on first use or after a parameter change)."""
+        if steps <= 0:
+            return
+        if self._graph is None or self._graph_steps != graph_steps:
+            self.invalidate()
+            _lib.call("tsphere_graph_begin", stream.handle)
+            try:
+                self.encode_steps(stream, graph_steps)
+            finally:
+                g = C.c_void_p()
+                rc = _lib.load().tsphere_graph_end(stream.handle, C.byref(g))
+            _lib.check(rc, "graph capture")
+            self._graph = g.value
+            self._graph_steps = graph_steps
+        full, rest = divmod(steps, graph_steps)
+        for _ in range(full):
+            _lib.call("tsphere_graph_launch", self._graph, stream.handle)
+        if rest:
+            self.encode_steps(stream, rest)
+
+    def describe(self) -> list[str]:
+        """plan.ts:486-488: '<kind> <label>' per op (the fusion guard counts
+        'kernel' lines)."""
+        return [f"{op.kind:<7} {op.label}" for op in self.ops]
+
+    def kernel_source(self) -> str:
+        return module_source([op.kernel for op in self.ops if isinstance(op, KernelOp)])
+
+    def destroy(self) -> None:
+        self.invalidate()
+        for h in self._kernels:
+            _lib.call("tsphere_kernel_destroy", h)
+        self._kernels.clear()
+        if self._module:
+            _lib.call("tsphere_module_destroy", self._module)
+            self._module = None
+        for b in self._owned:
+            b.free()
+        self._owned.clear()
+
+
+def _ident(name: str) -> str:
+    return "".join(ch if ch.isalnum() else "_" for ch in name)
+
+
+def _compile(src: str, label: str):
+    h = C.c_void_p()
+    rc = _lib.load().tsphere_module_compile(src.encode(), None, C.byref(h))
+    if rc != _lib.OK:
+        raise UnsupportedOnGpu(f"generated HIP failed to compile for '{label}':\n"
+                               f"{_lib.last_error()}\n--- source ---\n{src}")
+    return h.value
+
+
+def _external_call(stmt: Assign):
+    """plan.ts:498-511: `x = synth(y)` / `x = analys(y)`."""
+    e = stmt.expr
+    if e.kind != "Call" or e.name not in EXTERNAL_OPS:
+        return None
+    if len(e.args) != 1 or e.args[0].kind != "Var":
+        raise UnsupportedOnGpu(f"'{e.name}' must be applied to a single variable", stmt.span)
+    a = e.args[0]
+    return e.name, a.c_name, a.name
+
+
+def _schedule(ops: list, max_batch: int) -> list:
+    """List scheduling over the dependency DAG (read-after-write,
+    write-after-read, write-after-write between ops): element-wise kernels run
+    as soon as they are ready (program order among them), and when only
+    transforms are ready, every ready transform of the earliest one's kind
+    (up to max_batch) merges into one batched op."""
+    n = len(ops)
+    preds = [set() for _ in range(n)]
+    for j in range(n):
+        rj, wj = _reads(ops[j]), _writes(ops[j])
+        for i in range(j):
+            ri, wi = _reads(ops[i]), _writes(ops[i])
+            if (wi & rj) or (ri & wj) or (wi & wj):
+                preds[j].add(i)
+    done: set = set()
+    out = []
+    while len(done) < n:
+        ready = [i for i in range(n) if i not in done and preds[i] <= done]
+        kern = [i for i in ready if isinstance(ops[i], KernelOp)]
+        if kern:
+            i = kern[0]
+            out.append(ops[i])
+            done.add(i)
+            continue
+        xf = [i for i in ready if isinstance(ops[i], XformOp)]
+        if xf:
+            kind = ops[xf[0]].kind
+            group = [i for i in xf if ops[i].kind == kind][:max_batch]
+            if len(group) == 1:
+                out.append(ops[group[0]])
+            else:
+                out.append(XformOp(kind, " ; ".join(ops[i].label for i in group),
+                                   [s for i in group for s in ops[i].srcs],
+                                   [d for i in group for d in ops[i].dsts]))
+            done.update(group)
+            continue
+        i = ready[0]
+        out.append(ops[i])
+        done.add(i)
+    return out
