@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Benchmark: IMEX steps/s + SHT GFLOP/s at lmax = 1023 on one MI355X.
+
+Workload (the configuration the metric is quoted on): Brusselator, 2 species,
+lmax 1023 on the Gauss grid 1024 x 2048, FP32 state, A=4.5, B=6.96,
+D1=2e-4, D2=8*D1, dt=0.002; initial state (A, B/A) + uniform(-0.01, 0.01)
+noise (mulberry32 seed 0, u's draws then v's), analysed to spectral on the
+GPU.  A step is the full IMEX Euler step compiled from the Brusselator .m
+file: 2 syntheses + 2 analyses (batched as one of each), 5 generated
+element-wise kernels, replayed from a HIP graph.  Inputs are resident in HBM
+before the timed region; the timed region is `--steps` steps bracketed by
+device synchronisation.
+
+Printed: ONE JSON line with the metric, a `roofline` object for the dominant
+kernel (algorithmic flops per launch / its mean duration measured with HIP
+events on the stream it runs on) and a `cpu_baseline` object (the f64 oracle
+-- the reference's direct-summation algorithm -- timed on a bounded sample of
+the same step on this host's cores, extrapolated to steps/s).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legendre fp32|bf16x3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "turing-sphere_amd"))
+
+LMAX, NLAT, NPHI = 1023, 1024, 2048
+PARAMS = {"A": 4.5, "B": 6.96, "D1": 2e-4, "D2": 8 * 2e-4, "dt": 0.002}
+PEAK_FP32 = 157.3e12   # MI355X FP32 vector (packed) TFLOP/s, datasheet
+PEAK_BF16 = 2.5e15     # dense BF16 MFMA
+PEAK_HBM = 8.0e12
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def nlm_calc(lmax, mmax):
+    return (mmax + 1) * (lmax + 1) - (mmax * (mmax + 1)) // 2
+
+
+def flops_per_transform(lmax, nlat, nphi):
+    """Algorithmic flops of one transform of one field: 2*nlm*nlat for the
+    Legendre contraction (4 flops per complex x real MAC per latitude pair)
+    + 2.5 nlat nphi log2 nphi for the FFT."""
+    nlm = nlm_calc(lmax, lmax)
+    return 2.0 * nlm * nlat, 2.5 * nlat * nphi * np.log2(nphi)
+
+
+def initial_grid(npts, seed=0):
+    """mulberry32 uniform deviates (src/mgpu/noise.ts:15-24) -> 0.02 r - 0.01."""
+    out = np.empty(2 * npts)
+    s = seed & 0xFFFFFFFF
+    # vectorised mulberry32: the state sequence is s_k = seed + k * 0x6d2b79f5
+    k = np.arange(1, 2 * npts + 1, dtype=np.uint64)
+    t = ((np.uint64(s) + k * np.uint64(0x6D2B79F5)) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    with np.errstate(over="ignore"):
+        t = (t ^ (t >> np.uint32(15))) * (t | np.uint32(1))
+        t ^= t + (t ^ (t >> np.uint32(7))) * (t | np.uint32(61))
+        out[:] = (t ^ (t >> np.uint32(14))).astype(np.float64) / 4294967296.0
+    r = 0.02 * out - 0.01
+    return r[:npts], r[npts:]
+
+
+def time_stage(fn, stream, reps):
+    """Mean duration (ms) of `fn()` launches, HIP events on `stream`."""
+    from tsphere import Event
+
+    fn()
+    stream.sync()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    return e0.elapsed_ms(e1) / reps
+
+
+def cpu_baseline(rows: int, nfield: int = 2):
+    """The oracle (f64 direct summation, reference.ts:37-118) on `rows`
+    latitude rows of one synthesis and one analysis per field, on all host
+    threads; every row is an independent slice of the transform, so the full
+    step costs nlat/rows times the sample."""
+    import oracle as O
+
+    cfg = O.Cfg(LMAX, LMAX, NLAT, NPHI)
+    ref = O.ShtReference(cfg)
+    rng = np.random.default_rng(0)
+    q = rng.normal(size=2 * cfg.nlm) / 100
+    spat = rng.normal(size=NLAT * NPHI)
+    threads = O.get_threads()
+    t0 = time.perf_counter()
+    for _ in range(nfield):
+        ref.synth(q, rows=(0, rows))
+        gm = ref.fourier_analys(spat, rows=(0, rows))
+        ref.legendre_analys(gm, rows=(0, rows))
+    dt = time.perf_counter() - t0
+    step_s = dt * (NLAT / rows)
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": (f"f64 direct-summation synth+analys of {nfield} fields on {rows} of "
+                       f"{NLAT} latitude rows ({dt:.1f} s on {threads} threads), scaled by "
+                       f"{NLAT / rows:.0f} to one IMEX step; reaction/IMEX element-wise "
+                       f"work (<0.1% of it) not included")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--legendre", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+              f"step on a single MI355X -- see DESIGN.md for how it would shard.",
+              file=sys.stderr)
+        return 2
+
+    import tsphere
+    from tsphere import ShtConfig, ShtPlan
+    from tsphere.model import GpuModel
+    import ctypes as C
+    from tsphere import _lib
+
+    cfg = ShtConfig(LMAX, LMAX, NLAT, NPHI)
+    npts, nlm = cfg.npts, cfg.nlm
+    log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={args.legendre}")
+    sht = ShtPlan.create(cfg, legendre=args.legendre, max_fields=2)
+    src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
+    gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"])
+    gm.set_params(PARAMS)
+    gm.graph_steps = args.graph_steps
+    ru, rv = initial_grid(npts, 0)
+    u0 = (PARAMS["A"] + ru).astype(np.float32)
+    v0 = (PARAMS["B"] / PARAMS["A"] + rv).astype(np.float32)
+    U0 = sht.analys(u0)
+    V0 = sht.analys(v0)
+    gm.upload("U", U0)
+    gm.upload("V", V0)
+    log(f"[bench] step ops: {gm.describe()['step']}")
+
+    # warmup (also captures the graph)
+    gm.step(args.warmup)
+    gm.sync()
+    log(f"[bench] warmup {args.warmup} done")
+    t0 = time.perf_counter()
+    gm.step(args.steps)
+    gm.sync()
+    elapsed = time.perf_counter() - t0
+    steps_s = args.steps / elapsed
+    U = gm.read("U")
+    finite = bool(np.all(np.isfinite(U)))
+    log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms -> {steps_s:.1f} steps/s")
+
+    # per-kernel durations (HIP events on the model's stream)
+    L = _lib.load()
+    s = gm.stream
+    Ub = gm._host.get("U")
+    fm = tsphere.DeviceBuffer(2 * 2 * (LMAX + 1) * NLAT, "bench-fm")
+    qo = tsphere.DeviceBuffer(2 * 2 * nlm, "bench-q")
+    sp = tsphere.DeviceBuffer(2 * npts, "bench-spat")
+    h = sht.handle
+    reps = 20
+    kms = {
+        "leg_synth": time_stage(lambda: _lib.check(L.tsphere_leg_synth(
+            h, 2, Ub.ptr, 2 * nlm, fm.ptr, s.handle)), s, reps),
+        "fft_synth": time_stage(lambda: _lib.check(L.tsphere_fourier_synth(
+            h, 2, fm.ptr, sp.ptr, npts, s.handle)), s, reps),
+        "fft_analys": time_stage(lambda: _lib.check(L.tsphere_fourier_analys(
+            h, 2, sp.ptr, npts, fm.ptr, s.handle)), s, reps),
+        "leg_analys": time_stage(lambda: _lib.check(L.tsphere_leg_analys(
+            h, 2, fm.ptr, qo.ptr, 2 * nlm, s.handle)), s, reps),
+    }
+    log(f"[bench] kernel ms: {kms}")
+    f_leg, f_fft = flops_per_transform(LMAX, NLAT, NPHI)
+    dom = max(("leg_synth", "leg_analys"), key=lambda k: kms[k])
+    achieved = 2 * f_leg / (kms[dom] * 1e-3)  # 2 fields per launch
+    peak = PEAK_BF16 if args.legendre == "bf16x3" else PEAK_FP32
+    sht_flops_step = 4 * (f_leg + f_fft)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"{dom}_c2_pmc.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline on {args.cpu_rows} rows ...")
+        cpu = cpu_baseline(args.cpu_rows)
+
+    line = {
+        "metric": "IMEX steps/s + SHT GFLOP/s at lmax=1023 (split-BF16 MFMA vs plain FP32)",
+        "value": round(steps_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.legendre == "fp32" else "bf16x3",
+        "data": "synthetic",
+        "config": {"workload": "brusselator IMEX Euler, lmax 1023, Gauss 1024x2048, 2 species, "
+                               "FP32 state, dt 0.002", "model": "brusselator", "lmax": LMAX,
+                   "nlat": NLAT, "nphi": NPHI, "legendre": args.legendre,
+                   "parallelism": "single GPU"},
+        "sht_gflops": round(sht_flops_step * steps_s / 1e9, 2),
+        "transforms_per_s": round(4 * steps_s, 2),
+        "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
+        "finite": finite,
+        "roofline": {"bound": "mfma" if args.legendre == "bf16x3" else "valu",
+                     "kernel": dom, "achieved": round(achieved / 1e12, 3),
+                     "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "flops_per_launch": 2 * f_leg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    gm.destroy()
+    sht.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
