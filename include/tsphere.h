@@ -117,10 +117,12 @@ int tsphere_plan_grid(tsphere_plan p, double *theta, double *cos_theta, double *
 /* The plan's f32 device tables, computed on the host without a device (for
  * inspection and tests).  Any pointer may be NULL.  Sizes: ab 2*nlm,
  * amm mmax+1, ctstw 3*nlat ([cos | sin | w*2pi/nphi], sht.ts:194-199),
- * seed_y/seed_ny (mmax+1)*ceil(nlat/2) (ytilde_m^m at latitude pair j as
- * seed_y * 2^(56*seed_ny), seed_ny <= 0), trig 2*nphi. */
+ * seed_y0/seed_y1/seed_l (mmax+1)*ceil(nlat/2): for m and latitude pair j,
+ * the recurrence starts at l = seed_l (the first l where |ytilde_l^m| >
+ * 1e-15, rounded down to m + 4k; INT_MAX if none up to lmax) from the f64
+ * values ytilde_l = seed_y0, ytilde_{l+1} = seed_y1; trig 2*nphi. */
 int tsphere_plan_tables(const tsphere_sht_config *cfg, float *ab, float *amm, float *ctstw,
-                        float *seed_y, int *seed_ny, float *trig);
+                        float *seed_y0, float *seed_y1, int *seed_l, float *trig);
 
 /* synth / analys on caller-owned device buffers (createSynthBinding +
  * encodeSynthInto, sht.ts:299-338; createAnalysBinding + encodeAnalysInto,

@@ -94,32 +94,42 @@ def test_tables_match_oracle(lmax, nlat, nphi):
     assert np.array_equal(t["trig"], trig.astype(np.float32).reshape(-1))
 
 
-def test_seed_table_is_scaled_sin_power():
-    """seed_y * 2^(56 seed_ny) == amm[m] sin(theta_j)^m (f64 log domain), with
-    ny < 0 exactly when the value is below the significance threshold."""
+def test_significance_seeds_match_oracle_rows():
+    """For every (m, latitude pair): the seed sits at the first l where the
+    oracle's f64 ytilde_l^m exceeds 1e-15 (coeffs.ts:64-85, the reference's
+    significance rule), rounded down to m + 4k, and holds the oracle's values
+    there; pairs never significant up to lmax have no seed."""
     import tsphere
 
-    lmax, nlat = 1023, 1024
-    cfg = tsphere.ShtConfig(lmax, lmax, nlat, 2048)
-    t = tsphere.plan_tables(cfg)
-    npair = nlat // 2
-    x, _ = O.gauss_nodes_weights(nlat)
-    st = np.sqrt(1 - x[:npair] ** 2)
-    amm, _ = O.legendre_coeffs(lmax, lmax)
-    y = t["seed_y"].reshape(lmax + 1, npair).astype(np.float64)
-    ny = t["seed_ny"].reshape(lmax + 1, npair)
-    m = np.arange(lmax + 1)[:, None]
-    log2_true = np.log2(np.abs(amm))[:, None] + m * np.log2(st)[None, :]
-    log2_got = np.log2(np.abs(y)) + 56.0 * ny
-    assert np.max(np.abs(log2_got - log2_true)) < 1e-6
-    assert np.all(np.sign(y) == np.sign(amm)[:, None])
-    thr = 1e-15 * 2.0 ** 56 + 1
-    assert np.all(ny <= 0)
-    assert np.all(np.abs(y[ny < 0]) <= thr * (1 + 1e-6))
-    # unscaled seeds agree with the direct power where f64 can hold it
-    ok = ny == 0
-    direct = (amm[:, None] * st[None, :] ** m)
-    assert np.allclose(y[ok], direct[ok], rtol=1e-6, atol=0)
+    for lmax, nlat in [(63, 64), (255, 256)]:
+        cfg = tsphere.ShtConfig(lmax, lmax, nlat, 2 * nlat)
+        t = tsphere.plan_tables(cfg)
+        npair = nlat // 2
+        x, _ = O.gauss_nodes_weights(nlat)
+        amm, ab = O.legendre_coeffs(lmax, lmax)
+        y0 = t["seed_y0"].reshape(lmax + 1, npair)
+        y1 = t["seed_y1"].reshape(lmax + 1, npair)
+        sl = t["seed_l"].reshape(lmax + 1, npair)
+        step = 1 if lmax < 100 else 7
+        for m in range(0, lmax + 1, step):
+            for j in range(0, npair, 1 if lmax < 100 else 5):
+                row = O.legendre_row(amm, ab, lmax, m, x[j], np.sqrt(1 - x[j] ** 2))
+                sig = np.flatnonzero(np.abs(row) > 1e-15)
+                if sig.size == 0:
+                    assert sl[m, j] == 2 ** 31 - 1, (m, j)
+                    continue
+                first = m + sig[0]
+                assert sl[m, j] == m + 4 * ((first - m) // 4), (m, j, first, sl[m, j])
+                k = sl[m, j] - m
+                assert np.isclose(y0[m, j], row[k], rtol=2e-7, atol=0), (m, j)
+                if k + 1 <= lmax - m:
+                    assert np.isclose(y1[m, j], row[k + 1], rtol=2e-7, atol=1e-30), (m, j)
+        # the skipped volume: the part of the (l, m, latitude) triangle the
+        # recurrence never has to walk (7% at lmax 63, 20% at 255, 28% at 1023)
+        n_tot = sum((lmax - m + 1) * npair for m in range(lmax + 1))
+        start = np.minimum(sl, lmax + 1) - np.arange(lmax + 1)[:, None]
+        skipped = start.sum() / n_tot
+        assert {63: 0.05, 255: 0.15}[lmax] < skipped < 0.4, skipped
 
 
 def test_rtc_compiles_a_kernel_without_a_device():

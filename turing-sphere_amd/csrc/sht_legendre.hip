@@ -8,47 +8,55 @@
 //
 // The associated Legendre functions are generated on the fly by the 3-term
 // recurrence over l with f32 coefficients (a_l^m, b_l^m) (coeffs.ts:28-56).
-// Both kernels fold north/south latitude pairs onto one recurrence
-// (ytilde_l^m(-x) = (-1)^(l-m) ytilde_l^m(x)); even (l-m) terms accumulate in
-// E and odd ones in O, north = E + O and south = E - O (leg.ts:129-134,
-// 207-219).  Values whose magnitude is still below ACCURACY carry a negative
-// exponent ny and are not accumulated (leg.ts:140-157, common.ts); the seed
-// ytilde_m^m comes from the host f64 table, already scaled.
+// North/south latitude pairs share one recurrence (ytilde_l^m(-x) =
+// (-1)^(l-m) ytilde_l^m(x)): even (l-m) terms accumulate in E and odd ones in
+// O, north = E + O, south = E - O (leg.ts:129-134, 207-219).
 //
-// MI355X mapping:
-//  * synthesis: one 64-lane wave per (m, 64 latitude pairs).  m, l and the
-//    coefficient / spectral addresses are wave-uniform, so (a,b) and Q_lm are
-//    scalar loads shared by the 64 lanes; a lane does 3 VALU ops of
-//    recurrence + 2 FMAs per field per l.  Waves are ordered by m ascending,
-//    i.e. longest l-loop first, which balances the triangle.
-//  * analysis: one workgroup per m, up to 4 waves, K latitude pairs per lane
-//    in registers.  Each lane accumulates a span of l-values (64 partial sums
-//    per lane: span_l * fields * re/im) and the span is reduced across the
-//    wave by a transposing butterfly -- v_permlane32_swap, v_permlane16_swap,
-//    then DPP row_mirror / row_half_mirror / quad_perm -- which leaves lane i
-//    holding the complete sum of partial i after 141 VALU ops for all 64
-//    values (instead of 6 shuffles per value).  Waves combine through LDS,
-//    one barrier per span.
+// Significance seeding.  The reference carries a scaled exponent through the
+// recurrence and skips values below ACCURACY = 1e-15 (common.ts, leg.ts:
+// 140-157).  Here the host walks that part of the recurrence once, in f64,
+// and stores for every (m, latitude pair) the first l (rounded down to a
+// multiple of 4 above m) where the value is significant, with the exact f64
+// values ytilde_l, ytilde_{l+1} there.  A lane starts its f32 recurrence at
+// that point; before it, it contributes exactly 0, as the reference's lane
+// does.  No scaling, no per-lane tests in the loop, and the negligible part
+// of the (l, m, latitude) volume (about 1 - pi/4 of it) is skipped.
+//
+// MI355X mapping.  One workgroup per m-pair (m, mmax - m), whose l-loops
+// together have a constant length lmax + 2, so the triangle is balanced
+// without a tail.  The row's Q_lm (all batched fields) and (a, b) are staged
+// once into LDS with coalesced loads; the inner loop reads them as LDS
+// broadcasts and has no global or scalar loads.  Each lane carries K latitude
+// pairs (K independent dependency chains).
+//  * synthesis: a lane does 4 VALU ops of recurrence (a * cos(theta) with the
+//    f32 hi+lo node) + 2 FMAs per field per l;
+//  * analysis: each lane accumulates a span of l-values (64 partial sums:
+//    span_l * fields * re/im), reduced across the wave by a transposing
+//    butterfly -- v_permlane32_swap, v_permlane16_swap, DPP row_mirror /
+//    row_half_mirror / quad_perm -- after which lane i holds the complete sum
+//    of partial i (141 VALU ops for all 64); waves combine through LDS.
 #include "tsphere_internal.h"
 
 namespace tsphere {
 namespace {
 
+constexpr int kTile = 4;   // l-values between seed injections (seeds are 4-aligned)
+constexpr int kPadQ = 8;   // zero entries after a staged row (reads run past lmax)
+constexpr int kPadAnalys = 40;  // analysis spans read up to 32 + 3 entries past lmax
+
 __device__ __forceinline__ long long lm_base(int lmax, int m) {
   return (long long)m * (lmax + 1) - ((long long)m * (m - 1)) / 2;
 }
 
-// a * cos(theta) * y with cos(theta) = ct.x + ct.y (f32 hi + lo of the f64
-// node), plus c: rounding the node to f32 perturbs theta and, through it,
-// every ytilde_l by ~ l * 2^-24 / sin(theta) -- the dominant fp32 error of the
-// plain recurrence.  Carrying the low half costs one FMA.
+// a * cos(theta) * y + c, with cos(theta) = ct.x + ct.y (f32 hi + lo of the
+// f64 node): rounding the node to f32 would move theta, and every ytilde_l
+// with it, by ~ l 2^-24 / sin(theta).  The low half costs one FMA.
 __device__ __forceinline__ float act_fma(float a, float y, float2 ct, float c) {
   const float u = a * y;
   return fmaf(u, ct.x, fmaf(u, ct.y, c));
 }
 
-// One step of the recurrence: (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}), written
-// as ytilde_l = a_l (cos theta) ytilde_{l-1} + b_l ytilde_{l-2}.
+// (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}):  ytilde_l = a_l cos(theta) ytilde_{l-1} + b_l ytilde_{l-2}
 __device__ __forceinline__ void advance(float2 a0, float2 a1, float2 ct, float& y0, float& y1) {
   const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
   y1 = act_fma(a1.x, t0, ct, a1.y * y1);
@@ -60,97 +68,127 @@ __device__ __forceinline__ void cfma(float2& acc, float y, float2 q) {
   acc.y = fmaf(y, q.y, acc.y);
 }
 
-// ------------------------------------------------------------- synthesis
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+struct Chain {
+  float2 ct;
+  float y0, y1, s0, s1;
+  int ls;  // seed position; INT_MAX = never significant up to lmax
+};
+
+__device__ __forceinline__ void load_chain(const LegTables& T, int m, int j, Chain& c) {
+  if (j < T.npair) {
+    c.ct = T.ct[j];
+    const SeedL sd = T.seed[(long long)m * T.npair + j];
+    c.s0 = sd.y0;
+    c.s1 = sd.y1;
+    c.ls = sd.l;
+  } else {
+    c.ct = make_float2(0.f, 0.f);
+    c.s0 = c.s1 = 0.f;
+    c.ls = 0x7fffffff;
+  }
+  c.y0 = c.y1 = 0.f;
+}
+
+// Stage a row's (a, b) and the fields' Q into LDS, zero-padded.
 template <int NF>
-__global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, int nchunk,
-                                                   int nwork, Fields Fq,
+__device__ __forceinline__ void stage_row(const LegTables& T, int m, int f0, int nfield,
+                                          const Fields& Fq, bool with_q, float2* sab,
+                                          float2* sq, int pad) {
+  const int n = T.lmax - m + 1;
+  const long long base = lm_base(T.lmax, m);
+  for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
+    sab[i] = i < n ? T.ab[base + i] : make_float2(0.f, 0.f);
+    if (with_q) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int fld = min(f0 + f, nfield - 1);
+        sq[i * NF + f] =
+            i < n ? reinterpret_cast<const float2*>(Fq.at(fld))[base + i] : make_float2(0.f, 0.f);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- synthesis
+template <int NF, int K>
+__global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fields Fq,
                                                    float2* __restrict__ fm) {
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wave >= nwork) return;  // whole wave
-  const int m = __builtin_amdgcn_readfirstlane(wave / nchunk);
-  const int chunk = __builtin_amdgcn_readfirstlane(wave - m * nchunk);
+  extern __shared__ float2 lds[];
+  const int L = T.lmax, M = T.mmax;
+  float2* sab = lds;
+  float2* sq = lds + (L + 1 + kPadQ);
   const int f0 = blockIdx.y * NF;
-  const int lmax = T.lmax;
-  const int j = chunk * 64 + lane;
-  const bool active = j < T.npair;
-  const int jc = active ? j : T.npair - 1;
+  const int nthr = blockDim.x;
 
-  const float2 ct = T.ct[jc];
-  const Seed sd = T.seed[(long long)m * T.npair + jc];
-  const long long base = lm_base(lmax, m);
-  const float2* __restrict__ ab = T.ab + base;  // ab[l - m]
-  float y0 = sd.y;
-  int ny = sd.ny;
-  float y1 = (m < lmax) ? act_fma(ab[1].x, y0, ct, 0.0f) : 0.0f;
+  for (int r = 0; r < 2; ++r) {
+    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
+    if (r == 1 && m <= (int)blockIdx.x) break;
+    __syncthreads();  // the previous row's readers are done
+    stage_row<NF>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
+    __syncthreads();
 
-  const float2* qf[NF];
-  float2 accE[NF], accO[NF];
+    Chain c[K];
+    int lmin = 0x7fffffff;
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    const int fld = min(f0 + f, nfield - 1);
-    qf[f] = reinterpret_cast<const float2*>(Fq.at(fld)) + base;
-    accE[f] = make_float2(0.f, 0.f);
-    accO[f] = make_float2(0.f, 0.f);
-  }
+    for (int k = 0; k < K; ++k) {
+      load_chain(T, m, threadIdx.x + k * nthr, c[k]);
+      lmin = min(lmin, c[k].ls);
+    }
+    lmin = wave_min(lmin);
 
-  int l = m;
-  bool done = false;
-  // Phase 1: some lane still carries a scaled (negligible) value: the
-  // reference's per-lane rule, once per l-pair.
-  while (__builtin_amdgcn_ballot_w64(ny != 0) != 0) {
-    if (ny == 0) {
+    float2 accE[K][NF], accO[K][NF];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        cfma(accE[f], y0, qf[f][l - m]);
-        if (l + 1 <= lmax) cfma(accO[f], y1, qf[f][l + 1 - m]);
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) accE[k][f] = accO[k][f] = make_float2(0.f, 0.f);
+
+    for (int l = lmin; l <= L; l += kTile) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c[k].ls == l) {
+          c[k].y0 = c[k].s0;
+          c[k].y1 = c[k].s1;
+        }
+      const int i = l - m;
+#pragma unroll
+      for (int p = 0; p < kTile; p += 2) {
+        float2 q0[NF], q1[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          q0[f] = sq[(i + p) * NF + f];
+          q1[f] = sq[(i + p + 1) * NF + f];
+        }
+        const float2 a0 = sab[i + p + 2];
+        const float2 a1 = sab[i + p + 3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            cfma(accE[k][f], c[k].y0, q0[f]);
+            cfma(accO[k][f], c[k].y1, q1[f]);
+          }
+          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+        }
       }
-    } else if (fabsf(y0) > kRescaleThr) {
-      ny += 1;
-      y0 *= kInvScale;
-      y1 *= kInvScale;
     }
-    if (l + 2 > lmax) {
-      done = true;
-      break;
-    }
-    const float2 a0 = ab[l + 2 - m];
-    const float2 a1 = (l + 3 <= lmax) ? ab[l + 3 - m] : make_float2(0.f, 0.f);
-    advance(a0, a1, ct, y0, y1);
-    l += 2;
-  }
-  if (!done) {
-    // Phase 2: every lane significant -- no per-lane tests.
-#pragma unroll 2
-    for (; l + 3 <= lmax; l += 2) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        cfma(accE[f], y0, qf[f][l - m]);
-        cfma(accO[f], y1, qf[f][l + 1 - m]);
-      }
-      advance(ab[l + 2 - m], ab[l + 3 - m], ct, y0, y1);
-    }
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      if (l <= lmax) cfma(accE[f], y0, qf[f][l - m]);
-      if (l + 1 <= lmax) cfma(accO[f], y1, qf[f][l + 1 - m]);
-    }
-    if (l + 2 <= lmax) {
-      const float2 a0 = ab[l + 2 - m];
-      const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) cfma(accE[f], t0, qf[f][l + 2 - m]);
-    }
-  }
 
-  if (active) {
     const int nlat = T.nlat;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      if (f0 + f < nfield) {
-        float2* out = fm + ((long long)(f0 + f) * (T.mmax + 1) + m) * nlat;
-        out[j] = make_float2(accE[f].x + accO[f].x, accE[f].y + accO[f].y);
-        out[nlat - 1 - j] = make_float2(accE[f].x - accO[f].x, accE[f].y - accO[f].y);
+    for (int k = 0; k < K; ++k) {
+      const int j = threadIdx.x + k * nthr;
+      if (j >= T.npair) continue;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if (f0 + f >= nfield) continue;
+        float2* out = fm + ((long long)(f0 + f) * (M + 1) + m) * nlat;
+        out[j] = make_float2(accE[k][f].x + accO[k][f].x, accE[k][f].y + accO[k][f].y);
+        out[nlat - 1 - j] = make_float2(accE[k][f].x - accO[k][f].x, accE[k][f].y - accO[k][f].y);
       }
     }
   }
@@ -195,78 +233,45 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 }
 
 // -------------------------------------------------------------- analysis
-template <int NF, int K, bool CHECKED>
-__device__ __forceinline__ void analys_span(float (&acc)[64], const float2* __restrict__ ab,
-                                            int ls, int m, int lmax, const float2 (&ct)[K],
-                                            float (&y0)[K], float (&y1)[K], int (&ny)[K],
-                                            const float2 (&wp)[K][NF],
-                                            const float2 (&wm)[K][NF]) {
-  constexpr int SPAN = 32 / NF;
-#pragma unroll
-  for (int p = 0; p < SPAN; p += 2) {
-    const int l = ls + p;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (!CHECKED || ny[k] == 0) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          float* e = &acc[(p * NF + f) * 2];
-          float* o = &acc[((p + 1) * NF + f) * 2];
-          e[0] = fmaf(y0[k], wp[k][f].x, e[0]);
-          e[1] = fmaf(y0[k], wp[k][f].y, e[1]);
-          o[0] = fmaf(y1[k], wm[k][f].x, o[0]);
-          o[1] = fmaf(y1[k], wm[k][f].y, o[1]);
-        }
-      } else if (fabsf(y0[k]) > kRescaleThr) {
-        ny[k] += 1;
-        y0[k] *= kInvScale;
-        y1[k] *= kInvScale;
-      }
-    }
-    const float2 a0 = (l + 2 <= lmax) ? ab[l + 2 - m] : make_float2(0.f, 0.f);
-    const float2 a1 = (l + 3 <= lmax) ? ab[l + 3 - m] : make_float2(0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < K; ++k) advance(a0, a1, ct[k], y0[k], y1[k]);
-  }
-}
-
 template <int NF, int K>
-__global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int nwaves,
-                                                    const float2* __restrict__ fm,
-                                                    Fields Fq) {
-  constexpr int SPAN = 32 / NF;
-  static_assert(SPAN % 2 == 0 && SPAN * NF * 2 == 64, "64 partials per lane");
+__global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
+                                                    const float2* __restrict__ fm, Fields Fq) {
+  constexpr int SPAN = 32 / NF;  // l-values per reduced span
+  static_assert(SPAN % kTile == 0 && SPAN * NF * 2 == 64, "64 partials per lane");
+  extern __shared__ float2 lds[];
   __shared__ float red[2][4][64];
-  const int m = blockIdx.x;
+  __shared__ int smin[4];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  float2* sab = lds;
   const int f0 = blockIdx.y * NF;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int lmax = T.lmax, nlat = T.nlat, npair = T.npair;
-  const long long base = lm_base(lmax, m);
-  const float2* __restrict__ ab = T.ab + base;
-  const int nthr = nwaves * 64;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
 
-  float2 ct[K];
-  float y0[K], y1[K];
-  int ny[K];
-  float2 wp[K][NF], wm[K][NF];
+  for (int r = 0; r < 2; ++r) {
+    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
+    if (r == 1 && m <= (int)blockIdx.x) break;
+    const long long base = lm_base(L, m);
+    __syncthreads();
+    stage_row<NF>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+
+    Chain c[K];
+    float2 wp[K][NF], wm[K][NF];
+    int lmin = 0x7fffffff;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int j = tid + k * nthr;
-    if (j < npair) {
-      ct[k] = T.ct[j];
-      const Seed sd = T.seed[(long long)m * npair + j];
-      y0[k] = sd.y;
-      ny[k] = sd.ny;
-      y1[k] = (m < lmax) ? act_fma(ab[1].x, y0[k], ct[k], 0.0f) : 0.0f;
-      const float w = T.w[j];
-      const int js = nlat - 1 - j;
+    for (int k = 0; k < K; ++k) {
+      const int j = tid + k * nthr;
+      load_chain(T, m, j, c[k]);
+      lmin = min(lmin, c[k].ls);
+      const int jc = min(j, T.npair - 1);
+      const int js = nlat - 1 - jc;
+      const float w = j < T.npair ? T.w[jc] : 0.f;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int fld = min(f0 + f, nfield - 1);
-        const float2* g = fm + ((long long)fld * (T.mmax + 1) + m) * nlat;
-        const float2 gN = g[j];
+        const float2* g = fm + ((long long)fld * (M + 1) + m) * nlat;
+        const float2 gN = g[jc];
         const float2 gS = g[js];
-        if (js == j) {  // equator of an odd grid pairs with itself
+        if (js == jc) {  // equator of an odd grid pairs with itself
           wp[k][f] = make_float2(gN.x * w, gN.y * w);
           wm[k][f] = make_float2(0.f, 0.f);
         } else {
@@ -274,85 +279,124 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield, int
           wm[k][f] = make_float2((gN.x - gS.x) * w, (gN.y - gS.y) * w);
         }
       }
-    } else {
-      ct[k] = make_float2(0.f, 0.f);
-      y0[k] = 0.f;
-      y1[k] = 0.f;
-      ny[k] = 0;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        wp[k][f] = make_float2(0.f, 0.f);
-        wm[k][f] = make_float2(0.f, 0.f);
-      }
     }
-  }
+    lmin = wave_min(lmin);
+    if (lane == 0) smin[wv] = lmin;
+    __syncthreads();  // also publishes the staged row
+    int lblk = smin[0];
+    for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+    lblk = min(lblk, L + 1);
 
-  float acc[64];
-  int buf = 0;
-  for (int ls = m; ls <= lmax; ls += SPAN) {
+    // coefficients below every lane's significance point are exactly 0
+    for (int i = m + tid; i < lblk; i += nthr)
 #pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] = 0.f;
-    bool sc = false;
+      for (int f = 0; f < NF; ++f)
+        if (f0 + f < nfield)
+          reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
+              make_float2(0.f, 0.f);
+
+    int buf = 0;
+    for (int ls = lblk; ls <= L; ls += SPAN) {
+      float acc[64];
 #pragma unroll
-    for (int k = 0; k < K; ++k) sc |= (ny[k] != 0);
-    if (__builtin_amdgcn_ballot_w64(sc) != 0)
-      analys_span<NF, K, true>(acc, ab, ls, m, lmax, ct, y0, y1, ny, wp, wm);
-    else
-      analys_span<NF, K, false>(acc, ab, ls, m, lmax, ct, y0, y1, ny, wp, wm);
-    const float v = transpose_reduce64(acc);
-    red[buf][wv][lane] = v;
-    __syncthreads();
-    if (wv == 0) {
-      float s = red[buf][0][lane];
-      for (int w = 1; w < nwaves; ++w) s += red[buf][w][lane];
-      const int p = lane / (2 * NF);
-      const int rem = lane - p * 2 * NF;
-      const int f = rem >> 1;
-      const int l = ls + p;
-      if (l <= lmax && f0 + f < nfield) {
-        float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
-                     2 * (base + (l - m));
-        dst[rem & 1] = s;
+      for (int i = 0; i < 64; ++i) acc[i] = 0.f;
+#pragma unroll
+      for (int p = 0; p < SPAN; p += 2) {
+        const int l = ls + p;
+        if (p % kTile == 0) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (c[k].ls == l) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            float* e = &acc[(p * NF + f) * 2];
+            float* o = &acc[((p + 1) * NF + f) * 2];
+            e[0] = fmaf(c[k].y0, wp[k][f].x, e[0]);
+            e[1] = fmaf(c[k].y0, wp[k][f].y, e[1]);
+            o[0] = fmaf(c[k].y1, wm[k][f].x, o[0]);
+            o[1] = fmaf(c[k].y1, wm[k][f].y, o[1]);
+          }
+        }
+        const int i = l - m;
+        const float2 a0 = sab[i + 2];
+        const float2 a1 = sab[i + 3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
       }
+      const float v = transpose_reduce64(acc);
+      red[buf][wv][lane] = v;
+      __syncthreads();
+      if (wv == 0) {
+        float s = red[buf][0][lane];
+        for (int w = 1; w < nwaves; ++w) s += red[buf][w][lane];
+        const int p = lane / (2 * NF);
+        const int rem = lane - p * 2 * NF;
+        const int f = rem >> 1;
+        const int l = ls + p;
+        if (l <= L && f0 + f < nfield) {
+          float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
+                       2 * (base + (l - m));
+          dst[rem & 1] = s;
+        }
+      }
+      buf ^= 1;
     }
-    buf ^= 1;
   }
 }
 
-template <int NF>
-hipError_t leg_synth_nf(const LegTables& T, int nfield, const Fields& q, float2* fm,
-                        hipStream_t s) {
-  const int nchunk = (T.npair + 63) / 64;
-  const int nwork = (T.mmax + 1) * nchunk;
-  dim3 grid((nwork + 3) / 4, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL(k_leg_synth<NF>, grid, dim3(256), 0, s, T, nfield, nchunk, nwork, q, fm);
+int pick_k(int npair, int& nwaves) {
+  // up to 4 waves per workgroup; K latitude pairs per lane
+  nwaves = std::min(4, (npair + 63) / 64);
+  int K = (npair + 64 * nwaves - 1) / (64 * nwaves);
+  return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8;
+}
+
+size_t synth_lds(int lmax, int nf) { return sizeof(float2) * (lmax + 1 + kPadQ) * (1 + nf); }
+
+template <int NF, int K>
+hipError_t synth_nfk(const LegTables& T, int nfield, int nwaves, const Fields& q, float2* fm,
+                     hipStream_t s) {
+  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  hipLaunchKernelGGL((k_leg_synth<NF, K>), grid, dim3(64 * nwaves), synth_lds(T.lmax, NF), s, T,
+                     nfield, q, fm);
   return hipGetLastError();
 }
 
 template <int NF, int K>
-hipError_t leg_analys_nfk(const LegTables& T, int nfield, int nwaves, const float2* fm,
-                          const Fields& q, hipStream_t s) {
-  dim3 grid(T.mmax + 1, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL((k_leg_analys<NF, K>), grid, dim3(64 * nwaves), 0, s, T, nfield, nwaves,
-                     fm, q);
+hipError_t analys_nfk(const LegTables& T, int nfield, int nwaves, const float2* fm,
+                      const Fields& q, hipStream_t s) {
+  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  hipLaunchKernelGGL((k_leg_analys<NF, K>), grid, dim3(64 * nwaves),
+                     sizeof(float2) * (T.lmax + 1 + kPadAnalys), s, T, nfield, fm, q);
   return hipGetLastError();
 }
 
 template <int NF>
-hipError_t leg_analys_nf(const LegTables& T, int nfield, const float2* fm, const Fields& q,
-                         hipStream_t s) {
-  const int nwaves = std::min(4, (T.npair + 63) / 64);
-  const int K = (T.npair + 64 * nwaves - 1) / (64 * nwaves);
-  switch (K) {
-    case 1: return leg_analys_nfk<NF, 1>(T, nfield, nwaves, fm, q, s);
-    case 2: return leg_analys_nfk<NF, 2>(T, nfield, nwaves, fm, q, s);
-    case 3: return leg_analys_nfk<NF, 3>(T, nfield, nwaves, fm, q, s);
-    case 4: return leg_analys_nfk<NF, 4>(T, nfield, nwaves, fm, q, s);
-    case 5:
-    case 6: return leg_analys_nfk<NF, 6>(T, nfield, nwaves, fm, q, s);
-    case 7:
-    case 8: return leg_analys_nfk<NF, 8>(T, nfield, nwaves, fm, q, s);
-    default: return hipErrorInvalidValue;  // nlat > 4096: rejected at plan creation
+hipError_t synth_nf(const LegTables& T, int nfield, const Fields& q, float2* fm, hipStream_t s) {
+  int nw;
+  switch (pick_k(T.npair, nw)) {
+    case 1: return synth_nfk<NF, 1>(T, nfield, nw, q, fm, s);
+    case 2: return synth_nfk<NF, 2>(T, nfield, nw, q, fm, s);
+    case 4: return synth_nfk<NF, 4>(T, nfield, nw, q, fm, s);
+    default: return synth_nfk<NF, 8>(T, nfield, nw, q, fm, s);
+  }
+}
+
+template <int NF>
+hipError_t analys_nf(const LegTables& T, int nfield, const float2* fm, const Fields& q,
+                     hipStream_t s) {
+  int nw;
+  switch (pick_k(T.npair, nw)) {
+    case 1: return analys_nfk<NF, 1>(T, nfield, nw, fm, q, s);
+    case 2: return analys_nfk<NF, 2>(T, nfield, nw, fm, q, s);
+    case 4: return analys_nfk<NF, 4>(T, nfield, nw, fm, q, s);
+    default: return analys_nfk<NF, 8>(T, nfield, nw, fm, q, s);
   }
 }
 
@@ -362,19 +406,18 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
                             hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   switch (nfield) {
-    case 1: return leg_synth_nf<1>(T, nfield, q, fm, s);
-    case 2: return leg_synth_nf<2>(T, nfield, q, fm, s);
-    case 3: return leg_synth_nf<3>(T, nfield, q, fm, s);
-    default: return leg_synth_nf<4>(T, nfield, q, fm, s);
+    case 1: return synth_nf<1>(T, nfield, q, fm, s);
+    case 2: return synth_nf<2>(T, nfield, q, fm, s);
+    default: return synth_nf<4>(T, nfield, q, fm, s);
   }
 }
 
 hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
                              hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  if (nfield == 1) return leg_analys_nf<1>(T, nfield, fm, q, s);
-  if (nfield == 2) return leg_analys_nf<2>(T, nfield, fm, q, s);
-  return leg_analys_nf<4>(T, nfield, fm, q, s);
+  if (nfield == 1) return analys_nf<1>(T, nfield, fm, q, s);
+  if (nfield == 2) return analys_nf<2>(T, nfield, fm, q, s);
+  return analys_nf<4>(T, nfield, fm, q, s);
 }
 
 }  // namespace tsphere

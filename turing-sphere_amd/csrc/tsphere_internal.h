@@ -11,11 +11,14 @@
 
 namespace tsphere {
 
-// ytilde_m^m at one latitude pair: value = y * 2^(56*ny), ny <= 0.  The host
-// computes it in f64 in the log domain, so no underflow at any m.
-struct Seed {
-  float y;
-  int ny;
+// Where the f32 recurrence of one (m, latitude pair) starts: the first l at
+// which |ytilde_l^m| exceeds ACCURACY, rounded down to m + 4k, with the f64
+// values ytilde_l and ytilde_{l+1} there (l = INT_MAX: never significant up
+// to lmax).  Built on the host by a scaled f64 recurrence.
+struct SeedL {
+  float y0, y1;
+  int l;
+  int pad;
 };
 
 // fp32 extended-range constants of the recurrence (the SHTNS values the
@@ -33,7 +36,7 @@ struct LegTables {
   const float2* ct;      // [npair]: cos(theta_j) of the northern latitude of pair j,
                          //   as an unevaluated f32 sum (hi, lo) of the f64 node
   const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
-  const Seed* seed;      // [(mmax+1) * npair]
+  const SeedL* seed;     // [(mmax+1) * npair]
   int lmax, mmax, nlat, npair;
 };
 
@@ -45,7 +48,7 @@ struct Plan {
   float2* d_ab = nullptr;
   float2* d_ct = nullptr;
   float* d_w = nullptr;
-  Seed* d_seed = nullptr;
+  SeedL* d_seed = nullptr;
   float2* d_trig = nullptr;
   float2* d_fm = nullptr;  // Fourier scratch, max_fields * (mmax+1) * nlat
   size_t bytes = 0;
@@ -77,7 +80,7 @@ struct HostTables {
   std::vector<float> amm;      // mmax+1
   std::vector<float> ct, st, w;  // nlat each
   std::vector<float> ct_lo;      // nlat: f32(x - f32(x)), the low half of cos(theta)
-  std::vector<Seed> seed;      // (mmax+1)*npair
+  std::vector<SeedL> seed;     // (mmax+1)*npair
   std::vector<float> trig;     // 2*nphi
   std::vector<double> x, wg;   // f64 Gauss nodes / weights
 };

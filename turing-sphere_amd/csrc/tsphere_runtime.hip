@@ -148,22 +148,59 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
     t.w[i] = (float)(t.wg[i] * ((2.0 * M_PI) / nphi));
   }
 
-  // Seed ytilde_m^m(theta_j) = amm[m] sin^m(theta_j), in the log domain:
-  // e = log2|amm| + m log2(sin theta).  A value below ACCURACY is carried as
-  // y * 2^(56 ny) with |y| <= RESCALE_THR, exactly the state the reference's
-  // binary exponentiation (common.ts:27-55) hands the recurrence.
-  const double log2_thr = std::log2((double)kRescaleThr);
+  // Significance seeds.  Walk the recurrence of every (m, latitude pair) in
+  // f64 with an explicit power-of-two exponent (true value = y * 2^E, so
+  // amm[m] sin^m(theta) never underflows) until the first l where
+  // |ytilde_l| > ACCURACY (the reference's significance rule, common.ts:19),
+  // and keep the values at the last l = m + 4k at or before it.
   t.seed.resize((size_t)(mmax + 1) * npair);
+  const double thr = (double)kAccuracy;
   for (int m = 0; m <= mmax; m++) {
     const double la = std::log2(std::fabs(amm[m]));
     const double sg = amm[m] < 0 ? -1.0 : 1.0;
     for (int j = 0; j < npair; j++) {
-      const double st = std::sqrt(1.0 - t.x[j] * t.x[j]);
-      const double e = la + (m == 0 ? 0.0 : m * std::log2(st));
-      int ny = (int)std::ceil((e - log2_thr) / 56.0);
-      if (ny > 0) ny = 0;
-      const double y = sg * std::exp2(e - 56.0 * ny);
-      t.seed[(size_t)m * npair + j] = Seed{(float)y, ny};
+      const double x = t.x[j];
+      const double st = std::sqrt(1.0 - x * x);
+      const double e0 = la + (m == 0 ? 0.0 : m * std::log2(st));
+      int E = (int)std::floor(e0);
+      double y0 = sg * std::exp2(e0 - E);
+      double y1 = (m < lmax) ? std::sqrt(2.0 * m + 3.0) * x * y0 : 0.0;
+      SeedL seed{0.f, 0.f, 0x7fffffff, 0};
+      double c0 = y0, c1 = y1;
+      int cE = E, cl = m;
+      for (int l = m; l <= lmax; l += 2) {
+        if (((l - m) & 3) == 0) {  // candidate seed position
+          c0 = y0;
+          c1 = y1;
+          cE = E;
+          cl = l;
+        }
+        const bool sig0 = std::fabs(y0) * std::exp2((double)E) > thr;
+        const bool sig1 = l + 1 <= lmax && std::fabs(y1) * std::exp2((double)E) > thr;
+        if (sig0 || sig1) {
+          seed.y0 = (float)std::ldexp(c0, cE);
+          seed.y1 = (cl + 1 <= lmax) ? (float)std::ldexp(c1, cE) : 0.0f;
+          seed.l = cl;
+          break;
+        }
+        // two steps of coeffs.ts:28-56's recurrence, in f64
+        for (int s = 0; s < 2; s++) {
+          const int ll = l + 2 + s;
+          if (ll > lmax) break;
+          const double t1 = (double)(ll + m) * (ll - m);
+          const double t2 = (double)(ll - 1 + m) * (ll - 1 - m);
+          const double a = std::sqrt(((2.0 * ll + 1.0) * (2.0 * ll - 1.0)) / t1);
+          const double b = -std::sqrt(((2.0 * ll + 1.0) / (2.0 * ll - 3.0)) * (t2 / t1));
+          const double yn = a * x * (s == 0 ? y1 : y0) + b * (s == 0 ? y0 : y1);
+          if (s == 0) y0 = yn; else y1 = yn;
+        }
+        if (std::fabs(y0) > 0x1p200 || std::fabs(y1) > 0x1p200) {
+          y0 *= 0x1p-200;
+          y1 *= 0x1p-200;
+          E += 200;
+        }
+      }
+      t.seed[(size_t)m * npair + j] = seed;
     }
   }
 
@@ -324,7 +361,7 @@ static int validate(const tsphere_sht_config* c) {
 static bool fft_fits(int nphi) { return is_pow2(nphi) && nphi >= 8 && nphi <= 4096; }
 
 int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, float* ctstw,
-                        float* seed_y, int* seed_ny, float* trig) {
+                        float* seed_y0, float* seed_y1, int* seed_l, float* trig) {
   if (int r = validate(cfg)) return r;
   HostTables t;
   build_host_tables(*cfg, t);
@@ -340,10 +377,11 @@ int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, fl
     }
   }
   const size_t ns = (size_t)(cfg->mmax + 1) * npair;
-  if (seed_y)
-    for (size_t i = 0; i < ns; i++) seed_y[i] = t.seed[i].y;
-  if (seed_ny)
-    for (size_t i = 0; i < ns; i++) seed_ny[i] = t.seed[i].ny;
+  for (size_t i = 0; i < ns; i++) {
+    if (seed_y0) seed_y0[i] = t.seed[i].y0;
+    if (seed_y1) seed_y1[i] = t.seed[i].y1;
+    if (seed_l) seed_l[i] = t.seed[i].l;
+  }
   if (trig) std::memcpy(trig, t.trig.data(), sizeof(float) * 2 * cfg->nphi);
   return ok();
 }
@@ -415,7 +453,7 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     return hip_fail(e, "tsphere_plan_create");
   }
   p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + sizeof(float) * cfg->nlat +
-             sizeof(Seed) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm;
+             sizeof(SeedL) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm;
   *out = p;
   return ok();
 }

@@ -72,7 +72,7 @@ SIGNATURES = {
     "tsphere_plan_destroy": [_vp],
     "tsphere_plan_query": [_vp, C.POINTER(PlanInfoC)],
     "tsphere_plan_grid": [_vp, _vp, _vp, _vp],
-    "tsphere_plan_tables": [C.POINTER(ShtConfigC), _vp, _vp, _vp, _vp, _vp, _vp],
+    "tsphere_plan_tables": [C.POINTER(ShtConfigC), _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tsphere_synth": [_vp, _vp, _vp, _vp],
     "tsphere_analys": [_vp, _vp, _vp, _vp],
     "tsphere_synth_batch": [_vp, _i, _vp, _ll, _vp, _ll, _vp],

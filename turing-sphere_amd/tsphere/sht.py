@@ -52,13 +52,15 @@ def plan_tables(cfg: ShtConfig) -> dict:
         "ab": np.zeros(2 * nlm, np.float32),
         "amm": np.zeros(cfg.mmax + 1, np.float32),
         "ctstw": np.zeros(3 * cfg.nlat, np.float32),
-        "seed_y": np.zeros((cfg.mmax + 1) * npair, np.float32),
-        "seed_ny": np.zeros((cfg.mmax + 1) * npair, np.int32),
+        "seed_y0": np.zeros((cfg.mmax + 1) * npair, np.float32),
+        "seed_y1": np.zeros((cfg.mmax + 1) * npair, np.float32),
+        "seed_l": np.zeros((cfg.mmax + 1) * npair, np.int32),
         "trig": np.zeros(2 * cfg.nphi, np.float32),
     }
     c = _cfg_struct(cfg, "auto", "fp32", 1)
     _lib.call("tsphere_plan_tables", C.byref(c), *(out[k].ctypes.data for k in
-                                                   ("ab", "amm", "ctstw", "seed_y", "seed_ny", "trig")))
+                                                   ("ab", "amm", "ctstw", "seed_y0", "seed_y1",
+                                                    "seed_l", "trig")))
     return out
 
 
