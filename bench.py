@@ -111,36 +111,21 @@ def cpu_baseline(rows: int, nfield: int = 2):
                        f"work (<0.1% of it) not included")}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--legendre", default="fp32", choices=["fp32", "bf16x3"])
-    ap.add_argument("--graph-steps", type=int, default=10)
-    ap.add_argument("--cpu-rows", type=int, default=64)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
-              f"step on a single MI355X -- see DESIGN.md for how it would shard.",
-              file=sys.stderr)
-        return 2
-
+def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: bool):
+    """Time the IMEX step with one Legendre implementation; per-kernel means;
+    error of one synthesis of the initial state vs the f64 oracle."""
     import tsphere
-    from tsphere import ShtConfig, ShtPlan
+    from tsphere import ShtConfig, ShtPlan, _lib
     from tsphere.model import GpuModel
-    import ctypes as C
-    from tsphere import _lib
 
     cfg = ShtConfig(LMAX, LMAX, NLAT, NPHI)
     npts, nlm = cfg.npts, cfg.nlm
-    log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={args.legendre}")
-    sht = ShtPlan.create(cfg, legendre=args.legendre, max_fields=2)
+    log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={legendre}")
+    sht = ShtPlan.create(cfg, legendre=legendre, max_fields=2)
     src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
     gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"])
     gm.set_params(PARAMS)
-    gm.graph_steps = args.graph_steps
+    gm.graph_steps = graph_steps
     ru, rv = initial_grid(npts, 0)
     u0 = (PARAMS["A"] + ru).astype(np.float32)
     v0 = (PARAMS["B"] / PARAMS["A"] + rv).astype(np.float32)
@@ -149,21 +134,16 @@ def main() -> int:
     gm.upload("U", U0)
     gm.upload("V", V0)
     log(f"[bench] step ops: {gm.describe()['step']}")
-
-    # warmup (also captures the graph)
-    gm.step(args.warmup)
+    gm.step(warmup)
     gm.sync()
-    log(f"[bench] warmup {args.warmup} done")
     t0 = time.perf_counter()
-    gm.step(args.steps)
+    gm.step(steps)
     gm.sync()
     elapsed = time.perf_counter() - t0
-    steps_s = args.steps / elapsed
     U = gm.read("U")
     finite = bool(np.all(np.isfinite(U)))
-    log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms -> {steps_s:.1f} steps/s")
+    log(f"[bench] {legendre}: {steps} steps in {elapsed * 1e3:.1f} ms -> {steps / elapsed:.1f} steps/s")
 
-    # per-kernel durations (HIP events on the model's stream)
     L = _lib.load()
     s = gm.stream
     Ub = gm._host.get("U")
@@ -171,27 +151,93 @@ def main() -> int:
     qo = tsphere.DeviceBuffer(2 * 2 * nlm, "bench-q")
     sp = tsphere.DeviceBuffer(2 * npts, "bench-spat")
     h = sht.handle
-    reps = 20
     kms = {
         "leg_synth": time_stage(lambda: _lib.check(L.tsphere_leg_synth(
-            h, 2, Ub.ptr, 2 * nlm, fm.ptr, s.handle)), s, reps),
+            h, 2, Ub.ptr, 2 * nlm, fm.ptr, s.handle)), s, 20),
         "fft_synth": time_stage(lambda: _lib.check(L.tsphere_fourier_synth(
-            h, 2, fm.ptr, sp.ptr, npts, s.handle)), s, reps),
+            h, 2, fm.ptr, sp.ptr, npts, s.handle)), s, 20),
         "fft_analys": time_stage(lambda: _lib.check(L.tsphere_fourier_analys(
-            h, 2, sp.ptr, npts, fm.ptr, s.handle)), s, reps),
+            h, 2, sp.ptr, npts, fm.ptr, s.handle)), s, 20),
         "leg_analys": time_stage(lambda: _lib.check(L.tsphere_leg_analys(
-            h, 2, fm.ptr, qo.ptr, 2 * nlm, s.handle)), s, reps),
+            h, 2, fm.ptr, qo.ptr, 2 * nlm, s.handle)), s, 20),
     }
-    log(f"[bench] kernel ms: {kms}")
+    log(f"[bench] {legendre} kernel ms: {kms}")
+    out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "kernel_ms": kms, "finite": finite}
+    if err:
+        import oracle as O
+
+        ref = O.ShtReference(O.Cfg(LMAX, LMAX, NLAT, NPHI))
+        spat = sht.synth(U0)
+        out["synth_rel_err_vs_f64"] = O.rel_l2(spat, ref.synth_fast(U0.astype(np.float64)))
+        q = sht.analys(u0)
+        out["analys_rel_err_vs_f64"] = O.rel_l2(q, ref.analys_fast(u0.astype(np.float64)))
+        log(f"[bench] {legendre} errors vs f64: synth {out['synth_rel_err_vs_f64']:.2e} "
+            f"analys {out['analys_rel_err_vs_f64']:.2e}")
+    gm.destroy()
+    sht.destroy()
+    return out
+
+
+def batched_sht(lmax: int, nf: int, legendre: str, reps: int = 10):
+    """Round trips of `nf` fields at `lmax` (Gauss lmax+1 x 2(lmax+1)) per second."""
+    import tsphere
+
+    cfg = tsphere.ShtConfig(lmax, lmax, lmax + 1, 2 * (lmax + 1))
+    p = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=nf)
+    q = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
+    sb = tsphere.DeviceBuffer(nf * cfg.npts)
+    q.fill(0.01, p.stream)
+    bs = p.create_synth_binding(q, sb, nf, 2 * cfg.nlm, cfg.npts)
+    ba = p.create_analys_binding(sb, q, nf, cfg.npts, 2 * cfg.nlm)
+    ms = time_stage(lambda: (p.encode_synth_into(p.stream, bs), p.encode_analys_into(p.stream, ba)),
+                    p.stream, reps)
+    p.destroy()
+    f_leg, f_fft = flops_per_transform(lmax, lmax + 1, 2 * (lmax + 1))
+    return {"round_trips_per_s": nf / (ms * 1e-3), "ms_per_batch": ms,
+            "sht_gflops": 2 * nf * (f_leg + f_fft) / (ms * 1e-3) / 1e9}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--legendre", default="fp32", choices=["fp32", "bf16x3"],
+                    help="the variant reported as `value` (both are measured)")
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip errors and batched extras")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+              f"step on a single MI355X -- see DESIGN.md for how it would shard.",
+              file=sys.stderr)
+        return 2
+
+    variants = {}
+    for leg in ("fp32", "bf16x3"):
+        variants[leg] = run_variant(leg, args.steps, args.warmup, args.graph_steps,
+                                    not args.quick)
+    main_v = variants[args.legendre]
+    kms = main_v["kernel_ms"]
     f_leg, f_fft = flops_per_transform(LMAX, NLAT, NPHI)
     dom = max(("leg_synth", "leg_analys"), key=lambda k: kms[k])
     achieved = 2 * f_leg / (kms[dom] * 1e-3)  # 2 fields per launch
     peak = PEAK_BF16 if args.legendre == "bf16x3" else PEAK_FP32
+    steps_s = main_v["steps_per_s"]
     sht_flops_step = 4 * (f_leg + f_fft)
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"{dom}_c2_pmc.json")
+    prof = os.path.join(ROOT, "profiles", f"{dom}_brusselator_l1023_pmc.json")
     if os.path.exists(prof):
         traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+
+    extras = {}
+    if not args.quick:
+        for leg in ("fp32", "bf16x3"):
+            extras[f"sht_64fields_l511_{leg}"] = batched_sht(511, 64, leg)
+            log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
 
     cpu = None
     if not args.no_cpu:
@@ -205,7 +251,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": round(main_v["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -218,17 +264,21 @@ def main() -> int:
         "sht_gflops": round(sht_flops_step * steps_s / 1e9, 2),
         "transforms_per_s": round(4 * steps_s, 2),
         "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
-        "finite": finite,
-        "roofline": {"bound": "mfma" if args.legendre == "bf16x3" else "valu",
-                     "kernel": dom, "achieved": round(achieved / 1e12, 3),
+        "finite": main_v["finite"],
+        "variants": {k: {kk: (round(vv, 8) if isinstance(vv, float) else vv)
+                         for kk, vv in v.items() if kk != "kernel_ms"}
+                     | {"kernel_ms": {kk: round(vv, 5) for kk, vv in v["kernel_ms"].items()}}
+                     for k, v in variants.items()},
+        "batched": extras,
+        "roofline": {"bound": "mfma", "kernel": dom,
+                     "pipe": ("bf16 MFMA (split x3)" if args.legendre == "bf16x3"
+                              else "fp32 FMA on the vector ALU (157.3 TF peak = packed fp32)"), "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
                      "flops_per_launch": 2 * f_leg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    gm.destroy()
-    sht.destroy()
     return 0
 
 

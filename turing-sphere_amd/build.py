@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtsphere.so")
-SOURCES = ["tsphere_runtime.hip", "sht_legendre.hip", "sht_fourier.hip"]
+SOURCES = ["tsphere_runtime.hip", "sht_legendre.hip", "sht_legendre_mfma.hip",
+           "sht_fourier.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

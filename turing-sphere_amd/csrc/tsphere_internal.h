@@ -40,6 +40,22 @@ struct LegTables {
   int lmax, mmax, nlat, npair;
 };
 
+// The same seeds in f64, for the FP64 recurrence of the MFMA path.
+struct SeedD {
+  double y0, y1;
+  int l;
+  int pad;
+};
+
+// Tables of the split-BF16 MFMA path (its recurrence runs in FP64).
+struct LegTables64 {
+  const double2* ab;     // [nlm + pad]: (a_l^m, b_l^m) in f64
+  const double* ct;      // [npair]: cos(theta_j), f64
+  const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
+  const SeedD* seed;     // [(mmax+1) * npair]
+  int lmax, mmax, nlat, npair;
+};
+
 struct Plan {
   tsphere_sht_config cfg{};
   int fourier = TSPHERE_FOURIER_FFT;
@@ -49,12 +65,19 @@ struct Plan {
   float2* d_ct = nullptr;
   float* d_w = nullptr;
   SeedL* d_seed = nullptr;
+  double2* d_ab64 = nullptr;   // MFMA path only
+  double* d_ct64 = nullptr;
+  SeedD* d_seed64 = nullptr;
+  int ctc_synth = 4, ctc_analys = 1;  // MFMA column tiles per block (tunable)
   float2* d_trig = nullptr;
   float2* d_fm = nullptr;  // Fourier scratch, max_fields * (mmax+1) * nlat
   size_t bytes = 0;
   std::vector<double> theta, cos_theta, weights;
   LegTables tables() const {
     return LegTables{d_ab, d_ct, d_w, d_seed, cfg.lmax, cfg.mmax, cfg.nlat, npair};
+  }
+  LegTables64 tables64() const {
+    return LegTables64{d_ab64, d_ct64, d_w, d_seed64, cfg.lmax, cfg.mmax, cfg.nlat, npair};
   }
 };
 
@@ -81,6 +104,8 @@ struct HostTables {
   std::vector<float> ct, st, w;  // nlat each
   std::vector<float> ct_lo;      // nlat: f32(x - f32(x)), the low half of cos(theta)
   std::vector<SeedL> seed;     // (mmax+1)*npair
+  std::vector<SeedD> seed64;   // (mmax+1)*npair, same seeds in f64
+  std::vector<double> ab64;    // 2*(nlm + pad), f64 coefficients
   std::vector<float> trig;     // 2*nphi
   std::vector<double> x, wg;   // f64 Gauss nodes / weights
 };
@@ -96,6 +121,10 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
                             hipStream_t s);
 hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
                              hipStream_t s);
+hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields& q, float2* fm,
+                                 int ctc, hipStream_t s);
+hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
+                                  const Fields& q, int ctc, hipStream_t s);
 hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const float2* trig,
                                 int nfield, const float2* fm, const Fields& spat, hipStream_t s);
 hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, const float2* trig,

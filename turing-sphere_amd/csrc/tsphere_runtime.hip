@@ -121,17 +121,19 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
     amm[m] = -std::sqrt(tt);
     if (m % 2 == 0) amm[m] = -amm[m];
   }
-  t.ab.assign(2 * (nlm + kAbPad), 0.0f);
+  t.ab64.assign(2 * (nlm + kAbPad), 0.0);
   for (int m = 0; m <= mmax; m++) {
-    if (m + 1 <= lmax) t.ab[2 * lm_index(lmax, m + 1, m)] = (float)std::sqrt(2.0 * m + 3.0);
+    if (m + 1 <= lmax) t.ab64[2 * lm_index(lmax, m + 1, m)] = std::sqrt(2.0 * m + 3.0);
     for (int l = m + 2; l <= lmax; l++) {
       const long long lm = lm_index(lmax, l, m);
       const double t1 = (double)(l + m) * (l - m);
       const double t2 = (double)(l - 1 + m) * (l - 1 - m);
-      t.ab[2 * lm] = (float)std::sqrt(((2.0 * l + 1.0) * (2.0 * l - 1.0)) / t1);
-      t.ab[2 * lm + 1] = (float)-std::sqrt(((2.0 * l + 1.0) / (2.0 * l - 3.0)) * (t2 / t1));
+      t.ab64[2 * lm] = std::sqrt(((2.0 * l + 1.0) * (2.0 * l - 1.0)) / t1);
+      t.ab64[2 * lm + 1] = -std::sqrt(((2.0 * l + 1.0) / (2.0 * l - 3.0)) * (t2 / t1));
     }
   }
+  t.ab.assign(2 * (nlm + kAbPad), 0.0f);
+  for (size_t i = 0; i < t.ab64.size(); i++) t.ab[i] = (float)t.ab64[i];
   t.amm.resize(mmax + 1);
   for (int m = 0; m <= mmax; m++) t.amm[m] = (float)amm[m];
 
@@ -154,6 +156,7 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
   // |ytilde_l| > ACCURACY (the reference's significance rule, common.ts:19),
   // and keep the values at the last l = m + 4k at or before it.
   t.seed.resize((size_t)(mmax + 1) * npair);
+  t.seed64.resize((size_t)(mmax + 1) * npair);
   const double thr = (double)kAccuracy;
   for (int m = 0; m <= mmax; m++) {
     const double la = std::log2(std::fabs(amm[m]));
@@ -166,6 +169,7 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
       double y0 = sg * std::exp2(e0 - E);
       double y1 = (m < lmax) ? std::sqrt(2.0 * m + 3.0) * x * y0 : 0.0;
       SeedL seed{0.f, 0.f, 0x7fffffff, 0};
+      SeedD seed64{0.0, 0.0, 0x7fffffff, 0};
       double c0 = y0, c1 = y1;
       int cE = E, cl = m;
       for (int l = m; l <= lmax; l += 2) {
@@ -178,8 +182,11 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
         const bool sig0 = std::fabs(y0) * std::exp2((double)E) > thr;
         const bool sig1 = l + 1 <= lmax && std::fabs(y1) * std::exp2((double)E) > thr;
         if (sig0 || sig1) {
-          seed.y0 = (float)std::ldexp(c0, cE);
-          seed.y1 = (cl + 1 <= lmax) ? (float)std::ldexp(c1, cE) : 0.0f;
+          seed64.y0 = std::ldexp(c0, cE);
+          seed64.y1 = (cl + 1 <= lmax) ? std::ldexp(c1, cE) : 0.0;
+          seed64.l = cl;
+          seed.y0 = (float)seed64.y0;
+          seed.y1 = (float)seed64.y1;
           seed.l = cl;
           break;
         }
@@ -201,6 +208,7 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
         }
       }
       t.seed[(size_t)m * npair + j] = seed;
+      t.seed64[(size_t)m * npair + j] = seed64;
     }
   }
 
@@ -389,7 +397,8 @@ int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, fl
 static void plan_free(Plan* p) {
   if (!p) return;
   for (void* ptr : {(void*)p->d_ab, (void*)p->d_ct, (void*)p->d_w, (void*)p->d_seed,
-                    (void*)p->d_trig, (void*)p->d_fm})
+                    (void*)p->d_ab64, (void*)p->d_ct64, (void*)p->d_seed64, (void*)p->d_trig,
+                    (void*)p->d_fm})
     if (ptr) (void)hipFree(ptr);
 }
 
@@ -412,8 +421,6 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   if (cfg->fourier == TSPHERE_FOURIER_FFT && !fits)
     return fail(TSPHERE_ERR_INVALID,
                 "fourier:'fft' requires power-of-two nphi in [8, 4096] (nphi=%d)", cfg->nphi);
-  if (cfg->legendre == TSPHERE_LEG_BF16X3)
-    return fail(TSPHERE_ERR_UNSUPPORTED, "split-BF16 MFMA Legendre stage not built");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     (void)hipGetLastError();
@@ -444,6 +451,20 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   if (e == hipSuccess) e = upload(&p->d_ct, ct2.data(), (size_t)p->npair);
   if (e == hipSuccess) e = upload(&p->d_w, t.w.data(), (size_t)cfg->nlat);
   if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
+  size_t bytes64 = 0;
+  if (cfg->legendre == TSPHERE_LEG_BF16X3) {
+    if (e == hipSuccess)
+      e = upload(&p->d_ab64, reinterpret_cast<const double2*>(t.ab64.data()),
+                 (size_t)(p->nlm + kAbPad));
+    if (e == hipSuccess) e = upload(&p->d_ct64, t.x.data(), (size_t)p->npair);
+    if (e == hipSuccess) e = upload(&p->d_seed64, t.seed64.data(), t.seed64.size());
+    bytes64 = sizeof(double2) * (p->nlm + kAbPad) + sizeof(double) * p->npair +
+              sizeof(SeedD) * t.seed64.size();
+    // column tiles per synthesis block: the batched fields' columns, up to 4
+    const int ct = (2 * cfg->max_fields + 15) / 16;
+    p->ctc_synth = ct >= 4 ? 4 : ct >= 2 ? 2 : 1;
+    p->ctc_analys = ct >= 2 ? 2 : 1;
+  }
   if (e == hipSuccess)
     e = upload(&p->d_trig, reinterpret_cast<const float2*>(t.trig.data()), (size_t)cfg->nphi);
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_fm, sizeof(float2) * nfm);
@@ -453,7 +474,8 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     return hip_fail(e, "tsphere_plan_create");
   }
   p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + sizeof(float) * cfg->nlat +
-             sizeof(SeedL) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm;
+             sizeof(SeedL) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm +
+             bytes64;
   *out = p;
   return ok();
 }
@@ -526,8 +548,21 @@ static Fields listed(const void* const* v, int n) {
   return F;
 }
 
+static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hipStream_t s) {
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->ctc_synth, s);
+  return launch_leg_synth(p->tables(), nfield, q, fm, s);
+}
+
+static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, const Fields& q,
+                             hipStream_t s) {
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    return launch_leg_analys_mfma(p->tables64(), nfield, fm, q, p->ctc_analys, s);
+  return launch_leg_analys(p->tables(), nfield, fm, q, s);
+}
+
 static int run_synth(Plan* p, int nfield, const Fields& q, const Fields& spat, hipStream_t s) {
-  TS_HIP(launch_leg_synth(p->tables(), nfield, q, p->d_fm, s), "leg_synth");
+  TS_HIP(leg_synth(p, nfield, q, p->d_fm, s), "leg_synth");
   TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
                               nfield, p->d_fm, spat, s),
          "fourier_synth");
@@ -538,7 +573,7 @@ static int run_analys(Plan* p, int nfield, const Fields& spat, const Fields& q, 
   TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
                                nfield, spat, p->d_fm, s),
          "fourier_analys");
-  TS_HIP(launch_leg_analys(p->tables(), nfield, p->d_fm, q, s), "leg_analys");
+  TS_HIP(leg_analys(p, nfield, p->d_fm, q, s), "leg_analys");
   return ok();
 }
 
@@ -546,8 +581,8 @@ int tsphere_leg_synth(tsphere_plan p, int nfield, const float* qlm, long long ql
                       float* fm, tsphere_stream s) {
   if (int r = check_batch(p, nfield, qlm, fm)) return r;
   if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
-  TS_HIP(launch_leg_synth(p->tables(), nfield, strided(qlm, 4 * qlm_stride),
-                          reinterpret_cast<float2*>(fm), S(s)),
+  TS_HIP(leg_synth(p, nfield, strided(qlm, 4 * qlm_stride), reinterpret_cast<float2*>(fm),
+                   S(s)),
          "leg_synth");
   return ok();
 }
@@ -556,8 +591,8 @@ int tsphere_leg_analys(tsphere_plan p, int nfield, const float* fm, float* qlm,
                        long long qlm_stride, tsphere_stream s) {
   if (int r = check_batch(p, nfield, fm, qlm)) return r;
   if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
-  TS_HIP(launch_leg_analys(p->tables(), nfield, reinterpret_cast<const float2*>(fm),
-                           strided(qlm, 4 * qlm_stride), S(s)),
+  TS_HIP(leg_analys(p, nfield, reinterpret_cast<const float2*>(fm),
+                    strided(qlm, 4 * qlm_stride), S(s)),
          "leg_analys");
   return ok();
 }
