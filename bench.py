@@ -198,6 +198,94 @@ def batched_sht(lmax: int, nf: int, legendre: str, reps: int = 10):
             "sht_gflops": 2 * nf * (f_leg + f_fft) / (ms * 1e-3) / 1e9}
 
 
+def ensemble_bench(steps: int = 200, legendre: str = "fp32"):
+    """256 Allen-Cahn runs, 64 each at lmax 63/127/255/511 on (L+1) x 2(L+1)
+    Gauss grids, eps = U[0.01, 0.05] (mulberry32 seed 0, its own stream),
+    dt = 1e-3, u0 = U(-0.1, 0.1) (mulberry32 seed = run index), all runs
+    stepped by one HIP graph per 10 steps (groups as parallel branches)."""
+    import tsphere
+    from tsphere.ensemble import MixedEnsemble
+
+    src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "allencahn.m")).read()
+    groups = [(tsphere.ShtConfig(L, L, L + 1, 2 * (L + 1)), 64) for L in (63, 127, 255, 511)]
+    ens = MixedEnsemble.create(src, ["eps2", "dt"], ["U"], ["u"], groups, legendre=legendre,
+                               tune=True)
+    eps = 0.01 + 0.04 * mulberry_uniform(0, 256)
+    ens.set_params({"eps2": eps * eps, "dt": 1e-3})
+    noise = []
+    for g in ens.groups:
+        for r in range(g.runs):
+            noise.append((0.2 * mulberry_uniform(g.first_run + r, g.cfg.npts) - 0.1)
+                         .astype(np.float32))
+    ens.init(noise)
+    ens.graph_steps = 10
+    ens.step(20)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    dt = time.perf_counter() - t0
+    U = ens.read("U")
+    finite = all(bool(np.all(np.isfinite(u))) for u in U)
+    tiles = ens.tuning
+    ens.destroy()
+    f_step = sum(64 * 2 * sum(flops_per_transform(L, L + 1, 2 * (L + 1)))
+                 for L in (63, 127, 255, 511))
+    return {"workload": "256 Allen-Cahn runs, 64 each at lmax 63/127/255/511, one graph",
+            "steps": steps, "batched_steps_per_s": steps / dt,
+            "run_steps_per_s": 256 * steps / dt, "ms_per_step": 1e3 * dt / steps,
+            "sht_gflops": f_step * steps / dt / 1e9, "finite": finite,
+            "tuned_tiles": [{k: v for k, v in t.items() if k in ("synth", "analys", "cached")}
+                            for t in tiles]}
+
+
+def adjoint_bench(steps: int = 200, every: int = 20):
+    """Gradient of L = sum_grid (synth(U_200) - u*)^2 with respect to the
+    initial state and a, b, D1, D2 of Schnakenberg at lmax 255 (256 x 512),
+    dt = 0.01, 200 steps, checkpoints every 20; u* is the u field of a run
+    with (a, b, D1, D2) x 1.05 from the same start."""
+    import tsphere
+    from tsphere.adjoint import AdjointModel
+
+    L = 255
+    cfg = tsphere.ShtConfig(L, L, L + 1, 2 * (L + 1))
+    sht = tsphere.ShtPlan.create(cfg, max_fields=2)
+    src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "schnakenberg.m")).read()
+    names = ["a", "b", "D1", "D2", "dt"]
+    p = {"a": 0.1, "b": 0.9, "D1": 6.5e-4, "D2": 0.026, "dt": 0.01}
+    am = AdjointModel.create(sht, cfg, src, names, ["U", "V"], ["u", "v"], every)
+    ru, rv = initial_grid(cfg.npts, 0)
+    x0 = {"U": sht.analys((1.0 + ru).astype(np.float32)),
+          "V": sht.analys((0.9 + rv).astype(np.float32))}
+    am.set_params({k: (v * 1.05 if k != "dt" else v) for k, v in p.items()})
+    target = sht.synth(am.forward(x0, steps)["U"])
+    am.set_params(p)
+    t0 = time.perf_counter()
+    uN = sht.synth(am.forward(x0, steps)["U"])
+    cot = {"U": am.synth_adjoint(2.0 * (uN - target)), "V": np.zeros(2 * sht.nlm, np.float32)}
+    grads, pg = am.vjp(x0, steps, cot)
+    dt = time.perf_counter() - t0
+    out = {"workload": "Schnakenberg lmax 255 (256x512), 200 steps, loss on u_200, "
+                       "checkpoints every 20",
+           "ms_per_gradient": 1e3 * dt, "loss": float(np.sum((uN - target) ** 2)),
+           "dL_dparams": {k: pg[k] for k in ("a", "b", "D1", "D2")},
+           "finite": bool(np.all(np.isfinite(grads["U"])))}
+    am.destroy()
+    sht.destroy()
+    return out
+
+
+def mulberry_uniform(seed: int, n: int) -> np.ndarray:
+    """n mulberry32 deviates in [0, 1) (src/mgpu/noise.ts:15-24), vectorised."""
+    k = np.arange(1, n + 1, dtype=np.uint64)
+    t = ((np.uint64(seed & 0xFFFFFFFF) + k * np.uint64(0x6D2B79F5)) &
+         np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    with np.errstate(over="ignore"):
+        t = (t ^ (t >> np.uint32(15))) * (t | np.uint32(1))
+        t ^= t + (t ^ (t >> np.uint32(7))) * (t | np.uint32(61))
+        return (t ^ (t >> np.uint32(14))).astype(np.float64) / 4294967296.0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +326,10 @@ def main() -> int:
         for leg in ("fp32", "bf16x3"):
             extras[f"sht_64fields_l511_{leg}"] = batched_sht(511, 64, leg)
             log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
+        extras["ensemble_allencahn_256runs"] = ensemble_bench()
+        log(f"[bench] ensemble: {extras['ensemble_allencahn_256runs']}")
+        extras["adjoint_schnakenberg_l255"] = adjoint_bench()
+        log(f"[bench] adjoint: {extras['adjoint_schnakenberg_l255']}")
 
     cpu = None
     if not args.no_cpu:

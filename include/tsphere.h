@@ -68,6 +68,10 @@ int tsphere_fill_f32(float *dst, float value, long long count, tsphere_stream s)
 int tsphere_event_create(tsphere_event *out);
 int tsphere_event_destroy(tsphere_event e);
 int tsphere_event_record(tsphere_event e, tsphere_stream s);
+/* Make later work on `s` wait for `e` (no host block).  Inside a capture this
+ * forks/joins streams into one graph (the reference has one queue and needs
+ * no counterpart). */
+int tsphere_stream_wait(tsphere_stream s, tsphere_event e);
 /* Milliseconds between two recorded events (blocking on `end`). */
 int tsphere_event_elapsed(tsphere_event start, tsphere_event end, float *ms);
 
@@ -152,8 +156,10 @@ int tsphere_analys_multi(tsphere_plan p, int nfield, const float *const *spat, f
 
 /* Single stages (ShtPlan.encodeStage, sht.ts:368-401; what
  * scripts/diagnose-sht.ts reads to tell the halves apart).  fm is the
- * Fourier-space intermediate, complex [(f*(mmax+1) + m)*nlat + ilat] as
- * interleaved floats (for one field, the reference's fmBuf layout). */
+ * Fourier-space intermediate, complex [(f*nlat + ilat)*(mmax+1) + m] as
+ * interleaved floats: row-major per latitude, the transpose of the
+ * reference's fmBuf ([m*nlat + ilat], sht.ts:291-298), so that the FFT moves
+ * contiguous rows. */
 int tsphere_leg_synth(tsphere_plan p, int nfield, const float *qlm, long long qlm_stride,
                       float *fm, tsphere_stream s);
 int tsphere_fourier_synth(tsphere_plan p, int nfield, const float *fm, float *spat,
@@ -186,6 +192,50 @@ int tsphere_rtc_check(const char *source, const char *options);
  * params buffer, plan.ts:410-420). */
 int tsphere_kernel_launch(tsphere_kernel k, long long count, int nptr, void *const *ptrs,
                           int nscalar, const float *scalars, tsphere_stream s);
+
+/* Ensembles: nptr * per fields in one batched launch per stage, field f at
+ * ptr[f / per] + (f % per) * stride floats -- e.g. two species of 64 runs
+ * each.  (Batched form of encodeSynthInto/encodeAnalysInto, sht.ts:326-353.) */
+int tsphere_synth_fields(tsphere_plan p, int nptr, int per, const float *const *qlm,
+                         long long qlm_stride, float *const *spat, long long spat_stride,
+                         tsphere_stream s);
+int tsphere_analys_fields(tsphere_plan p, int nptr, int per, const float *const *spat,
+                          long long spat_stride, float *const *qlm, long long qlm_stride,
+                          tsphere_stream s);
+
+/* Transposes of the transforms, the vector-Jacobian products of reverse-mode
+ * gradients (no reference counterpart: the reference has no adjoint).  With
+ * Q the 2*nlm real coefficients and g the nlat*nphi grid values,
+ *   synth_adjoint:  qbar = S^T gbar   (S: Q -> grid, the map tsphere_synth applies)
+ *   analys_adjoint: gbar = A^T qbar   (A: grid -> Q, the map tsphere_analys applies)
+ * exactly, up to FP32 rounding; fields addressed as tsphere_synth_fields.
+ * Outputs are overwritten (use tsphere_accumulate to sum contributions). */
+int tsphere_synth_adjoint(tsphere_plan p, int nptr, int per, const float *const *gbar,
+                          long long g_stride, float *const *qbar, long long q_stride,
+                          tsphere_stream s);
+int tsphere_analys_adjoint(tsphere_plan p, int nptr, int per, const float *const *qbar,
+                           long long q_stride, float *const *gbar, long long g_stride,
+                           tsphere_stream s);
+int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream s); /* dst += src */
+
+/* ------------------------------------------------------- tile shapes
+ * The Legendre stage's workgroup shape (no reference counterpart: WebGPU's
+ * workgroup sizes are fixed in sht.ts:70-89).  Stage 0 = synthesis,
+ * 1 = analysis; fields left 0 use the built-in choice.
+ *   FP32 form:  nf fields per workgroup (1, 2, 4), nwaves waves (1..4),
+ *               k latitude pairs per lane (1, 2, 4, 8), 64*nwaves*k >= pairs.
+ *   MFMA form:  ctc 16-column tiles per workgroup (1, 2; 4 for synthesis). */
+typedef struct {
+  int nf, nwaves, k, ctc;
+} tsphere_leg_tile;
+
+int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile *tile); /* NULL = built-in */
+int tsphere_plan_get_tile(tsphere_plan p, int stage, int nfield, tsphere_leg_tile *out);
+/* Autotune: time every candidate tile of both stages for `nfield` fields
+ * (`reps` launches each, on `s`, on plan scratch and a temporary buffer),
+ * keep the fastest in the plan; best[2] / best_ms[2] may be NULL.  Blocks. */
+int tsphere_plan_tune(tsphere_plan p, int nfield, int reps, tsphere_stream s,
+                      tsphere_leg_tile *best, float *best_ms);
 
 /* ------------------------------------------------------- step recording
  * The reference re-records its op list every step (plan.ts:436-478); here a

@@ -250,6 +250,35 @@ class ShtReference:
     def analys_fast(self, spat) -> np.ndarray:
         return self.legendre_analys(self.fourier_analys_fast(spat))
 
+    # ---- transposes (reverse-mode gradients; no reference counterpart)
+    # synth:  f_ij = Re F_0i + 2 sum_{m>=1} Re(F_mi e^{i m phi_j}),  F_mi = sum_l q_lm P_lm(x_i)
+    # analys: q_lm = sum_i w_i P_lm(x_i) (2 pi / nphi) sum_j f_ij e^{-i m phi_j}
+    # Both are real-linear maps on (re, im)-interleaved coefficient vectors;
+    # their transposes follow term by term, and the dot-product identity
+    # <S x, y> = <x, S^T y> pins them (tests/test_oracle.py).
+    def synth_T(self, gbar) -> np.ndarray:
+        """S^T: q_lm = sum_i P_lm(x_i) c_m sum_j g_ij e^{-i m phi_j}, c_0 = 1, c_m = 2."""
+        c = self.cfg
+        g = np.asarray(gbar, np.float64).reshape(c.nlat, c.nphi)
+        G = np.fft.rfft(g, axis=1)[:, :c.mmax + 1]
+        G[:, 1:] *= 2.0
+        gm = np.empty((c.mmax + 1, c.nlat, 2))
+        gm[:, :, 0] = G.real.T
+        gm[:, :, 1] = G.imag.T
+        # legendre_analys weights row i by the Gauss weight: divide it out
+        _, w = gauss_nodes_weights(c.nlat)
+        gm /= w[None, :, None]
+        return self.legendre_analys(gm.reshape(-1))
+
+    def analys_T(self, qbar) -> np.ndarray:
+        """A^T: g_ij = w_i (2 pi / nphi) sum_{m>=0} Re(F_mi e^{i m phi_j})."""
+        c = self.cfg
+        F = self.legendre_synth(qbar).reshape(c.mmax + 1, c.nlat, 2).copy()
+        F[1:] *= 0.5
+        g = self.fourier_synth_fast(F.reshape(-1)).reshape(c.nlat, c.nphi)
+        _, w = gauss_nodes_weights(c.nlat)
+        return (g * (w * (2.0 * np.pi / c.nphi))[:, None]).reshape(-1)
+
 
 def random_spectrum(cfg: Cfg, seed: int = 12345) -> np.ndarray:
     """reference.ts:122-136 (float32 output, m=0 imaginary parts zero)."""
@@ -340,6 +369,83 @@ def imex_trajectory(cfg: Cfg, model: int, params: dict, U0, V0=None, steps: int 
         Vn = imex(dt, dt * params["D2"], V, ref.analys_fast(r2), lam)
         U, V = Un, Vn
     return U, V
+
+
+def _react_partials(model: int, params: dict, u, v):
+    """d(reaction)/d(u, v) and d(reaction)/d(params) of the models/*.m
+    reaction terms (the derivatives of spec.h:146-161's formulas)."""
+    if model == ALLENCAHN:
+        return {"uu": 1.0 - 3.0 * u * u}, {}
+    uv = u * v
+    uu = u * u
+    if model == SCHNAKENBERG:
+        d = {"uu": -1.0 + 2.0 * uv, "uv": uu, "vu": -2.0 * uv, "vv": -uu}
+        dp = {"a": (np.ones_like(u), None), "b": (None, np.ones_like(u))}
+        return d, dp
+    B = params["B"]
+    d = {"uu": -(B + 1.0) + 2.0 * uv, "uv": uu, "vu": B - 2.0 * uv, "vv": -uu}
+    dp = {"A": (np.ones_like(u), None), "B": (-u, u)}
+    return d, dp
+
+
+def imex_vjp(cfg: Cfg, model: int, params: dict, U0, V0, steps: int, Ubar, Vbar=None,
+             ref: ShtReference | None = None):
+    """f64 reverse-mode gradient of `steps` IMEX Euler steps (imex_trajectory)
+    for a cotangent (Ubar, Vbar) of the final state: returns
+    (U0bar, V0bar, {param: grad}) with grads for the reaction parameters, the
+    diffusion coefficients (D1, D2 / eps2) and dt.  All states are kept
+    (f64, small sizes); each step is differentiated by hand:
+      Un = (U + dt A r_u(S U, S V)) / (1 + dt D1 lam)."""
+    ref = ref or ShtReference(cfg)
+    lam = eigenvalues(cfg).astype(np.float64)
+    dt = params["dt"]
+    two = model != ALLENCAHN
+    Us, Vs = [np.array(U0, np.float64)], [None if V0 is None else np.array(V0, np.float64)]
+    for _ in range(steps):
+        U1, V1 = imex_trajectory(cfg, model, params, Us[-1], Vs[-1], 1, ref)
+        Us.append(U1)
+        Vs.append(V1)
+    Ub = np.array(Ubar, np.float64)
+    Vb = None if not two else np.array(Vbar if Vbar is not None else np.zeros_like(Ub),
+                                      np.float64)
+    dkey = ("eps2",) if not two else ("D1", "D2")
+    grads = {k: 0.0 for k in (list(dkey) + ["dt"] + (["a", "b"] if model == SCHNAKENBERG else
+                                                      ["A", "B"] if model == BRUSSELATOR else []))}
+    for k in reversed(range(steps)):
+        U, V = Us[k], Vs[k]
+        u = ref.synth_fast(U)
+        v = ref.synth_fast(V) if two else None
+        if two:
+            p0, p1 = (params["a"], params["b"]) if model == SCHNAKENBERG else (params["A"],
+                                                                              params["B"])
+            ru, rv = react(model, p0, p1, u, v)
+        else:
+            ru, rv = react(model, 0, 0, u)
+        d, dp = _react_partials(model, params, u, v)
+        outs = [("U", U, ru, dkey[0], Ub)] + ([("V", V, rv, dkey[1], Vb)] if two else [])
+        rbar = {}
+        new = {}
+        for name, X, r, dk, Xb in outs:
+            Ar = ref.analys_fast(r)
+            den = 1.0 + dt * params[dk] * lam
+            Xn = (X + dt * Ar) / den
+            new[name] = Xb / den
+            grads[dk] += float(np.sum(Xb * (-Xn * dt * lam / den)))
+            grads["dt"] += float(np.sum(Xb * (Ar / den - Xn * params[dk] * lam / den)))
+            rbar[name] = ref.analys_T(dt * Xb / den)
+        if two:
+            ub = rbar["U"] * d["uu"] + rbar["V"] * d["vu"]
+            vb = rbar["U"] * d["uv"] + rbar["V"] * d["vv"]
+            for pk, (cu, cv) in dp.items():
+                if cu is not None:
+                    grads[pk] += float(np.sum(rbar["U"] * cu))
+                if cv is not None:
+                    grads[pk] += float(np.sum(rbar["V"] * cv))
+            Ub = new["U"] + ref.synth_T(ub)
+            Vb = new["V"] + ref.synth_T(vb)
+        else:
+            Ub = new["U"] + ref.synth_T(rbar["U"] * d["uu"])
+    return Ub, Vb, grads
 
 
 def rel_l2(a, b) -> float:

@@ -199,3 +199,54 @@ def test_power_and_literals():
     assert np.float32(float(f32_lit(6.5e-4)[:-1])) == np.float32(6.5e-4)
     _, step = plan_cpu("allencahn")
     assert "pow_i3(" in step.kernel_source()
+
+
+def test_ensemble_plan_kernels():
+    """Ensemble form (many runs per launch): per-run scalars from a device
+    array, the shared eigenvalues indexed modulo one run, runs-times larger
+    buffers, and the generated HIP compiles for gfx950."""
+    from tsphere.plan import HostBuffers, ModelPlan, PlanSpec
+
+    R = 4
+    init_fn, step_fn, state, view = compile_fns("allencahn")
+    host = HostBuffers(allocate=False)
+    host.ensure_group(state, 2 * 2080 * R)
+    host.ensure("lam", 2 * 2080)
+    fb = state + [None] * len(view)
+    step = ModelPlan.create(None, PlanSpec(step_fn, fb), host, True, "step", runs=R,
+                            shared=("lam",))
+    src = step.kernel_source()
+    assert "const float* __restrict__ prm" in src
+    assert "% 4160LL" in src            # lam, shared by the runs
+    assert "prm[run * 2 + " in src      # eps2, dt per run
+    kernels = [op for op in step.ops if op.kind == "kernel"]
+    assert len(kernels) == 2 and all(op.count == 4160 * R or op.count == 8192 * R
+                                     for op in kernels)
+    assert host.get("U").count == 2 * 2080 * R and host.get("lam").count == 2 * 2080
+    with pytest.raises(ValueError):
+        ModelPlan.create(None, PlanSpec(step_fn, fb), host, True, "step", runs=0)
+
+
+@pytest.mark.parametrize("key", ["schnakenberg", "brusselator", "allencahn"])
+def test_adjoint_kernels_compile(key):
+    """Reverse mode: every element-wise statement of the step gets a
+    generated adjoint kernel (symbolic derivatives of its IR tree) that
+    compiles for gfx950; derivatives of the reaction terms appear."""
+    from tsphere import _lib
+    from tsphere.adjoint import build_adjoint_kernel
+    from tsphere.plan import HostBuffers, KernelOp, ModelPlan, PlanSpec
+
+    _, step_fn, state, view = compile_fns(key)
+    host = HostBuffers(allocate=False)
+    host.ensure_group(state, 2 * 2080)
+    tape = ModelPlan.create(None, PlanSpec(step_fn, [None] * len(step_fn.outputs)), host, True,
+                            "tape")
+    srcs = []
+    for i, op in enumerate(tape.ops):
+        if isinstance(op, KernelOp):
+            gi = [c for c in op.ins if c in tape._bufs and not c.startswith("lam")]
+            srcs.append(build_adjoint_kernel(op, f"adj{i}", gi, len(tape.param_names)))
+    src = ("__device__ __forceinline__ float ts_sign(float x) { return x; }\n" +
+           "\n".join(srcs))
+    assert "bar0[i] += ob *" in src and "partials[" in src
+    assert _lib.load().tsphere_rtc_check(src.encode(), None) == _lib.OK, _lib.last_error()

@@ -257,3 +257,62 @@ def test_imex_trajectory_logistic_uniform_exact():
     for _ in range(10):
         want = want + 0.05 * (want - want ** 3)
     assert abs(U[0] / U0[0] * u0 - want) < 1e-13
+
+
+# ------------------------------------------------ transposes and the adjoint
+@pytest.mark.parametrize("lmax,mmax,nlat,nphi", [(15, 15, 16, 32), (12, 9, 17, 32), (7, 7, 9, 20)])
+def test_transposes_dot_product(lmax, mmax, nlat, nphi):
+    """<S x, y> = <x, S^T y> and <A y, x> = <y, A^T x> in f64 for random
+    x (all 2 nlm reals, m=0 imaginary parts included) and y."""
+    cfg = O.Cfg(lmax, mmax, nlat, nphi)
+    ref = O.ShtReference(cfg)
+    rng = np.random.default_rng(lmax + nlat)
+    x = rng.normal(size=2 * cfg.nlm)
+    y = rng.normal(size=nlat * nphi)
+    a, b = np.dot(ref.synth_fast(x), y), np.dot(x, ref.synth_T(y))
+    assert abs(a - b) < 1e-11 * max(1.0, abs(a))
+    a, b = np.dot(ref.analys_fast(y), x), np.dot(y, ref.analys_T(x))
+    assert abs(a - b) < 1e-11 * max(1.0, abs(a))
+    # a wrong normalisation or a dropped m factor breaks the identity
+    bad = ref.synth_T(y) * 1.0001
+    assert abs(np.dot(ref.synth_fast(x), y) - np.dot(x, bad)) > 1e-9
+
+
+@pytest.mark.parametrize("model", [O.SCHNAKENBERG, O.BRUSSELATOR, O.ALLENCAHN])
+def test_imex_vjp_vs_finite_differences(model):
+    """The f64 adjoint against central differences of the f64 trajectory, for
+    the initial state (random direction) and every parameter."""
+    cfg = O.Cfg(7, 7, 8, 16)
+    ref = O.ShtReference(cfg)
+    rng = np.random.default_rng(7 + model)
+    params = {O.SCHNAKENBERG: {"a": 0.1, "b": 0.9, "D1": 4e-3, "D2": 8e-2, "dt": 0.05},
+              O.BRUSSELATOR: {"A": 3.0, "B": 9.0, "D1": 3e-3, "D2": 1.7e-2, "dt": 0.01},
+              O.ALLENCAHN: {"eps2": 1e-2, "dt": 0.05}}[model]
+    two = model != O.ALLENCAHN
+    U0 = rng.normal(size=2 * cfg.nlm) * 0.05
+    U0[0] += 1.0
+    V0 = (rng.normal(size=2 * cfg.nlm) * 0.05) if two else None
+    steps = 6
+    cu = rng.normal(size=2 * cfg.nlm)
+    cv = rng.normal(size=2 * cfg.nlm) if two else None
+
+    def J(p, U, V):
+        Uf, Vf = O.imex_trajectory(cfg, model, p, U, V, steps, ref)
+        return float(np.dot(cu, Uf) + (np.dot(cv, Vf) if two else 0.0))
+
+    Ub, Vb, g = O.imex_vjp(cfg, model, params, U0, V0, steps, cu, cv, ref)
+    dU = rng.normal(size=U0.size)
+    h = 1e-6
+    fd = (J(params, U0 + h * dU, V0) - J(params, U0 - h * dU, V0)) / (2 * h)
+    assert abs(fd - np.dot(Ub, dU)) < 1e-6 * max(1.0, abs(fd))
+    if two:
+        dV = rng.normal(size=V0.size)
+        fd = (J(params, U0, V0 + h * dV) - J(params, U0, V0 - h * dV)) / (2 * h)
+        assert abs(fd - np.dot(Vb, dV)) < 1e-6 * max(1.0, abs(fd))
+    for k in g:
+        hp = 1e-6 * max(abs(params[k]), 1e-3)
+        pp, pm = dict(params), dict(params)
+        pp[k] += hp
+        pm[k] -= hp
+        fd = (J(pp, U0, V0) - J(pm, U0, V0)) / (2 * hp)
+        assert abs(fd - g[k]) < 1e-5 * max(1.0, abs(fd)), (k, fd, g[k])

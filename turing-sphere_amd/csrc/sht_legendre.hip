@@ -127,8 +127,9 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fiel
   const int nthr = blockDim.x;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
-    if (r == 1 && m <= (int)blockIdx.x) break;
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
     __syncthreads();  // the previous row's readers are done
     stage_row<NF>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
     __syncthreads();
@@ -186,9 +187,12 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fiel
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         if (f0 + f >= nfield) continue;
-        float2* out = fm + ((long long)(f0 + f) * (M + 1) + m) * nlat;
-        out[j] = make_float2(accE[k][f].x + accO[k][f].x, accE[k][f].y + accO[k][f].y);
-        out[nlat - 1 - j] = make_float2(accE[k][f].x - accO[k][f].x, accE[k][f].y - accO[k][f].y);
+        // fm row-major per latitude: (f, ilat, m) at (f nlat + ilat)(M+1) + m
+        float2* out = fm + (long long)(f0 + f) * nlat * (M + 1) + m;
+        out[(long long)j * (M + 1)] =
+            make_float2(accE[k][f].x + accO[k][f].x, accE[k][f].y + accO[k][f].y);
+        out[(long long)(nlat - 1 - j) * (M + 1)] =
+            make_float2(accE[k][f].x - accO[k][f].x, accE[k][f].y - accO[k][f].y);
       }
     }
   }
@@ -248,8 +252,9 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
   const int nthr = blockDim.x, nwaves = nthr >> 6;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
-    if (r == 1 && m <= (int)blockIdx.x) break;
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
     const long long base = lm_base(L, m);
     __syncthreads();
     stage_row<NF>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
@@ -268,9 +273,9 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int fld = min(f0 + f, nfield - 1);
-        const float2* g = fm + ((long long)fld * (M + 1) + m) * nlat;
-        const float2 gN = g[jc];
-        const float2 gS = g[js];
+        const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
+        const float2 gN = g[(long long)jc * (M + 1)];
+        const float2 gS = g[(long long)js * (M + 1)];
         if (js == jc) {  // equator of an odd grid pairs with itself
           wp[k][f] = make_float2(gN.x * w, gN.y * w);
           wm[k][f] = make_float2(0.f, 0.f);
@@ -350,13 +355,6 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
   }
 }
 
-int pick_k(int npair, int& nwaves) {
-  // up to 4 waves per workgroup; K latitude pairs per lane
-  nwaves = std::min(4, (npair + 63) / 64);
-  int K = (npair + 64 * nwaves - 1) / (64 * nwaves);
-  return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8;
-}
-
 size_t synth_lds(int lmax, int nf) { return sizeof(float2) * (lmax + 1 + kPadQ) * (1 + nf); }
 
 template <int NF, int K>
@@ -378,46 +376,80 @@ hipError_t analys_nfk(const LegTables& T, int nfield, int nwaves, const float2* 
 }
 
 template <int NF>
-hipError_t synth_nf(const LegTables& T, int nfield, const Fields& q, float2* fm, hipStream_t s) {
-  int nw;
-  switch (pick_k(T.npair, nw)) {
-    case 1: return synth_nfk<NF, 1>(T, nfield, nw, q, fm, s);
-    case 2: return synth_nfk<NF, 2>(T, nfield, nw, q, fm, s);
-    case 4: return synth_nfk<NF, 4>(T, nfield, nw, q, fm, s);
-    default: return synth_nfk<NF, 8>(T, nfield, nw, q, fm, s);
+hipError_t synth_nf(const LegTables& T, int nfield, const LegTile& t, const Fields& q, float2* fm,
+                    hipStream_t s) {
+  switch (t.k) {
+    case 1: return synth_nfk<NF, 1>(T, nfield, t.nwaves, q, fm, s);
+    case 2: return synth_nfk<NF, 2>(T, nfield, t.nwaves, q, fm, s);
+    case 4: return synth_nfk<NF, 4>(T, nfield, t.nwaves, q, fm, s);
+    case 8: return synth_nfk<NF, 8>(T, nfield, t.nwaves, q, fm, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
 template <int NF>
-hipError_t analys_nf(const LegTables& T, int nfield, const float2* fm, const Fields& q,
-                     hipStream_t s) {
-  int nw;
-  switch (pick_k(T.npair, nw)) {
-    case 1: return analys_nfk<NF, 1>(T, nfield, nw, fm, q, s);
-    case 2: return analys_nfk<NF, 2>(T, nfield, nw, fm, q, s);
-    case 4: return analys_nfk<NF, 4>(T, nfield, nw, fm, q, s);
-    default: return analys_nfk<NF, 8>(T, nfield, nw, fm, q, s);
+hipError_t analys_nf(const LegTables& T, int nfield, const LegTile& t, const float2* fm,
+                     const Fields& q, hipStream_t s) {
+  switch (t.k) {
+    case 1: return analys_nfk<NF, 1>(T, nfield, t.nwaves, fm, q, s);
+    case 2: return analys_nfk<NF, 2>(T, nfield, t.nwaves, fm, q, s);
+    case 4: return analys_nfk<NF, 4>(T, nfield, t.nwaves, fm, q, s);
+    case 8: return analys_nfk<NF, 8>(T, nfield, t.nwaves, fm, q, s);
+    default: return hipErrorInvalidValue;
   }
+}
+
+int round_k(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 0; }
+
+}  // namespace
+
+int mfma_col_tiles(int nfield);
+
+LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile t) {
+  if (legendre == TSPHERE_LEG_BF16X3) {
+    const int ct = mfma_col_tiles(nfield);
+    if (t.ctc == 0)
+      t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
+    return t;
+  }
+  if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
+  if (t.nwaves == 0 && t.k == 0) t.nwaves = std::min(4, (npair + 63) / 64);
+  if (t.nwaves == 0) t.nwaves = std::min(4, (npair + 64 * t.k - 1) / (64 * t.k));
+  if (t.k == 0) t.k = round_k((npair + 64 * t.nwaves - 1) / (64 * t.nwaves));
+  return t;
+}
+
+namespace {
+
+bool fp32_tile_ok(const LegTile& t, int npair) {
+  return (t.nf == 1 || t.nf == 2 || t.nf == 4) && t.nwaves >= 1 && t.nwaves <= 4 &&
+         round_k(t.k) == t.k && 64 * t.nwaves * t.k >= npair;
 }
 
 }  // namespace
 
 hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, float2* fm,
-                            hipStream_t s) {
+                            LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  switch (nfield) {
-    case 1: return synth_nf<1>(T, nfield, q, fm, s);
-    case 2: return synth_nf<2>(T, nfield, q, fm, s);
-    default: return synth_nf<4>(T, nfield, q, fm, s);
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return synth_nf<1>(T, nfield, t, q, fm, s);
+    case 2: return synth_nf<2>(T, nfield, t, q, fm, s);
+    default: return synth_nf<4>(T, nfield, t, q, fm, s);
   }
 }
 
 hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
-                             hipStream_t s) {
+                             LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  if (nfield == 1) return analys_nf<1>(T, nfield, fm, q, s);
-  if (nfield == 2) return analys_nf<2>(T, nfield, fm, q, s);
-  return analys_nf<4>(T, nfield, fm, q, s);
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return analys_nf<1>(T, nfield, t, fm, q, s);
+    case 2: return analys_nf<2>(T, nfield, t, fm, q, s);
+    default: return analys_nf<4>(T, nfield, t, fm, q, s);
+  }
 }
 
 }  // namespace tsphere

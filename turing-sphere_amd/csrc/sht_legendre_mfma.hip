@@ -114,8 +114,9 @@ __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfiel
   const int g = lane >> 4, r16 = lane & 15;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
-    if (r == 1 && m <= (int)blockIdx.x) break;
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
     const int n = L - m + 1;
     const long long base = lm_base(L, m);
     __syncthreads();
@@ -228,14 +229,15 @@ __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfiel
         const int col = (ct0 + t) * 16 + r16;
         const int f = col >> 1;
         if (f >= nfield) continue;
-        float* out = reinterpret_cast<float*>(fm + ((long long)f * (M + 1) + m) * nlat);
+        // fm row-major per latitude: complex (f, ilat, m) at (f nlat + ilat)(M+1) + m
+        float* out = reinterpret_cast<float*>(fm + (long long)f * nlat * (M + 1) + m) + (col & 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int j = pair0 + 16 * rt + 4 * g + q;
           if (j >= T.npair) continue;
           const float e = accE[rt][t][q], o = accO[rt][t][q];
-          out[2 * j + (col & 1)] = e + o;
-          out[2 * (nlat - 1 - j) + (col & 1)] = e - o;
+          out[2LL * j * (M + 1)] = e + o;
+          out[2LL * (nlat - 1 - j) * (M + 1)] = e - o;
         }
       }
   }
@@ -267,8 +269,9 @@ __global__ __launch_bounds__(256) void k_leg_analys_mfma(LegTables64 T, int nfie
   const int g = lane >> 4, r16 = lane & 15;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = r == 0 ? (int)blockIdx.x : M - (int)blockIdx.x;
-    if (r == 1 && m <= (int)blockIdx.x) break;
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
     const int n = L - m + 1;
     const long long base = lm_base(L, m);
     __syncthreads();
@@ -295,8 +298,8 @@ __global__ __launch_bounds__(256) void k_leg_analys_mfma(LegTables64 T, int nfie
         const int col = (ct0 + t) * 16 + r16;
         const int f = col >> 1;
         const bool live = f < nfield;
-        const float* gg =
-            reinterpret_cast<const float*>(fm + ((long long)(live ? f : 0) * (M + 1) + m) * nlat);
+        const float* gg = reinterpret_cast<const float*>(
+                              fm + (long long)(live ? f : 0) * nlat * (M + 1) + m) + (col & 1);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           const int j = pair0 + 32 * kb + 8 * g + s;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(256) void k_leg_analys_mfma(LegTables64 T, int nfie
           if (live && j < T.npair) {
             const int js = nlat - 1 - j;
             const float w = T.w[j];
-            const float gN = gg[2 * j + (col & 1)], gS = gg[2 * js + (col & 1)];
+            const float gN = gg[2LL * j * (M + 1)], gS = gg[2LL * js * (M + 1)];
             if (js == j) {
               vp = gN * w;
             } else {
@@ -429,23 +432,27 @@ hipError_t analys_kb(const LegTables64& T, int nfield, const float2* fm, const F
 }  // namespace
 
 // CTC (column tiles per block) is the tile-shape knob of the synthesis.
+int mfma_col_tiles(int nfield) { return (2 * nfield + 15) / 16; }
+
 hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields& q, float2* fm,
-                                 int ctc, hipStream_t s) {
+                                 LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  switch (ctc) {
+  switch (resolve_leg_tile(TSPHERE_LEG_BF16X3, 0, T.npair, nfield, tile).ctc) {
     case 1: return synth_ctc<1>(T, nfield, q, fm, s);
     case 2: return synth_ctc<2>(T, nfield, q, fm, s);
-    default: return synth_ctc<4>(T, nfield, q, fm, s);
+    case 4: return synth_ctc<4>(T, nfield, q, fm, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
-                                  const Fields& q, int ctc, hipStream_t s) {
+                                  const Fields& q, LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   // pairs per wave: 4 waves cover npair, rounded up to 64-pair multiples
   const int pw = ((T.npair + 3) / 4 + 63) / 64 * 64;
   const int kb = pw / 32;
-  if (ctc >= 2) {
+  const int ctc = resolve_leg_tile(TSPHERE_LEG_BF16X3, 1, T.npair, nfield, tile).ctc;
+  if (ctc == 2) {
     switch (kb) {
       case 2: return analys_kb<2, 2>(T, nfield, fm, q, s);
       case 4: return analys_kb<4, 2>(T, nfield, fm, q, s);
@@ -454,6 +461,7 @@ hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2
       default: return hipErrorInvalidValue;
     }
   }
+  if (ctc != 1) return hipErrorInvalidValue;
   switch (kb) {
     case 2: return analys_kb<2, 1>(T, nfield, fm, q, s);
     case 4: return analys_kb<4, 1>(T, nfield, fm, q, s);

@@ -56,6 +56,14 @@ struct LegTables64 {
   int lmax, mmax, nlat, npair;
 };
 
+// Tile shape of a Legendre launch (0 = the built-in choice for that entry).
+//  FP32 kernels: nf fields per workgroup (1, 2, 4), nwaves waves, k latitude
+//  pairs per lane (1, 2, 4, 8; nwaves * 64 * k must cover the pairs).
+//  MFMA kernels: ctc column tiles (16 columns each) per workgroup.
+struct LegTile {
+  int nf, nwaves, k, ctc;
+};
+
 struct Plan {
   tsphere_sht_config cfg{};
   int fourier = TSPHERE_FOURIER_FFT;
@@ -64,37 +72,57 @@ struct Plan {
   float2* d_ab = nullptr;
   float2* d_ct = nullptr;
   float* d_w = nullptr;
+  float* d_ones = nullptr;  // [nlat] of 1.0: the unit weights of the synthesis transpose
   SeedL* d_seed = nullptr;
   double2* d_ab64 = nullptr;   // MFMA path only
   double* d_ct64 = nullptr;
   SeedD* d_seed64 = nullptr;
-  int ctc_synth = 4, ctc_analys = 1;  // MFMA column tiles per block (tunable)
+  LegTile tile[2] = {};  // [synthesis, analysis]; zeros = built-in heuristic
   float2* d_trig = nullptr;
   float2* d_fm = nullptr;  // Fourier scratch, max_fields * (mmax+1) * nlat
   size_t bytes = 0;
   std::vector<double> theta, cos_theta, weights;
-  LegTables tables() const {
-    return LegTables{d_ab, d_ct, d_w, d_seed, cfg.lmax, cfg.mmax, cfg.nlat, npair};
+  LegTables tables(bool unit_w = false) const {
+    return LegTables{d_ab, d_ct, unit_w ? d_ones : d_w, d_seed, cfg.lmax, cfg.mmax, cfg.nlat,
+                     npair};
   }
-  LegTables64 tables64() const {
-    return LegTables64{d_ab64, d_ct64, d_w, d_seed64, cfg.lmax, cfg.mmax, cfg.nlat, npair};
+  LegTables64 tables64(bool unit_w = false) const {
+    return LegTables64{d_ab64,    d_ct64,   unit_w ? d_ones : d_w, d_seed64, cfg.lmax,
+                       cfg.mmax, cfg.nlat, npair};
   }
 };
 
-// Where the fields of a batched call live: either up to kMaxList explicit
-// pointers (stride == 0) or base p[0] plus f * stride bytes.  Passed by value
-// in the kernel arguments; `f` is wave-uniform, so the lookup is scalar.
+// Where the fields of a batched call live: field f is at
+// p[f / per] + (f % per) * stride bytes -- a pointer list (stride 0, per 1),
+// one strided base (per = INT_MAX), or several strided bases (an ensemble's
+// state variables, each holding `per` runs).  Passed by value in the kernel
+// arguments; `f` is wave-uniform, so the lookup is scalar.
 constexpr int kMaxList = 8;
 struct Fields {
   const char* p[kMaxList];
-  long long stride;  // bytes; 0 = use the pointer list
-  __host__ __device__ const char* at(int f) const { return stride ? p[0] + f * stride : p[f]; }
+  long long stride;  // bytes
+  int per;           // fields per pointer
+  __host__ __device__ const char* at(int f) const {
+    return per <= 1 ? p[f] : p[f / per] + (long long)(f % per) * stride;
+  }
 };
 inline Fields strided(const void* base, long long stride_bytes) {
   Fields F{};
   F.p[0] = static_cast<const char*>(base);
   F.stride = stride_bytes;
+  F.per = 0x7fffffff;
   return F;
+}
+
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8, each XCD with its own L2).  This maps b to a contiguous range
+// per XCD, so neighbouring m (which share the cache lines of the latitude
+// rows of the Fourier scratch) are handled under one L2.  A bijection on
+// [0, nb) for any nb.
+__device__ __forceinline__ int xcd_chunk(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Host-side table construction (no device needed).
@@ -117,18 +145,31 @@ long long nlm_calc(int lmax, int mmax);
 // Spectral fields (Q, 2*nlm floats each) and spatial fields (nlat*nphi
 // floats each) are addressed through Fields; the Fourier scratch fm is the
 // plan's, contiguous per field.
+// `tile` fields left 0 take the launcher's own choice; an invalid shape
+// returns hipErrorInvalidValue without launching.
 hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, float2* fm,
-                            hipStream_t s);
+                            LegTile tile, hipStream_t s);
 hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, const Fields& q,
-                             hipStream_t s);
+                             LegTile tile, hipStream_t s);
 hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields& q, float2* fm,
-                                 int ctc, hipStream_t s);
+                                 LegTile tile, hipStream_t s);
 hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
-                                  const Fields& q, int ctc, hipStream_t s);
+                                  const Fields& q, LegTile tile, hipStream_t s);
+// The shape a launch of `nfield` fields would use (tile fields resolved).
+LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile tile);
+// Scaling hooks of the Fourier stage (identity by default): mscale multiplies
+// every m >= 1 coefficient, rowscale (if set) each synthesised latitude row.
+struct FourierScale {
+  float mscale = 1.f;
+  const float* rowscale = nullptr;
+};
 hipError_t launch_fourier_synth(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                int nfield, const float2* fm, const Fields& spat, hipStream_t s);
+                                int nfield, const float2* fm, const Fields& spat, hipStream_t s,
+                                FourierScale sc = {});
 hipError_t launch_fourier_analys(int fourier, int mmax, int nlat, int nphi, const float2* trig,
-                                 int nfield, const Fields& spat, float2* fm, hipStream_t s);
+                                 int nfield, const Fields& spat, float2* fm, hipStream_t s,
+                                 FourierScale sc = {});
+hipError_t launch_accumulate(float* dst, const float* src, long long n, hipStream_t s);
 hipError_t launch_fill(float* dst, float v, long long n, hipStream_t s);
 
 }  // namespace tsphere

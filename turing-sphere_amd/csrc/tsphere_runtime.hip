@@ -324,6 +324,12 @@ int tsphere_event_destroy(tsphere_event e) {
   return ok();
 }
 
+int tsphere_stream_wait(tsphere_stream s, tsphere_event e) {
+  if (!e) return fail(TSPHERE_ERR_INVALID, "event is NULL");
+  TS_HIP(hipStreamWaitEvent(S(s), e->e, 0), "hipStreamWaitEvent");
+  return ok();
+}
+
 int tsphere_event_record(tsphere_event e, tsphere_stream s) {
   if (!e) return fail(TSPHERE_ERR_INVALID, "event is NULL");
   TS_HIP(hipEventRecord(e->e, S(s)), "hipEventRecord");
@@ -396,7 +402,7 @@ int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, fl
 
 static void plan_free(Plan* p) {
   if (!p) return;
-  for (void* ptr : {(void*)p->d_ab, (void*)p->d_ct, (void*)p->d_w, (void*)p->d_seed,
+  for (void* ptr : {(void*)p->d_ab, (void*)p->d_ct, (void*)p->d_w, (void*)p->d_ones, (void*)p->d_seed,
                     (void*)p->d_ab64, (void*)p->d_ct64, (void*)p->d_seed64, (void*)p->d_trig,
                     (void*)p->d_fm})
     if (ptr) (void)hipFree(ptr);
@@ -450,6 +456,8 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   for (int j = 0; j < p->npair; j++) ct2[j] = make_float2(t.ct[j], t.ct_lo[j]);
   if (e == hipSuccess) e = upload(&p->d_ct, ct2.data(), (size_t)p->npair);
   if (e == hipSuccess) e = upload(&p->d_w, t.w.data(), (size_t)cfg->nlat);
+  const std::vector<float> ones(cfg->nlat, 1.0f);
+  if (e == hipSuccess) e = upload(&p->d_ones, ones.data(), (size_t)cfg->nlat);
   if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
   size_t bytes64 = 0;
   if (cfg->legendre == TSPHERE_LEG_BF16X3) {
@@ -460,10 +468,6 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     if (e == hipSuccess) e = upload(&p->d_seed64, t.seed64.data(), t.seed64.size());
     bytes64 = sizeof(double2) * (p->nlm + kAbPad) + sizeof(double) * p->npair +
               sizeof(SeedD) * t.seed64.size();
-    // column tiles per synthesis block: the batched fields' columns, up to 4
-    const int ct = (2 * cfg->max_fields + 15) / 16;
-    p->ctc_synth = ct >= 4 ? 4 : ct >= 2 ? 2 : 1;
-    p->ctc_analys = ct >= 2 ? 2 : 1;
   }
   if (e == hipSuccess)
     e = upload(&p->d_trig, reinterpret_cast<const float2*>(t.trig.data()), (size_t)cfg->nphi);
@@ -473,7 +477,7 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     delete p;
     return hip_fail(e, "tsphere_plan_create");
   }
-  p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + sizeof(float) * cfg->nlat +
+  p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + 2 * sizeof(float) * cfg->nlat +
              sizeof(SeedL) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm +
              bytes64;
   *out = p;
@@ -545,20 +549,21 @@ static Fields listed(const void* const* v, int n) {
   Fields F{};
   for (int f = 0; f < n; f++) F.p[f] = static_cast<const char*>(v[f]);
   F.stride = 0;
+  F.per = 1;
   return F;
 }
 
 static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hipStream_t s) {
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
-    return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->ctc_synth, s);
-  return launch_leg_synth(p->tables(), nfield, q, fm, s);
+    return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->tile[0], s);
+  return launch_leg_synth(p->tables(), nfield, q, fm, p->tile[0], s);
 }
 
 static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, const Fields& q,
-                             hipStream_t s) {
+                             hipStream_t s, bool unit_w = false) {
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
-    return launch_leg_analys_mfma(p->tables64(), nfield, fm, q, p->ctc_analys, s);
-  return launch_leg_analys(p->tables(), nfield, fm, q, s);
+    return launch_leg_analys_mfma(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
+  return launch_leg_analys(p->tables(unit_w), nfield, fm, q, p->tile[1], s);
 }
 
 static int run_synth(Plan* p, int nfield, const Fields& q, const Fields& spat, hipStream_t s) {
@@ -657,6 +662,235 @@ int tsphere_analys_multi(tsphere_plan p, int nfield, const float* const* spat, f
   if (int r = check_list(p, nfield, (const void* const*)spat, (const void* const*)qlm)) return r;
   return run_analys(p, nfield, listed((const void* const*)spat, nfield),
                     listed((const void* const*)qlm, nfield), S(s));
+}
+
+static int check_fields(tsphere_plan p, int nptr, int per, const void* const* a,
+                        const void* const* b) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nptr < 1 || nptr > kMaxList || per < 1)
+    return fail(TSPHERE_ERR_INVALID, "nptr=%d per=%d: need 1 <= nptr <= %d, per >= 1", nptr, per,
+                kMaxList);
+  if ((long long)nptr * per > p->cfg.max_fields)
+    return fail(TSPHERE_ERR_INVALID, "nptr*per=%lld exceeds max_fields=%d",
+                (long long)nptr * per, p->cfg.max_fields);
+  if (!a || !b) return fail(TSPHERE_ERR_INVALID, "NULL pointer list");
+  for (int f = 0; f < nptr; f++)
+    if (!a[f] || !b[f]) return fail(TSPHERE_ERR_INVALID, "NULL buffer for pointer %d", f);
+  return TSPHERE_OK;
+}
+
+static Fields grouped(const void* const* v, int nptr, int per, long long stride_floats) {
+  Fields F{};
+  for (int f = 0; f < nptr; f++) F.p[f] = static_cast<const char*>(v[f]);
+  F.stride = 4 * stride_floats;
+  F.per = per;
+  return F;
+}
+
+int tsphere_synth_fields(tsphere_plan p, int nptr, int per, const float* const* qlm,
+                         long long qlm_stride, float* const* spat, long long spat_stride,
+                         tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)qlm, (const void* const*)spat))
+    return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, per, "spat_stride"))
+    return r;
+  return run_synth(p, nptr * per, grouped((const void* const*)qlm, nptr, per, qlm_stride),
+                   grouped((const void* const*)spat, nptr, per, spat_stride), S(s));
+}
+
+int tsphere_analys_fields(tsphere_plan p, int nptr, int per, const float* const* spat,
+                          long long spat_stride, float* const* qlm, long long qlm_stride,
+                          tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)spat, (const void* const*)qlm))
+    return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, per, "spat_stride"))
+    return r;
+  return run_analys(p, nptr * per, grouped((const void* const*)spat, nptr, per, spat_stride),
+                    grouped((const void* const*)qlm, nptr, per, qlm_stride), S(s));
+}
+
+// ------------------------------------------------------- transposes
+// synth^T: G = 2^{[m>=1]} * FourierAnalysis(g), q = sum_i P_lm(theta_i) G_m(theta_i)
+// analys^T: F = LegendreSynth(q), g_ij = w_i * FourierSynth(F with m>=1 halved)
+int tsphere_synth_adjoint(tsphere_plan p, int nptr, int per, const float* const* gbar,
+                          long long g_stride, float* const* qbar, long long q_stride,
+                          tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)gbar, (const void* const*)qbar))
+    return r;
+  if (int r = check_stride(q_stride, 2 * p->nlm, per, "q_stride")) return r;
+  if (int r = check_stride(g_stride, (long long)p->cfg.nlat * p->cfg.nphi, per, "g_stride"))
+    return r;
+  const int nf = nptr * per;
+  FourierScale sc;
+  sc.mscale = 2.f;
+  TS_HIP(launch_fourier_analys(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig, nf,
+                               grouped((const void* const*)gbar, nptr, per, g_stride), p->d_fm,
+                               S(s), sc),
+         "fourier_analys (transpose)");
+  TS_HIP(leg_analys(p, nf, p->d_fm, grouped((const void* const*)qbar, nptr, per, q_stride), S(s),
+                    true),
+         "leg_analys (transpose)");
+  return ok();
+}
+
+int tsphere_analys_adjoint(tsphere_plan p, int nptr, int per, const float* const* qbar,
+                           long long q_stride, float* const* gbar, long long g_stride,
+                           tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)qbar, (const void* const*)gbar))
+    return r;
+  if (int r = check_stride(q_stride, 2 * p->nlm, per, "q_stride")) return r;
+  if (int r = check_stride(g_stride, (long long)p->cfg.nlat * p->cfg.nphi, per, "g_stride"))
+    return r;
+  const int nf = nptr * per;
+  TS_HIP(leg_synth(p, nf, grouped((const void* const*)qbar, nptr, per, q_stride), p->d_fm, S(s)),
+         "leg_synth (transpose)");
+  FourierScale sc;
+  sc.mscale = 0.5f;
+  sc.rowscale = p->d_w;
+  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig, nf,
+                              p->d_fm, grouped((const void* const*)gbar, nptr, per, g_stride),
+                              S(s), sc),
+         "fourier_synth (transpose)");
+  return ok();
+}
+
+int tsphere_accumulate(float* dst, const float* src, long long n, tsphere_stream s) {
+  if (n < 0) return fail(TSPHERE_ERR_INVALID, "n must be >= 0");
+  if (n > 0 && (!dst || !src)) return fail(TSPHERE_ERR_INVALID, "NULL buffer");
+  TS_HIP(launch_accumulate(dst, src, n, S(s)), "accumulate");
+  return ok();
+}
+
+// ------------------------------------------------------------ tile tuning
+static LegTile to_tile(const tsphere_leg_tile& t) { return LegTile{t.nf, t.nwaves, t.k, t.ctc}; }
+static tsphere_leg_tile from_tile(const LegTile& t) {
+  tsphere_leg_tile o{};
+  o.nf = t.nf;
+  o.nwaves = t.nwaves;
+  o.k = t.k;
+  o.ctc = t.ctc;
+  return o;
+}
+
+int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* tile) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
+  const LegTile t = tile ? to_tile(*tile) : LegTile{};
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0)
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0");
+  // validate against a one-field and a max-fields launch
+  for (int nf : {1, p->cfg.max_fields}) {
+    const LegTile r = resolve_leg_tile(p->cfg.legendre, stage, p->npair, nf, t);
+    if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
+      if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
+        return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
+    } else {
+      const bool kok = r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8;
+      if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.nwaves < 1 || r.nwaves > 4 || !kok ||
+          64LL * r.nwaves * r.k < p->npair)
+        return fail(TSPHERE_ERR_INVALID,
+                    "FP32 tile nf=%d nwaves=%d k=%d: need nf in {1,2,4}, nwaves in 1..4, "
+                    "k in {1,2,4,8}, 64*nwaves*k >= %d latitude pairs",
+                    r.nf, r.nwaves, r.k, p->npair);
+    }
+  }
+  p->tile[stage] = t;
+  return ok();
+}
+
+int tsphere_plan_get_tile(tsphere_plan p, int stage, int nfield, tsphere_leg_tile* out) {
+  if (!p || !out) return fail(TSPHERE_ERR_INVALID, "plan_get_tile: NULL argument");
+  if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
+  if (nfield < 1) return fail(TSPHERE_ERR_INVALID, "nfield must be >= 1");
+  *out = from_tile(resolve_leg_tile(p->cfg.legendre, stage, p->npair, nfield, p->tile[stage]));
+  return ok();
+}
+
+// Candidate tiles of one stage: FP32 -- every fields-per-block choice with
+// every wave count at the smallest k that covers the pairs; MFMA -- the
+// column tiles per block.
+static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield) {
+  std::vector<LegTile> c;
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
+    const int ct = (2 * nfield + 15) / 16;
+    for (int ctc : {1, 2, 4})
+      if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc});
+    return c;
+  }
+  for (int nf : {1, 2, 4}) {
+    if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
+    for (int nw = 1; nw <= 4; nw++) {
+      int k = 1;
+      while (k <= 8 && 64 * nw * k < p->npair) k *= 2;
+      if (k > 8) continue;
+      if (nw > 1 && 64 * (nw - 1) * k >= p->npair) continue;  // a wave with no pairs
+      c.push_back(LegTile{nf, nw, k, 0});
+    }
+  }
+  return c;
+}
+
+int tsphere_plan_tune(tsphere_plan p, int nfield, int reps, tsphere_stream stream,
+                      tsphere_leg_tile* best, float* best_ms) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1 || nfield > p->cfg.max_fields)
+    return fail(TSPHERE_ERR_INVALID, "nfield=%d outside [1, max_fields=%d]", nfield,
+                p->cfg.max_fields);
+  if (reps < 1) reps = 1;
+  hipStream_t s = S(stream);
+  const long long qn = 2 * p->nlm;
+  float* q = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&q, sizeof(float) * qn * nfield);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_fill(q, 0.01f, qn * nfield, s);
+  if (e == hipSuccess) e = launch_fill(reinterpret_cast<float*>(p->d_fm), 0.01f,
+                                       2LL * nfield * (p->cfg.mmax + 1) * p->cfg.nlat, s);
+  const Fields F = strided(q, 4 * qn);
+  int rc = TSPHERE_OK;
+  for (int stage = 0; stage < 2 && e == hipSuccess; stage++) {
+    LegTile chosen{};
+    float chosen_ms = 0.f;
+    for (const LegTile& t : tile_candidates(p, stage, nfield)) {
+      auto launch = [&]() -> hipError_t {
+        if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+          return stage == 0 ? launch_leg_synth_mfma(p->tables64(), nfield, F, p->d_fm, t, s)
+                            : launch_leg_analys_mfma(p->tables64(), nfield, p->d_fm, F, t, s);
+        return stage == 0 ? launch_leg_synth(p->tables(), nfield, F, p->d_fm, t, s)
+                          : launch_leg_analys(p->tables(), nfield, p->d_fm, F, t, s);
+      };
+      if ((e = launch()) != hipSuccess) break;  // warm-up (and code load)
+      if ((e = hipEventRecord(e0, s)) != hipSuccess) break;
+      for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) break;
+      ms /= reps;
+      if (chosen_ms == 0.f || ms < chosen_ms) {
+        chosen = t;
+        chosen_ms = ms;
+      }
+    }
+    if (e != hipSuccess) break;
+    p->tile[stage] = chosen;
+    if (best) best[stage] = from_tile(resolve_leg_tile(p->cfg.legendre, stage, p->npair, nfield,
+                                                       chosen));
+    if (best_ms) best_ms[stage] = chosen_ms;
+  }
+  if (e != hipSuccess) rc = hip_fail(e, "tsphere_plan_tune");
+  // cleanup: the first error (if any) is already recorded
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (q) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(q);
+  }
+  return rc == TSPHERE_OK ? ok() : rc;
 }
 
 int tsphere_synth(tsphere_plan p, const float* qlm, float* spat, tsphere_stream s) {

@@ -43,6 +43,10 @@ class PlanInfoC(C.Structure):
                 ("leg_lat", C.c_int), ("max_fields", C.c_int), ("scratch_bytes", C.c_size_t)]
 
 
+class LegTileC(C.Structure):
+    _fields_ = [("nf", C.c_int), ("nwaves", C.c_int), ("k", C.c_int), ("ctc", C.c_int)]
+
+
 _vp = C.c_void_p
 _ll = C.c_longlong
 _i = C.c_int
@@ -68,6 +72,7 @@ SIGNATURES = {
     "tsphere_event_destroy": [_vp],
     "tsphere_event_record": [_vp, _vp],
     "tsphere_event_elapsed": [_vp, _vp, C.POINTER(C.c_float)],
+    "tsphere_stream_wait": [_vp, _vp],
     "tsphere_plan_create": [C.POINTER(ShtConfigC), C.POINTER(_vp)],
     "tsphere_plan_destroy": [_vp],
     "tsphere_plan_query": [_vp, C.POINTER(PlanInfoC)],
@@ -79,6 +84,14 @@ SIGNATURES = {
     "tsphere_analys_batch": [_vp, _i, _vp, _ll, _vp, _ll, _vp],
     "tsphere_synth_multi": [_vp, _i, C.POINTER(_vp), C.POINTER(_vp), _vp],
     "tsphere_analys_multi": [_vp, _i, C.POINTER(_vp), C.POINTER(_vp), _vp],
+    "tsphere_synth_fields": [_vp, _i, _i, C.POINTER(_vp), _ll, C.POINTER(_vp), _ll, _vp],
+    "tsphere_analys_fields": [_vp, _i, _i, C.POINTER(_vp), _ll, C.POINTER(_vp), _ll, _vp],
+    "tsphere_synth_adjoint": [_vp, _i, _i, C.POINTER(_vp), _ll, C.POINTER(_vp), _ll, _vp],
+    "tsphere_analys_adjoint": [_vp, _i, _i, C.POINTER(_vp), _ll, C.POINTER(_vp), _ll, _vp],
+    "tsphere_accumulate": [_vp, _vp, _ll, _vp],
+    "tsphere_plan_set_tile": [_vp, _i, C.POINTER(LegTileC)],
+    "tsphere_plan_get_tile": [_vp, _i, _i, C.POINTER(LegTileC)],
+    "tsphere_plan_tune": [_vp, _i, _i, _vp, C.POINTER(LegTileC), _fp],
     "tsphere_leg_synth": [_vp, _i, _vp, _ll, _vp, _vp],
     "tsphere_fourier_synth": [_vp, _i, _vp, _vp, _ll, _vp],
     "tsphere_fourier_analys": [_vp, _i, _vp, _ll, _vp, _vp],

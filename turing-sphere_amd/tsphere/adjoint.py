@@ -1,0 +1,502 @@
+"""Reverse-mode gradients through IMEX steps of a compiled .m model.
+
+No reference counterpart: turing-sphere only runs models forward.  The step
+function is compiled as for the forward run (src/mgpu/plan.ts), but as a
+*tape*: every value keeps its own buffer (no in-place feedback), so after
+one step all of the step's intermediates are on the device.  Reverse mode
+then walks the op list backwards:
+
+* element-wise statement ``out = f(in..., p...)``: one generated HIP kernel
+  per statement adds ``out_bar * df/din`` to each input's cotangent and
+  reduces ``out_bar * df/dp`` per workgroup into a per-parameter partial
+  slot (deterministic: no atomics; the slots are summed in f64 at the end).
+  The derivatives are taken symbolically from the statement's IR tree;
+* ``x = synth(y)``: ``y_bar += S^T x_bar`` (``tsphere_synth_adjoint``);
+* ``x = analys(y)``: ``y_bar += A^T x_bar`` (``tsphere_analys_adjoint``).
+
+Trajectories are checkpointed: the forward pass keeps the state every
+``checkpoint_every`` steps; the backward pass re-runs each segment from its
+checkpoint, keeping the segment's step inputs, and differentiates its steps
+in reverse, re-running each step once more to regenerate its intermediates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceBuffer, Stream
+from .hipgen import CALL_FNS, UnsupportedOnGpu, _Ctx, _emit, f32_lit, module_source
+from .ir import is_numeric, is_tensor
+from .layout import ShtConfig
+from .mfront import CompiledModel, model_bindings
+from .model import eigenvalues
+from .plan import HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
+
+WG = 256
+
+
+# ------------------------------------------------------ symbolic derivative
+def _add(a, b):
+    if a is None:
+        return b
+    if b is None:
+        return a
+    return f"({a} + {b})"
+
+
+def _sub(a, b):
+    if b is None:
+        return a
+    if a is None:
+        return f"(-{b})"
+    return f"({a} - {b})"
+
+
+def _mul(a, b):
+    if a is None or b is None:
+        return None
+    if a == "1.0f":
+        return b
+    if b == "1.0f":
+        return a
+    return f"({a} * {b})"
+
+
+def _div(a, b):
+    if a is None:
+        return None
+    return f"({a} / {b})"
+
+
+def derivative(e, wrt: str, ctx: _Ctx):
+    """HIP code for d(e)/d(wrt) (None = identically zero).  `wrt` is a tensor
+    c_name or a runtime-parameter c_name; derived scalars are differentiated
+    through their defining expressions."""
+    io = ctx.io
+    k = e.kind
+    if k == "NumLit":
+        return None
+    if k == "Var":
+        if is_tensor(e.ty):
+            return "1.0f" if e.c_name == wrt else None
+        if is_numeric(e.ty) and isinstance(e.ty.exact, (int, float)):
+            return None
+        if e.c_name in io.params:
+            return "1.0f" if e.c_name == wrt else None
+        if e.c_name in io.scalars:
+            return derivative(io.scalars[e.c_name][1], wrt, ctx)
+        raise UnsupportedOnGpu(f"cannot differentiate '{e.name}'", e.span)
+    if k == "Unary":
+        d = derivative(e.operand, wrt, ctx)
+        if e.builtin == "uminus":
+            return None if d is None else f"(-{d})"
+        return d
+    if k == "Binary":
+        b = e.builtin
+        if b in ("power", "mpower"):
+            kv = e.right.value if e.right.kind == "NumLit" else e.right.ty.exact
+            kk = int(kv)
+            d = derivative(e.left, wrt, ctx)
+            if d is None or kk == 0:
+                return None
+            x = _emit(e.left, ctx)
+            if kk == 1:
+                return d
+            pw = x if kk - 1 == 1 else (f"pow_i{kk - 1}({x})" if kk - 1 > 0 else
+                                        f"(1.0f / pow_i{1 - kk}({x}))" if kk - 1 < -1 else
+                                        f"(1.0f / {x})")
+            return _mul(f"{f32_lit(float(kk))} * {pw}", d)
+        dl = derivative(e.left, wrt, ctx)
+        dr = derivative(e.right, wrt, ctx)
+        if b == "plus":
+            return _add(dl, dr)
+        if b == "minus":
+            return _sub(dl, dr)
+        if b in ("times", "mtimes"):
+            return _add(_mul(dl, _emit(e.right, ctx)), _mul(_emit(e.left, ctx), dr))
+        if b in ("rdivide", "mrdivide"):
+            r = _emit(e.right, ctx)
+            t1 = _div(dl, r)
+            t2 = None if dr is None else f"({_emit(e.left, ctx)} * {dr} / ({r} * {r}))"
+            return _sub(t1, t2)
+        raise UnsupportedOnGpu(f"cannot differentiate operator '{b}'", e.span)
+    if k == "Call":
+        if e.name in ("ones", "zeros", "sign", "floor", "ceil", "round"):
+            return None
+        args = [_emit(a, ctx) for a in e.args]
+        ds = [derivative(a, wrt, ctx) for a in e.args]
+        x = args[0] if args else None
+        d = ds[0] if ds else None
+        rules = {
+            "exp": lambda: _mul(f"expf({x})", d),
+            "log": lambda: _div(d, x),
+            "log2": lambda: _div(d, f"({x} * 0.6931471805599453f)"),
+            "sqrt": lambda: _div(d, f"(2.0f * sqrtf({x}))"),
+            "sin": lambda: _mul(f"cosf({x})", d),
+            "cos": lambda: _mul(f"(-sinf({x}))", d),
+            "tan": lambda: _div(d, f"(cosf({x}) * cosf({x}))"),
+            "sinh": lambda: _mul(f"coshf({x})", d),
+            "cosh": lambda: _mul(f"sinhf({x})", d),
+            "tanh": lambda: _mul(f"(1.0f - tanhf({x}) * tanhf({x}))", d),
+            "abs": lambda: _mul(f"ts_sign({x})", d),
+            "atan": lambda: _div(d, f"(1.0f + {x} * {x})"),
+            "asin": lambda: _div(d, f"sqrtf(1.0f - {x} * {x})"),
+            "acos": lambda: None if d is None else f"(-{d} / sqrtf(1.0f - {x} * {x}))",
+        }
+        if e.name in rules:
+            return rules[e.name]()
+        if e.name in ("max", "min"):
+            op = ">=" if e.name == "max" else "<="
+            a0 = ds[0] or "0.0f"
+            a1 = ds[1] or "0.0f"
+            if ds[0] is None and ds[1] is None:
+                return None
+            return f"(({args[0]} {op} {args[1]}) ? {a0} : {a1})"
+        if e.name == "atan2":
+            y, xx = args
+            dy, dx = ds
+            num = _sub(_mul(xx, dy), _mul(y, dx))
+            return _div(num, f"({xx} * {xx} + {y} * {y})")
+        raise UnsupportedOnGpu(f"cannot differentiate '{e.name}'", e.span)
+    raise UnsupportedOnGpu(f"cannot differentiate '{k}'", getattr(e, "span", None))
+
+
+def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: int):
+    """HIP source of the adjoint of one element-wise statement.
+
+    Signature: (long long n, const float* obar, const float* in0.., float*
+    bar_<k>.. (one per entry of `grad_inputs`, in that order), float*
+    partials, float p0..): bar_k[i] += obar[i] * d out / d in_k, and
+    partials[block * nparams + j] += sum over the block of obar * d out / d p_j.
+    """
+    io = op.io
+    ctx = _Ctx(io)
+    _emit(op.stmt.expr, ctx)  # binds derived scalars used by the value (prologue)
+    dins = []
+    for c in grad_inputs:
+        d = derivative(op.stmt.expr, c, ctx)
+        dins.append(d)
+    pnames = sorted(io.params, key=lambda c: io.params[c])
+    dps = [derivative(op.stmt.expr, c, ctx) for c in pnames]
+    nin = len(io.tensors)
+    loads = "\n".join(f"    const float a{i} = in{i}[i];" for i in range(nin))
+    pro = "\n".join("    " + line.strip() for line in ctx.prologue)
+    kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
+    kbar = "".join(f", float* __restrict__ bar{j}" for j in range(len(grad_inputs)))
+    kpar = "".join(f", float p{i}" for i in range(nparams))
+    upd = "\n".join(f"    bar{j}[i] += ob * {d};" for j, d in enumerate(dins) if d is not None)
+    acc = "\n".join(f"    c{j} = ob * {d};" for j, d in enumerate(dps) if d is not None)
+    np_ = max(1, nparams)
+    cdecl = " ".join(f"float c{j} = 0.0f;" for j in range(nparams))
+    live = [j for j, d in enumerate(dps) if d is not None]  # parameters this statement uses
+    red = ""
+    for j in live:
+        red += f"""  for (int o = 32; o > 0; o >>= 1) c{j} += __shfl_xor(c{j}, o);
+  if ((threadIdx.x & 63) == 0) red[{j}][threadIdx.x >> 6] = c{j};
+"""
+    fin = "\n".join(
+        f"    partials[(long long)blockIdx.x * {np_} + {j}] += "
+        f"(red[{j}][0] + red[{j}][1]) + (red[{j}][2] + red[{j}][3]);" for j in live)
+    used = {int(m) for m in __import__("re").findall(r"\bpow_i(\d+)\(", upd + acc + pro)}
+    helpers = "\n".join(
+        f"__device__ __forceinline__ float pow_i{k}(float x) {{ return {'x' + ' * x' * (k - 1)}; }}"
+        for k in sorted(used))
+    code = f"""{helpers}
+// adjoint of: {op.label}
+extern "C" __global__ __launch_bounds__({WG}) void {name}(long long n, const float* __restrict__ obar{kin}{kbar}, float* __restrict__ partials{kpar}) {{
+  __shared__ float red[{np_}][4];
+  const long long i = (long long)blockIdx.x * {WG} + threadIdx.x;
+  {cdecl}
+  if (i < n) {{
+    const float ob = obar[i];
+{loads}
+{pro}
+{upd}
+{acc}
+  }}
+{red}{"  __syncthreads();" if live else ""}
+  if ({"threadIdx.x == 0" if live else "false"}) {{
+{fin}
+  }}
+}}
+"""
+    return code
+
+
+# ------------------------------------------------------------- the engine
+class AdjointModel:
+    """Forward trajectories and vector-Jacobian products of a .m model's
+    step, on one spherical-harmonic plan."""
+
+    def __init__(self):
+        self.stream = Stream()
+        self._module = None
+        self._kernels = []
+        self._owned = []
+
+    @staticmethod
+    def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
+               checkpoint_every: int = 20, file_name: str = "model.m") -> "AdjointModel":
+        self = AdjointModel()
+        self.sht, self.cfg = sht, cfg
+        self.state, self.param_names = list(state), list(param_names)
+        self.checkpoint_every = max(1, int(checkpoint_every))
+        npts, nlm = cfg.nlat * cfg.nphi, sht.nlm
+        self.npts, self.nlm = npts, nlm
+        cm = CompiledModel(source, model_bindings(state, param_names, npts, nlm),
+                           {"npts": npts, "nlm": nlm}, file_name)
+        fn = cm.specialize("step", len(state) + len(view))
+        cm.finish()
+        assigned = [s.c_name for s in fn.body if s.kind == "Assign"]
+        if len(assigned) != len(set(assigned)):
+            raise UnsupportedOnGpu("reverse mode needs each value of the step assigned once")
+        host = HostBuffers()
+        host.ensure_group(self.state, 2 * nlm)
+        host.ensure("lam", 2 * nlm)
+        self._host = host
+        tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape")
+        self.tape = tape
+        host.upload("lam", eigenvalues(cfg, nlm), self.stream)
+        # state outputs: the value each state variable takes after the step
+        self.out_of = {}
+        for i, s in enumerate(self.state):
+            self.out_of[s] = fn.outputs[i].c_name
+        in_c = {p.name: p.c_name for p in fn.params}
+        self.in_of = {s: in_c[s] for s in self.state}
+        # cotangent buffers for every tensor value except host constants
+        const_c = {in_c[n] for n in ("lam", "noise") if n in in_c}
+        self._bar = {}
+        for c, b in tape._bufs.items():
+            if c in const_c:
+                continue
+            self._bar[c] = DeviceBuffer(b.count, f"bar-{c}")
+        self._tmp = {}
+        # adjoint kernels, in reverse op order
+        self._adj = []
+        srcs = []
+        self.param_slots = list(tape.param_names)
+        npar = len(self.param_slots)
+        for idx, op in enumerate(reversed(tape.ops)):
+            if isinstance(op, KernelOp):
+                gi = [c for c in op.ins if c in self._bar]
+                name = f"adj{idx}"
+                code = build_adjoint_kernel(op, name, gi, npar)
+                nblk = (op.count + WG - 1) // WG
+                part = DeviceBuffer(nblk * max(1, npar), f"partials-{name}")
+                self._owned.append(part)
+                self._adj.append(("kernel", op, name, gi, part, nblk))
+                srcs.append(code)
+            elif isinstance(op, XformOp):
+                for s_ in op.srcs:
+                    if s_ not in self._tmp:
+                        self._tmp[s_] = DeviceBuffer(tape._bufs[s_].count, f"tmp-{s_}")
+                self._adj.append((op.kind, op))
+            else:
+                raise UnsupportedOnGpu("unexpected op in the step tape")
+        pre = ("__device__ __forceinline__ float ts_sign(float x) "
+               "{ return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }\n")
+        self.adjoint_source = pre + "\n".join(_dedupe_helpers(srcs))
+        if srcs:
+            self._module = _compile(self.adjoint_source, "adjoint")
+            for i, a in enumerate(self._adj):
+                if a[0] == "kernel":
+                    h = C.c_void_p()
+                    _lib.call("tsphere_module_kernel", self._module, a[2].encode(), C.byref(h))
+                    self._kernels.append(h.value)
+                    self._adj[i] = a + (h.value,)
+        self.state_count = 2 * nlm * len(self.state)
+        self._params = np.zeros(max(1, npar), np.float32)
+        return self
+
+    # -------------------------------------------------------------- utils
+    def set_params(self, values: dict) -> None:
+        self.tape.set_params(values)
+        self._params[:len(self.param_slots)] = self.tape._params[:len(self.param_slots)]
+
+    def _state_buf(self, s):
+        return self._host.get(s)
+
+    def _load_state(self, src: DeviceBuffer, off: int = 0) -> None:
+        """Copy one state vector (all species) from `src` at float offset off."""
+        L = _lib.load()
+        n = 2 * self.nlm
+        for i, s in enumerate(self.state):
+            b = self._state_buf(s)
+            _lib.check(L.tsphere_memcpy_dtod(b.ptr, _off(src, off + i * n), 4 * n,
+                                             self.stream.handle), "load state")
+
+    def _store_state(self, dst: DeviceBuffer, off: int, from_outputs: bool) -> None:
+        L = _lib.load()
+        n = 2 * self.nlm
+        for i, s in enumerate(self.state):
+            b = self.tape._bufs[self.out_of[s]] if from_outputs else self._state_buf(s)
+            _lib.check(L.tsphere_memcpy_dtod(_off(dst, off + i * n), b.ptr, 4 * n,
+                                             self.stream.handle), "store state")
+
+    def _step_once(self) -> None:
+        self.tape.encode_steps(self.stream, 1)
+
+    def _advance(self) -> None:
+        """One step: state <- step(state)."""
+        self._step_once()
+        L = _lib.load()
+        n = 2 * self.nlm
+        for s in self.state:
+            _lib.check(L.tsphere_memcpy_dtod(self._state_buf(s).ptr,
+                                             self.tape._bufs[self.out_of[s]].ptr, 4 * n,
+                                             self.stream.handle), "advance")
+
+    # ------------------------------------------------------------ forward
+    def forward(self, state0: dict, steps: int) -> dict:
+        """Run `steps` steps from state0 (host arrays); returns the final state."""
+        for s in self.state:
+            self._state_buf(s).upload(np.asarray(state0[s], np.float32), self.stream)
+        for _ in range(steps):
+            self._advance()
+        return {s: self._state_buf(s).download(self.stream) for s in self.state}
+
+    # ------------------------------------------------------------ backward
+    def _zero_bars(self):
+        for b in self._bar.values():
+            b.fill(0.0, self.stream)
+
+    def _backward_step(self) -> None:
+        """With the step's intermediates on the device and the cotangents of
+        its outputs in their bar buffers, add the cotangents of its inputs."""
+        L = _lib.load()
+        sh = self.stream.handle
+        pv = self._params.ctypes.data_as(C.POINTER(C.c_float))
+        npar = len(self.param_slots)
+        nq, ns = 2 * self.nlm, self.npts
+        for a in self._adj:
+            if a[0] == "kernel":
+                _, op, name, gi, part, nblk, h = a
+                ptrs = ([self._bar[op.out].ptr] + [self.tape._bufs[c].ptr for c in op.ins] +
+                        [self._bar[c].ptr for c in gi] + [part.ptr])
+                arr = (C.c_void_p * len(ptrs))(*ptrs)
+                _lib.check(L.tsphere_kernel_launch(h, op.count, len(ptrs), arr, npar, pv, sh),
+                           f"adjoint of {op.label}")
+                continue
+            kind, op = a
+            n = len(op.srcs)
+            obar = (C.c_void_p * n)(*[self._bar[d].ptr for d in op.dsts])
+            tmp = (C.c_void_p * n)(*[self._tmp[s].ptr for s in op.srcs])
+            if kind == "synth":   # x = S y: y_bar += S^T x_bar
+                rc = L.tsphere_synth_adjoint(self.sht.handle, n, 1, obar, 0, tmp, 0, sh)
+            else:                 # x = A y: y_bar += A^T x_bar
+                rc = L.tsphere_analys_adjoint(self.sht.handle, n, 1, obar, 0, tmp, 0, sh)
+            _lib.check(rc, f"transpose of {op.label}")
+            for s_ in op.srcs:
+                if s_ in self._bar:
+                    _lib.check(L.tsphere_accumulate(self._bar[s_].ptr, self._tmp[s_].ptr,
+                                                    self._tmp[s_].count, sh), "accumulate")
+
+    def vjp(self, state0: dict, steps: int, cot: dict):
+        """Cotangents of the initial state and of the parameters for a
+        cotangent `cot` of the state after `steps` steps.  Returns
+        ({state: array}, {param: float})."""
+        n1 = self.state_count
+        C_ = self.checkpoint_every
+        nseg = max(1, math.ceil(steps / C_))
+        chk = DeviceBuffer(nseg * n1, "checkpoints")
+        seg = DeviceBuffer(C_ * n1, "segment-inputs")
+        carry = DeviceBuffer(n1, "state-cotangent")
+        try:
+            for s in self.state:
+                self._state_buf(s).upload(np.asarray(state0[s], np.float32), self.stream)
+            for k in range(steps):
+                if k % C_ == 0:
+                    self._store_state(chk, (k // C_) * n1, from_outputs=False)
+                self._advance()
+            carry.upload(np.concatenate([np.asarray(cot[s], np.float32).reshape(-1)
+                                         for s in self.state]), self.stream)
+            for a in self._adj:
+                if a[0] == "kernel":
+                    a[4].fill(0.0, self.stream)
+            L = _lib.load()
+            nq = 2 * self.nlm
+            for g in reversed(range(nseg)):
+                k0 = g * C_
+                k1 = min(steps, k0 + C_)
+                self._load_state(chk, g * n1)
+                for j in range(k1 - k0):
+                    self._store_state(seg, j * n1, from_outputs=False)
+                    if j < k1 - k0 - 1:
+                        self._advance()
+                for j in reversed(range(k1 - k0)):
+                    self._load_state(seg, j * n1)
+                    self._step_once()          # regenerate this step's intermediates
+                    self._zero_bars()
+                    for i, s in enumerate(self.state):
+                        _lib.check(L.tsphere_memcpy_dtod(
+                            self._bar[self.out_of[s]].ptr, _off(carry, i * nq), 4 * nq,
+                            self.stream.handle), "cotangent in")
+                    self._backward_step()
+                    for i, s in enumerate(self.state):
+                        _lib.check(L.tsphere_memcpy_dtod(
+                            _off(carry, i * nq), self._bar[self.in_of[s]].ptr, 4 * nq,
+                            self.stream.handle), "cotangent out")
+            out = carry.download(self.stream)
+            grads = {s: out[i * nq:(i + 1) * nq].copy() for i, s in enumerate(self.state)}
+            pg = np.zeros(len(self.param_slots))
+            for a in self._adj:
+                if a[0] == "kernel" and self.param_slots:
+                    pp = a[4].download(self.stream).astype(np.float64)
+                    pg += pp.reshape(-1, max(1, len(self.param_slots)))[:, :len(pg)].sum(0)
+            return grads, {n: float(pg[i]) for i, n in enumerate(self.param_slots)}
+        finally:
+            self.stream.sync()
+            chk.free()
+            seg.free()
+            carry.free()
+
+    # ------------------------------------------------------------ helpers
+    def synth_adjoint(self, gbar) -> np.ndarray:
+        """S^T gbar for one grid field (host in, host out)."""
+        g = DeviceBuffer(self.npts, "gbar")
+        q = DeviceBuffer(2 * self.nlm, "qbar")
+        try:
+            g.upload(np.asarray(gbar, np.float32), self.stream)
+            _lib.call("tsphere_synth_adjoint", self.sht.handle, 1, 1,
+                      (C.c_void_p * 1)(g.ptr), 0, (C.c_void_p * 1)(q.ptr), 0, self.stream.handle)
+            return q.download(self.stream)
+        finally:
+            g.free()
+            q.free()
+
+    def destroy(self) -> None:
+        self.stream.sync()
+        for h in self._kernels:
+            _lib.call("tsphere_kernel_destroy", h)
+        self._kernels = []
+        if self._module:
+            _lib.call("tsphere_module_destroy", self._module)
+            self._module = None
+        for b in list(self._bar.values()) + list(self._tmp.values()) + self._owned:
+            b.free()
+        self._bar, self._tmp, self._owned = {}, {}, []
+        self.tape.destroy()
+        self._host.destroy()
+
+
+def _off(buf: DeviceBuffer, floats: int):
+    return C.c_void_p(buf.ptr + 4 * floats)
+
+
+def _dedupe_helpers(srcs):
+    seen = set()
+    out = []
+    for code in srcs:
+        lines = []
+        for line in code.splitlines():
+            if line.startswith("__device__ __forceinline__ float pow_i"):
+                key = line.split("(")[0]
+                if key in seen:
+                    continue
+                seen.add(key)
+            lines.append(line)
+        out.append("\n".join(lines))
+    return out

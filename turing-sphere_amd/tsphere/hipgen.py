@@ -231,8 +231,15 @@ class Kernel:
     body: str = ""   # the per-element expression (for tests / describe)
 
 
-def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: str) -> Kernel:
-    """buildKernel (wgsl.ts:321-372): one element-wise Assign -> one kernel."""
+def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: str,
+                 batch: "Batch | None" = None) -> Kernel:
+    """buildKernel (wgsl.ts:321-372): one element-wise Assign -> one kernel.
+
+    With ``batch`` the kernel runs an ensemble: ``count`` elements per run,
+    ``batch.runs`` runs back to back in every per-run buffer, runtime scalars
+    read per run from a device array (last pointer argument, ``npar`` floats
+    per run), and inputs listed in ``batch.shared`` (one copy for all runs,
+    e.g. the eigenvalues) indexed modulo ``count``."""
     if not is_numeric(stmt.ty):
         raise UnsupportedOnGpu(f"'{stmt.name}' is not a numeric array", stmt.span)
     if stmt.ty.is_complex:
@@ -249,26 +256,28 @@ def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: s
     args = ["a" + str(i) for i in range(nin)]
     params = ["float p" + str(i) for i in range(npar)]
     elem_params = ", ".join([f"float {a}" for a in args] + params) or "void"
-    call_params = ", ".join(args + [f"p{i}" for i in range(npar)])
     kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
-    kpar = "".join(f", float p{i}" for i in range(npar))
     pro = "\n".join("  " + line.strip() for line in ctx.prologue)
-    vec_loads = "\n".join(f"    const float4 v{i} = *reinterpret_cast<const float4*>(in{i} + i0);"
-                          for i in range(nin))
-
-    def lane(c):
-        return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)])
-
-    aligned = " && ".join(["ts_al(out)"] + [f"ts_al(in{i})" for i in range(nin)])
-    scalar_args = ", ".join([f"in{i}[i]" for i in range(nin)] + [f"p{i}" for i in range(npar)])
     helpers = _pow_helpers(used)
-    code = f"""{helpers}
+    elem = f"""{helpers}
 // {label}
 __device__ __forceinline__ float {name}_elem({elem_params}) {{
 {pro}
   return {body};
 }}
-extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}{kpar}) {{
+"""
+    aligned = " && ".join(["ts_al(out)"] + [f"ts_al(in{i})" for i in range(nin)])
+    if batch is None:
+        kpar = "".join(f", float p{i}" for i in range(npar))
+        vec_loads = "\n".join(
+            f"    const float4 v{i} = *reinterpret_cast<const float4*>(in{i} + i0);"
+            for i in range(nin))
+
+        def lane(c):
+            return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)])
+
+        scalar_args = ", ".join([f"in{i}[i]" for i in range(nin)] + [f"p{i}" for i in range(npar)])
+        code = elem + f"""extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}{kpar}) {{
   const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
   if (i0 >= n) return;
   if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
@@ -284,8 +293,61 @@ extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long 
   }}
 }}
 """
-    return Kernel(code=code, name=name, count=count, label=label, ninputs=nin, nparams=npar,
-                  body=body)
+        return Kernel(code=code, name=name, count=count, label=label, ninputs=nin, nparams=npar,
+                      body=body)
+
+    per = int(count)
+    shared = {io.tensors[c] for c in batch.shared if c in io.tensors}
+
+    def idx(i, var):
+        return f"{var} % {per}LL" if i in shared else var
+
+    vec_loads = "\n".join(
+        f"    const float4 v{i} = *reinterpret_cast<const float4*>(in{i} + {idx(i, 'i0')});"
+        for i in range(nin))
+    pload = "".join(f"    const float p{k} = prm[run * {npar} + {k}];\n" for k in range(npar))
+
+    def lane(c):
+        return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)])
+
+    scalar_args = ", ".join([f"in{i}[{idx(i, 'i')}]" for i in range(nin)] +
+                            [f"p{i}" for i in range(npar)])
+    if per % ELEMS_PER_THREAD == 0:
+        # the four elements of a thread always belong to one run
+        vec = f"""  if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
+    const long long run = i0 / {per}LL;
+{pload}{vec_loads}
+    float4 r;
+    r.x = {name}_elem({lane('x')});
+    r.y = {name}_elem({lane('y')});
+    r.z = {name}_elem({lane('z')});
+    r.w = {name}_elem({lane('w')});
+    *reinterpret_cast<float4*>(out + i0) = r;
+    return;
+  }}
+"""
+    else:
+        vec = "  // runs are not a multiple of 4 elements long: element by element\n"
+    code = elem + f"""// ensemble form: {batch.runs} runs x {per} elements, {npar} scalar(s) per run in prm
+extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}, const float* __restrict__ prm) {{
+  const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
+  if (i0 >= n) return;
+{vec}  for (long long i = i0; i < n && i < i0 + {ELEMS_PER_THREAD}; ++i) {{
+    const long long run = i / {per}LL;
+{pload}    out[i] = {name}_elem({scalar_args});
+  }}
+}}
+"""
+    return Kernel(code=code, name=name, count=count * batch.runs, label=label, ninputs=nin,
+                  nparams=npar, body=body)
+
+
+@dataclass
+class Batch:
+    """Ensemble shape of a generated kernel: `runs` runs; inputs whose
+    c_names are in `shared` hold one copy for all runs."""
+    runs: int
+    shared: frozenset = frozenset()
 
 
 MODULE_PRELUDE = f"""// generated element-wise kernels (tsphere.hipgen)

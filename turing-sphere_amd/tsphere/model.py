@@ -39,13 +39,18 @@ class GpuModel:
         self._stashed_ran = "init"
         self._destroyed = False
         self.graph_steps = 1
+        self.runs = 1
 
     @staticmethod
     def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
                stream: Stream | None = None, batch: bool = True,
-               file_name: str = "model.m") -> "GpuModel":
-        """GpuModel.create (model.ts:103-163)."""
+               file_name: str = "model.m", runs: int = 1) -> "GpuModel":
+        """GpuModel.create (model.ts:103-163).  ``runs`` > 1: an ensemble of
+        that many independent runs of the model (same grid), each value
+        holding the runs back to back; the plan needs
+        max_fields >= runs * len(state)."""
         self = GpuModel()
+        self.runs = int(runs)
         npts = cfg.nlat * cfg.nphi
         nlm = sht.nlm
         self.sht, self.cfg = sht, cfg
@@ -60,20 +65,28 @@ class GpuModel:
         compiled.finish()
         feedback = list(state) + [None] * len(view)
 
+        R = self.runs
+        if R > 1 and R * max(1, len(state)) > sht.max_fields:
+            raise ValueError(f"an ensemble of {R} runs x {len(state)} species needs a plan "
+                             f"with max_fields >= {R * len(state)} (has {sht.max_fields})")
         host = HostBuffers()
-        host.ensure_group(list(state), 2 * nlm)
+        host.ensure_group(list(state), 2 * nlm * R)
         host.ensure("lam", 2 * nlm)
-        host.ensure("noise", npts)
+        host.ensure("noise", npts * R)
         self._host = host
-        self._init = ModelPlan.create(sht, PlanSpec(init_fn, feedback), host, batch, "init")
-        self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step")
+        shared = ("lam",)
+        self._init = ModelPlan.create(sht, PlanSpec(init_fn, feedback), host, batch, "init",
+                                      runs=R, shared=shared)
+        self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step",
+                                      runs=R, shared=shared)
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
-        self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm, "mgpu-state-stash")
+        self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm * R, "mgpu-state-stash")
         return self
 
     def set_params(self, params: dict) -> None:
-        self._init.set_params(params)
-        self._step.set_params(params)
+        """Scalars; an ensemble also takes one value per run."""
+        self._init.set_params(params, self.stream)
+        self._step.set_params(params, self.stream)
 
     def upload(self, name: str, data) -> None:
         """model.ts:175-177: write a host-owned value (state or input field)."""

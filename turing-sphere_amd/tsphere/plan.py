@@ -31,7 +31,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceBuffer, Stream
-from .hipgen import KernelInputs, UnsupportedOnGpu, build_kernel, module_source
+from .hipgen import Batch, KernelInputs, UnsupportedOnGpu, build_kernel, module_source
 from .ir import Assign, CompiledFunction, is_numeric, is_tensor, numel, walk
 
 EXTERNAL_OPS = ("synth", "analys")  # externals.ts:95
@@ -123,6 +123,8 @@ class KernelOp:
     ins: list               # c_names, slot order
     handle: object = None   # tsphere_kernel
     count: int = 0
+    stmt: object = None     # the IR Assign (reverse mode differentiates it)
+    io: object = None       # its KernelInputs
 
 
 @dataclass
@@ -174,21 +176,34 @@ class ModelPlan:
         self._graph = None
         self._graph_steps = 0
         self._ptr_cache = {}
+        self.runs = 1
+        self._prm = None       # ensembles: device copy of the per-run scalars
 
     # --------------------------------------------------------- creation
     @staticmethod
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
-               label: str = "step") -> "ModelPlan":
-        """ModelPlan.create (plan.ts:206-407)."""
+               label: str = "step", runs: int = 1, shared=()) -> "ModelPlan":
+        """ModelPlan.create (plan.ts:206-407).
+
+        ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
+        to back in one buffer (host bindings named in ``shared`` keep one
+        copy, e.g. the eigenvalues), each transform moves all runs of all its
+        species in one batched launch per stage, and element-wise kernels
+        read their scalars per run from a device array -- so a parameter
+        change is a small upload, not a re-capture."""
         self = ModelPlan()
         self._sht = sht
+        self.runs = runs = int(runs)
+        if runs < 1:
+            raise ValueError("runs must be >= 1")
+        shared = set(shared)
         fn = spec.fn
         host_of: dict[str, str] = {}       # c_name -> host name, for host-bound args
         param_slots: dict[str, int] = {}
         for p in fn.params:
             if p.binding.kind == "tensor":
                 count = int(np.prod(p.binding.shape))
-                buf = host.ensure(p.name, count)
+                buf = host.ensure(p.name, count * (1 if p.name in shared else runs))
                 self._bufs[p.c_name] = buf
                 self._by_name[p.name] = buf
                 host_of[p.c_name] = p.name
@@ -197,6 +212,9 @@ class ModelPlan:
                 self.param_names.append(p.name)
             # 'const' arguments are exact in the IR and fold into the kernels
         self._params = np.zeros(max(1, len(self.param_names)), np.float32)
+        if runs > 1:
+            self._params = np.zeros((runs, max(1, len(self.param_names))), np.float32)
+        shared_c = frozenset(c for c, n in host_of.items() if n in shared)
 
         derived: dict[str, tuple] = {}
         counts: dict[str, int] = {}
@@ -219,7 +237,8 @@ class ModelPlan:
             if stmt.c_name in counts and counts[stmt.c_name] != count:
                 raise UnsupportedOnGpu(f"'{stmt.name}' changes size between assignments",
                                        stmt.span)
-            if stmt.c_name in self._bufs and self._bufs[stmt.c_name].count != count:
+            if stmt.c_name in self._bufs and self._bufs[stmt.c_name].count != count * (
+                    1 if stmt.c_name in shared_c else runs):
                 raise UnsupportedOnGpu(f"'{stmt.name}' changes size between assignments",
                                        stmt.span)
             counts[stmt.c_name] = count
@@ -242,9 +261,14 @@ class ModelPlan:
                     tensors[node.c_name] = len(tensors)
             kname = f"k{len(pending_kernels)}_{_ident(stmt.name)}"
             lbl = f"{stmt.name} = <{count} elements, element-wise>"
+            if stmt.c_name in shared_c:
+                raise UnsupportedOnGpu(f"'{stmt.name}' is shared by all runs and cannot be "
+                                       f"assigned", stmt.span)
             kern = build_kernel(stmt, KernelInputs(tensors, param_slots, derived), count, lbl,
-                                kname)
-            op = KernelOp("kernel", lbl, kern, stmt.c_name, list(tensors), count=count)
+                                kname, Batch(runs, shared_c) if runs > 1 else None)
+            kio = KernelInputs(tensors, param_slots, derived)
+            op = KernelOp("kernel", lbl, kern, stmt.c_name, list(tensors), count=count * runs,
+                          stmt=stmt, io=kio)
             pending_kernels.append(op)
             ops.append(op)
 
@@ -259,7 +283,8 @@ class ModelPlan:
             dst = host.get(to)
             if dst is None:
                 raise UnsupportedOnGpu(f"'{to}' is not a host binding")
-            n_src = counts.get(out.c_name, self._bufs[out.c_name].count if out.c_name in self._bufs else 0)
+            n_src = (counts[out.c_name] * runs if out.c_name in counts else
+                     self._bufs[out.c_name].count if out.c_name in self._bufs else 0)
             if n_src != dst.count:
                 raise UnsupportedOnGpu(
                     f"'{out.name}' ({n_src} elements) cannot feed '{to}' ({dst.count})")
@@ -269,14 +294,15 @@ class ModelPlan:
             ops.append(CopyOp("copy", f"{out.name} -> {to}", out.c_name, to, keys))
 
         if batch:
-            ops = _schedule(ops, max_batch=min(8, sht.max_fields) if sht is not None else 8)
+            cap = sht.max_fields // runs if sht is not None else 8
+            ops = _schedule(ops, max_batch=max(1, min(8, cap)))
         ops = self._fold_feedback(ops, host, host_of)
 
         # Buffers for every value the .m computes (reused across steps).
         for c, n in counts.items():
             if c not in self._bufs:
-                b = (DeviceBuffer(n, f"mgpu-{names[c]}") if host.allocate
-                     else Placeholder(n, names[c]))
+                b = (DeviceBuffer(n * runs, f"mgpu-{names[c]}") if host.allocate
+                     else Placeholder(n * runs, names[c]))
                 self._owned.append(b)
                 self._bufs[c] = b
             self._by_name[names[c]] = self._bufs[c]
@@ -297,6 +323,8 @@ class ModelPlan:
                 op.handle = h.value
                 self._kernels.append(h.value)
         self._host = host
+        if runs > 1 and host.allocate:
+            self._prm = DeviceBuffer(self._params.size, f"mgpu-{label}-params")
         return self
 
     def _fold_feedback(self, ops, host: HostBuffers, host_of: dict) -> list:
@@ -326,13 +354,24 @@ class ModelPlan:
         return out
 
     # --------------------------------------------------------- params
-    def set_params(self, values: dict) -> None:
+    def set_params(self, values: dict, stream: Stream | None = None) -> None:
         """plan.ts:410-420.  Scalars are launch arguments: a change only
-        invalidates the recorded graph; nothing recompiles."""
+        invalidates the recorded graph; nothing recompiles.  Ensembles take a
+        scalar (all runs) or one value per run, uploaded on `stream` (ordered
+        with the replays already queued there; the graph stays valid)."""
+        if self.runs == 1:
+            for i, name in enumerate(self.param_names):
+                v = values.get(name)
+                self._params[i] = v if (v is not None and np.isfinite(v)) else 0.0
+            self.invalidate()
+            return
         for i, name in enumerate(self.param_names):
             v = values.get(name)
-            self._params[i] = v if (v is not None and np.isfinite(v)) else 0.0
-        self.invalidate()
+            a = np.zeros(self.runs) if v is None else np.broadcast_to(
+                np.asarray(v, np.float64), (self.runs,))
+            self._params[:, i] = np.where(np.isfinite(a), a, 0.0)
+        if self._prm is not None:
+            self._prm.upload(self._params, stream)
 
     def invalidate(self) -> None:
         if self._graph is not None:
@@ -352,7 +391,8 @@ class ModelPlan:
         key = tuple(cnames)
         arr = self._ptr_cache.get(key)
         if arr is None:
-            arr = (C.c_void_p * len(cnames))(*[self._bufs[c].ptr for c in cnames])
+            arr = (C.c_void_p * len(cnames))(*[
+                self._prm.ptr if c == "@prm" else self._bufs[c].ptr for c in cnames])
             self._ptr_cache[key] = arr
         return arr
 
@@ -362,15 +402,28 @@ class ModelPlan:
         sh = stream.handle
         pvals = self._params.ctypes.data_as(C.POINTER(C.c_float))
         np_ = len(self.param_names)
+        ens = self.runs > 1
+        if ens:
+            np_ = 0
+            nq, ns = 2 * self._sht.nlm, self._sht.cfg.npts
         for _ in range(steps):
             for op in self.ops:
                 if isinstance(op, KernelOp):
-                    ptrs = self._ptrs([op.out] + op.ins)
+                    ptrs = self._ptrs([op.out] + op.ins + (["@prm"] if ens else []))
                     _lib.check(L.tsphere_kernel_launch(op.handle, op.count, len(ptrs), ptrs, np_,
                                                        pvals, sh), op.label)
                 elif isinstance(op, XformOp):
                     srcs = self._ptrs(op.srcs)
                     dsts = self._ptrs(op.dsts)
+                    if ens:
+                        if op.kind == "synth":
+                            rc = L.tsphere_synth_fields(self._sht.handle, len(op.srcs), self.runs,
+                                                        srcs, nq, dsts, ns, sh)
+                        else:
+                            rc = L.tsphere_analys_fields(self._sht.handle, len(op.srcs),
+                                                         self.runs, srcs, ns, dsts, nq, sh)
+                        _lib.check(rc, op.label)
+                        continue
                     fn = L.tsphere_synth_multi if op.kind == "synth" else L.tsphere_analys_multi
                     _lib.check(fn(self._sht.handle, len(op.srcs), srcs, dsts, sh), op.label)
                 else:
@@ -419,6 +472,9 @@ class ModelPlan:
         for b in self._owned:
             b.free()
         self._owned.clear()
+        if self._prm is not None:
+            self._prm.free()
+            self._prm = None
 
 
 def _ident(name: str) -> str:

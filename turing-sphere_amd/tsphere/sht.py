@@ -108,6 +108,28 @@ class ShtPlan:
     def handle(self):
         return self._h
 
+    # ------------------------------------------------------ tile shapes
+    STAGES = {"synth": 0, "analys": 1}
+
+    def set_tile(self, stage: str, nf: int = 0, nwaves: int = 0, k: int = 0,
+                 ctc: int = 0) -> None:
+        """Fix the Legendre workgroup shape of one stage (0 = built-in)."""
+        t = _lib.LegTileC(nf, nwaves, k, ctc)
+        _lib.call("tsphere_plan_set_tile", self._h, self.STAGES[stage], C.byref(t))
+
+    def get_tile(self, stage: str, nfield: int) -> dict:
+        t = _lib.LegTileC()
+        _lib.call("tsphere_plan_get_tile", self._h, self.STAGES[stage], nfield, C.byref(t))
+        return {"nf": t.nf, "nwaves": t.nwaves, "k": t.k, "ctc": t.ctc}
+
+    def tune(self, nfield: int, reps: int = 5, cache: bool = True) -> dict:
+        """Autotune the Legendre tiles for `nfield` fields (timed on the
+        device, blocking); results are cached per device and shape (see
+        tsphere.tune) so later plans of the same shape skip the timing."""
+        from . import tune as _tune
+
+        return _tune.tune_plan(self, nfield, reps, cache)
+
     # ------------------------------------------------------ bindings
     def create_synth_binding(self, qlm_in: DeviceBuffer, spat_out: DeviceBuffer,
                              nfield: int = 1, qlm_stride: int = 0,
@@ -184,8 +206,13 @@ class ShtPlan:
         return self.qlm_out.download(self.stream)
 
     def read_fm(self) -> np.ndarray:
-        """The Fourier-space intermediate of the last staged transform."""
-        return self.fm_buf.download(self.stream)
+        """The Fourier-space intermediate of the last staged transform, in the
+        reference's fmBuf order (complex [m][ilat], interleaved re/im).  On
+        the device the scratch is row-major per latitude ([ilat][m]), so the
+        FFT reads and writes contiguous rows; this transposes it back."""
+        raw = self.fm_buf.download(self.stream)
+        m1, nlat = self.cfg.mmax + 1, self.cfg.nlat
+        return np.ascontiguousarray(raw.reshape(nlat, m1, 2).transpose(1, 0, 2)).reshape(-1)
 
     def destroy(self) -> None:
         """sht.ts:463-468"""
