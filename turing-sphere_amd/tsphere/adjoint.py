@@ -36,6 +36,7 @@ from .model import eigenvalues
 from .plan import HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
 
 WG = 256
+EPT = 4  # elements per thread (tsphere_kernel_launch launches count / 4 threads)
 
 
 # ------------------------------------------------------ symbolic derivative
@@ -182,13 +183,13 @@ def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: in
     pnames = sorted(io.params, key=lambda c: io.params[c])
     dps = [derivative(op.stmt.expr, c, ctx) for c in pnames]
     nin = len(io.tensors)
-    loads = "\n".join(f"    const float a{i} = in{i}[i];" for i in range(nin))
-    pro = "\n".join("    " + line.strip() for line in ctx.prologue)
+    loads = "\n".join(f"      const float a{i} = in{i}[i];" for i in range(nin))
+    pro = "\n".join("      " + line.strip() for line in ctx.prologue)
     kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
     kbar = "".join(f", float* __restrict__ bar{j}" for j in range(len(grad_inputs)))
     kpar = "".join(f", float p{i}" for i in range(nparams))
-    upd = "\n".join(f"    bar{j}[i] += ob * {d};" for j, d in enumerate(dins) if d is not None)
-    acc = "\n".join(f"    c{j} = ob * {d};" for j, d in enumerate(dps) if d is not None)
+    upd = "\n".join(f"      bar{j}[i] += ob * {d};" for j, d in enumerate(dins) if d is not None)
+    acc = "\n".join(f"      c{j} += ob * {d};" for j, d in enumerate(dps) if d is not None)
     np_ = max(1, nparams)
     cdecl = " ".join(f"float c{j} = 0.0f;" for j in range(nparams))
     live = [j for j, d in enumerate(dps) if d is not None]  # parameters this statement uses
@@ -208,10 +209,11 @@ def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: in
 // adjoint of: {op.label}
 extern "C" __global__ __launch_bounds__({WG}) void {name}(long long n, const float* __restrict__ obar{kin}{kbar}, float* __restrict__ partials{kpar}) {{
   __shared__ float red[{np_}][4];
-  const long long i = (long long)blockIdx.x * {WG} + threadIdx.x;
+  // {EPT} consecutive elements per thread: the launch contract of tsphere_kernel_launch
+  const long long i0 = ((long long)blockIdx.x * {WG} + threadIdx.x) * {EPT};
   {cdecl}
-  if (i < n) {{
-    const float ob = obar[i];
+  for (long long i = i0; i < n && i < i0 + {EPT}; ++i) {{
+      const float ob = obar[i];
 {loads}
 {pro}
 {upd}
@@ -284,7 +286,7 @@ class AdjointModel:
                 gi = [c for c in op.ins if c in self._bar]
                 name = f"adj{idx}"
                 code = build_adjoint_kernel(op, name, gi, npar)
-                nblk = (op.count + WG - 1) // WG
+                nblk = (op.count + WG * EPT - 1) // (WG * EPT)
                 part = DeviceBuffer(nblk * max(1, npar), f"partials-{name}")
                 self._owned.append(part)
                 self._adj.append(("kernel", op, name, gi, part, nblk))
