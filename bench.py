@@ -122,6 +122,8 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     npts, nlm = cfg.npts, cfg.nlm
     log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={legendre}")
     sht = ShtPlan.create(cfg, legendre=legendre, max_fields=2)
+    tuned = sht.tune(2, reps=5)
+    log(f"[bench] {legendre} tuned tiles: {tuned}")
     src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
     gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"])
     gm.set_params(PARAMS)
@@ -163,7 +165,8 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     }
     log(f"[bench] {legendre} kernel ms: {kms}")
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
-           "kernel_ms": kms, "finite": finite}
+           "kernel_ms": kms, "finite": finite,
+           "tiles": {k: tuned[k] for k in ("synth", "analys")}}
     if err:
         import oracle as O
 
@@ -316,10 +319,15 @@ def main() -> int:
     peak = PEAK_BF16 if args.legendre == "bf16x3" else PEAK_FP32
     steps_s = main_v["steps_per_s"]
     sht_flops_step = 4 * (f_leg + f_fft)
+    # HBM bytes per launch of the dominant kernel, from the committed counter
+    # summary (FETCH_SIZE / WRITE_SIZE runs, calibrated on a copy of known size)
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"{dom}_brusselator_l1023_pmc.json")
-    if os.path.exists(prof):
-        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+    prof = os.path.join(ROOT, "profiles", f"{dom}_brusselator_l1023_pmc.md")
+    if args.legendre == "fp32" and os.path.exists(prof):
+        import re
+
+        m = re.search(r"^hbm_bytes_per_launch: (\d+)", open(prof).read(), re.M)
+        traffic = int(m.group(1)) if m else None
 
     extras = {}
     if not args.quick:

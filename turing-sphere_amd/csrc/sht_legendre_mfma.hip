@@ -18,13 +18,12 @@
 // FP32's exponent range, so seeded values (> 1e-17) need no row scaling.
 //
 // Layout on the chip.  Each lane carries one latitude pair's recurrence and
-// produces, per tile of consecutive l, a row of P; the rows are written to a
-// wave-private LDS tile as BF16 hi/lo planes and read back in the MFMA
+// produces, per tile of consecutive l, a row of P, which must reach the MFMA
 // operand layout (A/B: lane L holds 8 consecutive K values of row/column
 // L % 16, K block L / 16; D: lane L holds rows 4(L/16)..+3 of column L % 16):
-//  * synthesis: tile [pair][l] (64 l = 32 even + 32 odd), A = P rows,
-//    B = Q (loaded from HBM per tile, split on the fly); the tile loop is
-//    wave-uniform.
+//  * synthesis: A = P rows (64 l = 32 even + 32 odd per tile), formed in
+//    registers by a permlane transpose (no LDS), B = Q (loaded from HBM per
+//    tile, split on the fly); the tile loop is wave-uniform.
 //  * analysis: tile [l][pair] (32 l), A = P^T rows, B = W (split once per
 //    row and held in registers); per tile the four waves' partial sums over
 //    their pairs are added through LDS.
@@ -93,12 +92,36 @@ __device__ __forceinline__ void step2(double2 a0, double2 a1, double ct, double&
 
 // ------------------------------------------------------------- synthesis
 // Block: 4 waves, 256 latitude pairs (one per lane); grid (m-pair, pair chunk,
-// column chunk of CTC 16-column tiles).
+// column chunk of CTC 16-column tiles).  Per tile of 64 l (32 even + 32 odd)
+// each lane generates its pair's values as four BF16 planes (E hi/lo, O
+// hi/lo), each four K-blocks of 8; a 4 x 4 transpose of (16-lane group x
+// K-block) -- two v_permlane32_swap then two v_permlane16_swap per dword --
+// leaves in register block rt exactly the MFMA A operand of row tile rt
+// (pairs 16 rt .. 16 rt + 15).  No LDS tile: LDS holds only the row's (a, b).
 constexpr int SY_TL = 64;                  // l per tile (32 even + 32 odd)
-constexpr int SY_ROW = 32 * 2 + 16;        // bytes per LDS row: 32 bf16 + pad
-constexpr int SY_PLANE = 64 * SY_ROW;      // one plane of a wave's tile
-constexpr int SY_WAVE = 4 * SY_PLANE;      // E_hi, E_lo, O_hi, O_lo
 constexpr int SY_PADL = SY_TL + 8;         // zero (a,b) entries after a staged row
+
+typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+
+// X[b][g] (lane group b, register block g) -> X[g][b], per dword.
+__device__ __forceinline__ void transpose4x4(bf16x8 (&P)[4]) {
+  u4 R[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) R[g] = __builtin_bit_cast(u4, P[g]);
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const auto r02 = __builtin_amdgcn_permlane32_swap(R[0][w], R[2][w], false, false);
+    const auto r13 = __builtin_amdgcn_permlane32_swap(R[1][w], R[3][w], false, false);
+    const auto r01 = __builtin_amdgcn_permlane16_swap(r02[0], r13[0], false, false);
+    const auto r23 = __builtin_amdgcn_permlane16_swap(r02[1], r13[1], false, false);
+    R[0][w] = r01[0];
+    R[1][w] = r01[1];
+    R[2][w] = r23[0];
+    R[3][w] = r23[1];
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) P[g] = __builtin_bit_cast(bf16x8, R[g]);
+}
 
 template <int CTC>
 __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfield, Fields Fq,
@@ -106,9 +129,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfiel
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int L = T.lmax, M = T.mmax;
   double2* sab = reinterpret_cast<double2*>(smem);
-  unsigned char* tile = smem + sizeof(double2) * (L + 1 + SY_PADL);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  unsigned char* wt = tile + wv * SY_WAVE;
   const int pair0 = blockIdx.y * 256 + wv * 64;  // this wave's first pair
   const int ct0 = blockIdx.z * CTC;              // first 16-column tile
   const int g = lane >> 4, r16 = lane & 15;
@@ -135,48 +156,28 @@ __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfiel
       for (int t = 0; t < CTC; ++t) accE[rt][t] = accO[rt][t] = f4{0.f, 0.f, 0.f, 0.f};
 
     for (int lt = lmin; lt <= L; lt += SY_TL) {
-      // 1) this lane's P row for l = lt .. lt+63, split into BF16 planes
-      __bf16 ph[4][8], pl[4][8];  // scratch per 8-value group
+      // 1) this lane's P row for l = lt .. lt+63: planes [E hi, E lo, O hi, O lo],
+      //    K-block kb holds even (odd) values k = 8 kb .. 8 kb + 7, l = lt + 2k (+1)
+      bf16x8 P[4][4];
       const int i0 = lt - m;
 #pragma unroll
-      for (int grp = 0; grp < 8; ++grp) {  // 8 groups of 8 consecutive l
-#pragma unroll
-        for (int t = 0; t < 8; t += 2) {
-          const int tt = grp * 8 + t;  // offset in tile, even
-          if ((tt & 3) == 0 && c.ls == lt + tt) {
-            c.y0 = c.s0;
-            c.y1 = c.s1;
-          }
-          // y0 -> even plane (k = tt/2), y1 -> odd plane
-          const int k = tt >> 1;
-          __bf16 h, lo;
-          split((float)c.y0, h, lo);
-          ph[0][k & 7] = h;
-          pl[0][k & 7] = lo;
-          split((float)c.y1, h, lo);
-          ph[1][k & 7] = h;
-          pl[1][k & 7] = lo;
-          step2(sab[i0 + tt + 2], sab[i0 + tt + 3], c.ct, c.y0, c.y1);
+      for (int tt = 0; tt < SY_TL; tt += 2) {
+        if ((tt & 3) == 0 && c.ls == lt + tt) {
+          c.y0 = c.s0;
+          c.y1 = c.s1;
         }
-        if (grp & 1) {  // 8 even + 8 odd values complete: k = 8*(grp/2) .. +7
-          const int kb = grp >> 1;
-          unsigned char* row = wt + lane * SY_ROW + kb * 16;
-          bf16x8 v;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) v[s] = ph[0][s];
-          *reinterpret_cast<bf16x8*>(row + 0 * SY_PLANE) = v;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) v[s] = pl[0][s];
-          *reinterpret_cast<bf16x8*>(row + 1 * SY_PLANE) = v;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) v[s] = ph[1][s];
-          *reinterpret_cast<bf16x8*>(row + 2 * SY_PLANE) = v;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) v[s] = pl[1][s];
-          *reinterpret_cast<bf16x8*>(row + 3 * SY_PLANE) = v;
-        }
+        const int k = tt >> 1, kb = k >> 3, s8 = k & 7;
+        __bf16 h, lo;
+        split((float)c.y0, h, lo);
+        P[0][kb][s8] = h;
+        P[1][kb][s8] = lo;
+        split((float)c.y1, h, lo);
+        P[2][kb][s8] = h;
+        P[3][kb][s8] = lo;
+        step2(sab[i0 + tt + 2], sab[i0 + tt + 3], c.ct, c.y0, c.y1);
       }
-      wave_lds_sync();
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) transpose4x4(P[pl]);
 
       // 2) B operands: Q for this tile, lane column r16 of each column tile,
       //    K block g: l = lt + 2(8g + s) (even) and +1 (odd)
@@ -206,18 +207,12 @@ __global__ __launch_bounds__(256) void k_leg_synth_mfma(LegTables64 T, int nfiel
       // 3) 4 row tiles x CTC column tiles x (E, O) x 3 products
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
-        const unsigned char* row = wt + (16 * rt + r16) * SY_ROW + g * 16;
-        const bf16x8 aEh = *reinterpret_cast<const bf16x8*>(row + 0 * SY_PLANE);
-        const bf16x8 aEl = *reinterpret_cast<const bf16x8*>(row + 1 * SY_PLANE);
-        const bf16x8 aOh = *reinterpret_cast<const bf16x8*>(row + 2 * SY_PLANE);
-        const bf16x8 aOl = *reinterpret_cast<const bf16x8*>(row + 3 * SY_PLANE);
 #pragma unroll
         for (int t = 0; t < CTC; ++t) {
-          accE[rt][t] = mma3(aEh, aEl, bEh[t], bEl[t], accE[rt][t]);
-          accO[rt][t] = mma3(aOh, aOl, bOh[t], bOl[t], accO[rt][t]);
+          accE[rt][t] = mma3(P[0][rt], P[1][rt], bEh[t], bEl[t], accE[rt][t]);
+          accO[rt][t] = mma3(P[2][rt], P[3][rt], bOh[t], bOl[t], accO[rt][t]);
         }
       }
-      wave_lds_sync();  // the tile is rewritten next iteration
     }
 
     // 4) north = E + O, south = E - O
@@ -411,7 +406,7 @@ hipError_t synth_ctc(const LegTables64& T, int nfield, const Fields& q, float2* 
   const int ncol = 2 * nfield;
   const int ct = (ncol + 15) / 16;
   dim3 grid(T.mmax / 2 + 1, (T.npair + 255) / 256, (ct + CTC - 1) / CTC);
-  const size_t lds = sizeof(double2) * (T.lmax + 1 + SY_PADL) + 4 * (size_t)SY_WAVE;
+  const size_t lds = sizeof(double2) * (T.lmax + 1 + SY_PADL);
   hipLaunchKernelGGL(k_leg_synth_mfma<CTC>, grid, dim3(256), lds, s, T, nfield, q, fm);
   return hipGetLastError();
 }
