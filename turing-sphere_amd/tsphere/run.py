@@ -9,7 +9,7 @@ from __future__ import annotations
 import json
 import sys
 
-from .session import format_command, format_digest, parse_args, run, StateDigest
+from .session import StateDigest, _num, format_command, format_digest, parse_args, run
 
 
 def main(argv=None) -> int:
@@ -51,7 +51,7 @@ def main(argv=None) -> int:
         print(f"  {thr['msPerStep']:.3f} ms/step   {thr['stepsPerSec']:.1f} steps/s "
               f"(batches of {thr['batch']})")
         if rec["digest"]:
-            print(f"  state after {spec.steps} steps from seed {spec.seed}:")
+            print(f"  state after {spec.steps} steps from seed {_num(spec.seed)}:")
             print(f"    {format_digest(StateDigest(**rec['digest']))}")
     return 0 if rec["state"]["finite"] else 1
 
