@@ -222,11 +222,14 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  * The Legendre stage's workgroup shape (no reference counterpart: WebGPU's
  * workgroup sizes are fixed in sht.ts:70-89).  Stage 0 = synthesis,
  * 1 = analysis; fields left 0 use the built-in choice.
- *   FP32 form:  nf fields per workgroup (1, 2, 4), nwaves waves (1..4),
- *               k latitude pairs per lane (1, 2, 4, 8), 64*nwaves*k >= pairs.
+ *   FP32 form:  impl 1 = contraction on the VALU, 2 = on the FP32 matrix cores
+ *               (v_mfma_f32_16x16x4_f32; the recurrence stays on the VALU);
+ *               nf fields per workgroup (1, 2, 4), nwaves waves (1..4),
+ *               k latitude pairs per lane (1, 2, 4, 8; matrix cores 1, 2, 4),
+ *               64*nwaves*k >= pairs.
  *   MFMA form:  ctc 16-column tiles per workgroup (1, 2; 4 for synthesis). */
 typedef struct {
-  int nf, nwaves, k, ctc;
+  int nf, nwaves, k, ctc, impl;
 } tsphere_leg_tile;
 
 int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile *tile); /* NULL = built-in */

@@ -399,6 +399,321 @@ hipError_t analys_nf(const LegTables& T, int nfield, const LegTile& t, const flo
   }
 }
 
+// ------------------------------------------ contraction on FP32 matrix cores
+// The same Legendre stage with the sum over l (synthesis) or over latitude
+// pairs (analysis) on v_mfma_f32_16x16x4_f32 -- FP32 operands, FP32
+// accumulation -- while the VALU only runs the recurrence (4 ops per l per
+// pair instead of 8-13).  Operand layout (checked on gfx950): A lane L holds
+// A[L % 16][L / 16], B lane L holds B[L / 16][L % 16], D lane L holds
+// D[4 (L / 16) + r][L % 16].  Columns are (parity, field, re/im): even and
+// odd l - m terms go to separate columns, so one product serves both (the
+// north / south recombination happens on the way out).  Each lane carries KC
+// latitude pairs; a wave's pairs pass through a wave-private LDS tile of 16 l
+// that turns "lane = pair" into the operand layout.
+typedef __attribute__((ext_vector_type(4))) float f4v;
+constexpr int MF_TL = 16;   // l per tile
+constexpr int MF_PAD = 24;  // zero (a, b) / Q entries after a staged row
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ f4v mf4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Synthesis: D[pair][col] = sum_l P[pair][l] B[l][col], B = Q masked by parity.
+// A row tile is 16 pairs; K-block kb of a 16-l tile holds l = lt + 4 (L/16) + kb
+// for lane L, so each lane reads its 4 K values as one 16-byte LDS load.
+template <int NF, int KC>
+__global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, Fields Fq,
+                                                      float2* __restrict__ fm) {
+  constexpr int NCOL = 4 * NF;
+  constexpr int QW = 64 * KC;        // pairs per wave
+  constexpr int NRT = QW / 16;       // row tiles per wave
+  constexpr int PROW = MF_TL + 4;    // floats per pair row of the tile (padded)
+  extern __shared__ float smem[];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const int nrow = (L + 1 + MF_PAD + 1) & ~1;  // even: keeps the tile 16-byte aligned
+  float2* sab = reinterpret_cast<float2*>(smem);
+  float2* sq = sab + nrow;
+  float* tl = reinterpret_cast<float*>(sq + nrow * NF) + wv * (QW * PROW);
+  const int f0 = blockIdx.y * NF;
+  const int wp0 = wv * QW;  // this wave's first pair
+  // this lane's output column
+  const int par = r16 / (2 * NF), cf = (r16 % (2 * NF)) >> 1, ri = r16 & 1;
+
+  for (int r = 0; r < 2; ++r) {
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
+    __syncthreads();
+    stage_row<NF>(T, m, f0, nfield, Fq, true, sab, sq, MF_PAD);
+    __syncthreads();
+
+    Chain c[KC];
+    int lmin = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      load_chain(T, m, wp0 + lane + 64 * k, c[k]);
+      lmin = min(lmin, c[k].ls);
+    }
+    lmin = wave_min(lmin);
+
+    f4v acc[NRT];
+#pragma unroll
+    for (int t = 0; t < NRT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    for (int lt = lmin; lt <= L; lt += MF_TL) {
+      // 1) 16 values of each chain -> its pair row, four 16-byte stores
+      float v[KC][MF_TL];
+#pragma unroll
+      for (int t = 0; t < MF_TL; t += 2) {
+        if (t % kTile == 0) {
+#pragma unroll
+          for (int k = 0; k < KC; ++k)
+            if (c[k].ls == lt + t) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+        }
+        const int i = lt + t - m;
+        const float2 a0 = sab[i + 2], a1 = sab[i + 3];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          v[k][t] = c[k].y0;
+          v[k][t + 1] = c[k].y1;
+          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KC; ++k)
+#pragma unroll
+        for (int t4 = 0; t4 < MF_TL / 4; ++t4)
+          *reinterpret_cast<float4*>(tl + (lane + 64 * k) * PROW + 4 * t4) =
+              make_float4(v[k][4 * t4], v[k][4 * t4 + 1], v[k][4 * t4 + 2], v[k][4 * t4 + 3]);
+      wave_sync_lds();
+      // 2) B: Q of l = lt + 4 kg + kb for this lane's column, zero off its parity
+      float b[4];
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        const int l = lt + 4 * kg + kb;
+        const float2 q = sq[(l - m) * NF + cf];
+        b[kb] = (r16 < NCOL && ((l - m) & 1) == par && l <= L) ? (ri ? q.y : q.x) : 0.f;
+      }
+      // 3) 4 products per row tile
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt) {
+        const float4 a4 = *reinterpret_cast<const float4*>(tl + (16 * rt + r16) * PROW + 4 * kg);
+        acc[rt] = mf4(a4.x, b[0], acc[rt]);
+        acc[rt] = mf4(a4.y, b[1], acc[rt]);
+        acc[rt] = mf4(a4.z, b[2], acc[rt]);
+        acc[rt] = mf4(a4.w, b[3], acc[rt]);
+      }
+      wave_sync_lds();  // the tile is rewritten next iteration
+    }
+
+    // north = E + O, south = E - O: the odd column of (field, re/im) is 2 NF lanes up
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float e = acc[rt][q];
+        const float o = __shfl(e, lane + 2 * NF);
+        const int j = wp0 + 16 * rt + 4 * kg + q;
+        if (r16 < 2 * NF && f0 + cf < nfield && j < T.npair) {
+          float* out = reinterpret_cast<float*>(fm + (long long)(f0 + cf) * nlat * (M + 1) + m) + ri;
+          out[2LL * j * (M + 1)] = e + o;
+          out[2LL * (nlat - 1 - j) * (M + 1)] = e - o;
+        }
+      }
+  }
+}
+
+// Analysis: D[l][col] = sum_pairs P[pair][l] W[pair][col], W = (G_N +/- G_S) w
+// by parity (leg.ts:207-219).  K-block kb of the wave holds pairs
+// (L / 16) * NKB + kb for lane L, so a lane's A values for four K-blocks are one
+// 16-byte load; W stays in registers for the whole row.  The four waves' partial
+// sums of a tile meet in LDS.
+template <int NF, int KC>
+__global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
+                                                       const float2* __restrict__ fm, Fields Fq) {
+  constexpr int NCOL = 4 * NF;
+  constexpr int QW = 64 * KC;   // pairs per wave
+  constexpr int NKB = QW / 4;   // K-blocks per wave
+  constexpr int TROW = QW + 4;  // floats per l row of the tile (padded)
+  extern __shared__ float smem[];
+  __shared__ int smin[4];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  float2* sab = reinterpret_cast<float2*>(smem);
+  float* tiles = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
+  float* tl = tiles + wv * (MF_TL * TROW);
+  float* red = tiles + 4 * MF_TL * TROW;  // [4][256]
+  const int f0 = blockIdx.y * NF;
+  const int wp0 = wv * QW;
+  const int par = r16 / (2 * NF), cf = (r16 % (2 * NF)) >> 1, ri = r16 & 1;
+
+  for (int r = 0; r < 2; ++r) {
+    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+    const int m = r == 0 ? mp : M - mp;
+    if (r == 1 && m <= mp) break;
+    const long long base = lm_base(L, m);
+    __syncthreads();
+    stage_row<NF>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+
+    Chain c[KC];
+    int lmin = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      load_chain(T, m, wp0 + lane + 64 * k, c[k]);
+      lmin = min(lmin, c[k].ls);
+    }
+    // B operands for the whole row: pair wp0 + kg NKB + kb, this lane's column
+    float wB[NKB];
+    {
+      const bool live = r16 < NCOL && f0 + cf < nfield;
+      const int fld = min(f0 + cf, nfield - 1);
+      const float* g =
+          reinterpret_cast<const float*>(fm + (long long)fld * nlat * (M + 1) + m) + ri;
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        const int j = wp0 + kg * NKB + kb;
+        float v = 0.f;
+        if (live && j < T.npair) {
+          const int js = nlat - 1 - j;
+          const float w = T.w[j];
+          const float gN = g[2LL * j * (M + 1)], gS = g[2LL * js * (M + 1)];
+          if (js == j)  // equator of an odd grid pairs with itself
+            v = par == 0 ? gN * w : 0.f;
+          else
+            v = par == 0 ? (gN + gS) * w : (gN - gS) * w;
+        }
+        wB[kb] = v;
+      }
+    }
+    lmin = wave_min(lmin);
+    if (lane == 0) smin[wv] = lmin;
+    __syncthreads();  // also publishes the staged row
+    int lblk = smin[0];
+    for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+    lblk = min(lblk, L + 1);
+
+    // coefficients below every lane's significance point are exactly 0
+    for (int i = m + tid; i < lblk; i += nthr)
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        if (f0 + f < nfield)
+          reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
+              make_float2(0.f, 0.f);
+
+    for (int lt = lblk; lt <= L; lt += MF_TL) {
+      // 1) 16 values of each chain -> row t of the tile, column = pair
+#pragma unroll
+      for (int t = 0; t < MF_TL; t += 2) {
+        if (t % kTile == 0) {
+#pragma unroll
+          for (int k = 0; k < KC; ++k)
+            if (c[k].ls == lt + t) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+        }
+        const int i = lt + t - m;
+        const float2 a0 = sab[i + 2], a1 = sab[i + 3];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          tl[t * TROW + lane + 64 * k] = c[k].y0;
+          tl[(t + 1) * TROW + lane + 64 * k] = c[k].y1;
+          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+        }
+      }
+      wave_sync_lds();
+      // 2) this wave's pairs: NKB products
+      f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
+      const float* arow = tl + r16 * TROW + kg * NKB;
+#pragma unroll
+      for (int kb = 0; kb < NKB; kb += 4) {
+        const float4 a4 = *reinterpret_cast<const float4*>(arow + kb);
+        acc = mf4(a4.x, wB[kb], acc);
+        acc = mf4(a4.y, wB[kb + 1], acc);
+        acc = mf4(a4.z, wB[kb + 2], acc);
+        acc = mf4(a4.w, wB[kb + 3], acc);
+      }
+      // 3) the waves' partial sums meet in LDS; one thread per (l, column)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red[wv * 256 + (4 * kg + q) * 16 + r16] = acc[q];
+      __syncthreads();
+      for (int idx = tid; idx < 256; idx += nthr) {
+        const int row = idx >> 4, col = idx & 15;
+        const int l = lt + row;
+        const int cpar = col / (2 * NF);
+        if (col < NCOL && l <= L && ((l - m) & 1) == cpar) {
+          float s = red[idx];
+          for (int w = 1; w < nwaves; ++w) s += red[w * 256 + idx];
+          const int f = (col % (2 * NF)) >> 1;
+          if (f0 + f < nfield)
+            reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f)))[2 * (base + (l - m)) +
+                                                                        (col & 1)] = s;
+        }
+      }
+      __syncthreads();  // red and the tiles are rewritten next iteration
+    }
+  }
+}
+
+template <int NF, int KC>
+hipError_t synth_mf(const LegTables& T, int nfield, int nwaves, const Fields& q, float2* fm,
+                    hipStream_t s) {
+  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  const size_t lds = sizeof(float2) * ((T.lmax + 1 + MF_PAD + 1) & ~1) * (1 + NF) +
+                     sizeof(float) * (size_t)nwaves * 64 * KC * (MF_TL + 4);
+  hipLaunchKernelGGL((k_leg_synth_mf<NF, KC>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm);
+  return hipGetLastError();
+}
+
+template <int NF, int KC>
+hipError_t analys_mf(const LegTables& T, int nfield, int nwaves, const float2* fm,
+                     const Fields& q, hipStream_t s) {
+  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  const size_t lds = sizeof(float2) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+                     sizeof(float) * (4 * MF_TL * (size_t)(64 * KC + 4) + 4 * 256);
+  hipLaunchKernelGGL((k_leg_analys_mf<NF, KC>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
+                     q);
+  return hipGetLastError();
+}
+
+template <int NF>
+hipError_t synth_mf_nf(const LegTables& T, int nfield, const LegTile& t, const Fields& q,
+                       float2* fm, hipStream_t s) {
+  switch (t.k) {
+    case 1: return synth_mf<NF, 1>(T, nfield, t.nwaves, q, fm, s);
+    case 2: return synth_mf<NF, 2>(T, nfield, t.nwaves, q, fm, s);
+    case 4: return synth_mf<NF, 4>(T, nfield, t.nwaves, q, fm, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NF>
+hipError_t analys_mf_nf(const LegTables& T, int nfield, const LegTile& t, const float2* fm,
+                        const Fields& q, hipStream_t s) {
+  switch (t.k) {
+    case 1: return analys_mf<NF, 1>(T, nfield, t.nwaves, fm, q, s);
+    case 2: return analys_mf<NF, 2>(T, nfield, t.nwaves, fm, q, s);
+    case 4: return analys_mf<NF, 4>(T, nfield, t.nwaves, fm, q, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Built-in choice of the FP32 form's contraction unit (1 VALU, 2 matrix cores).
+constexpr int kDefaultImpl = 1;
+
 int round_k(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 0; }
 
 }  // namespace
@@ -412,6 +727,8 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
     return t;
   }
+  if (t.impl == 0) t.impl = kDefaultImpl;
+  if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;  // beyond the matrix-core kernels' reach
   if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
   if (t.nwaves == 0 && t.k == 0) t.nwaves = std::min(4, (npair + 63) / 64);
   if (t.nwaves == 0) t.nwaves = std::min(4, (npair + 64 * t.k - 1) / (64 * t.k));
@@ -422,8 +739,9 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
 namespace {
 
 bool fp32_tile_ok(const LegTile& t, int npair) {
-  return (t.nf == 1 || t.nf == 2 || t.nf == 4) && t.nwaves >= 1 && t.nwaves <= 4 &&
-         round_k(t.k) == t.k && 64 * t.nwaves * t.k >= npair;
+  const bool kok = t.impl == 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
+  return (t.impl == 1 || t.impl == 2) && (t.nf == 1 || t.nf == 2 || t.nf == 4) &&
+         t.nwaves >= 1 && t.nwaves <= 4 && kok && 64 * t.nwaves * t.k >= npair;
 }
 
 }  // namespace
@@ -433,6 +751,11 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 2) switch (t.nf) {
+      case 1: return synth_mf_nf<1>(T, nfield, t, q, fm, s);
+      case 2: return synth_mf_nf<2>(T, nfield, t, q, fm, s);
+      default: return synth_mf_nf<4>(T, nfield, t, q, fm, s);
+    }
   switch (t.nf) {
     case 1: return synth_nf<1>(T, nfield, t, q, fm, s);
     case 2: return synth_nf<2>(T, nfield, t, q, fm, s);
@@ -445,6 +768,11 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 2) switch (t.nf) {
+      case 1: return analys_mf_nf<1>(T, nfield, t, fm, q, s);
+      case 2: return analys_mf_nf<2>(T, nfield, t, fm, q, s);
+      default: return analys_mf_nf<4>(T, nfield, t, fm, q, s);
+    }
   switch (t.nf) {
     case 1: return analys_nf<1>(T, nfield, t, fm, q, s);
     case 2: return analys_nf<2>(T, nfield, t, fm, q, s);

@@ -57,11 +57,13 @@ struct LegTables64 {
 };
 
 // Tile shape of a Legendre launch (0 = the built-in choice for that entry).
-//  FP32 kernels: nf fields per workgroup (1, 2, 4), nwaves waves, k latitude
-//  pairs per lane (1, 2, 4, 8; nwaves * 64 * k must cover the pairs).
-//  MFMA kernels: ctc column tiles (16 columns each) per workgroup.
+//  FP32 form: impl 1 = contraction on the VALU, 2 = on the FP32 matrix cores;
+//  nf fields per workgroup (1, 2, 4), nwaves waves, k latitude pairs per lane
+//  (1, 2, 4, 8 on the VALU, 1, 2, 4 on the matrix cores; nwaves * 64 * k must
+//  cover the pairs).
+//  Split-BF16 form: ctc column tiles (16 columns each) per workgroup.
 struct LegTile {
-  int nf, nwaves, k, ctc;
+  int nf, nwaves, k, ctc, impl;
 };
 
 struct Plan {

@@ -764,13 +764,16 @@ int tsphere_accumulate(float* dst, const float* src, long long n, tsphere_stream
 }
 
 // ------------------------------------------------------------ tile tuning
-static LegTile to_tile(const tsphere_leg_tile& t) { return LegTile{t.nf, t.nwaves, t.k, t.ctc}; }
+static LegTile to_tile(const tsphere_leg_tile& t) {
+  return LegTile{t.nf, t.nwaves, t.k, t.ctc, t.impl};
+}
 static tsphere_leg_tile from_tile(const LegTile& t) {
   tsphere_leg_tile o{};
   o.nf = t.nf;
   o.nwaves = t.nwaves;
   o.k = t.k;
   o.ctc = t.ctc;
+  o.impl = t.impl;
   return o;
 }
 
@@ -778,8 +781,8 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
   if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
   const LegTile t = tile ? to_tile(*tile) : LegTile{};
-  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0)
-    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0");
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 2)
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0, 1 or 2)");
   // validate against a one-field and a max-fields launch
   for (int nf : {1, p->cfg.max_fields}) {
     const LegTile r = resolve_leg_tile(p->cfg.legendre, stage, p->npair, nf, t);
@@ -787,13 +790,15 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
       if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
         return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
     } else {
-      const bool kok = r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8;
+      const bool kok = r.impl == 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
+                                   : (r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8);
       if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.nwaves < 1 || r.nwaves > 4 || !kok ||
           64LL * r.nwaves * r.k < p->npair)
         return fail(TSPHERE_ERR_INVALID,
-                    "FP32 tile nf=%d nwaves=%d k=%d: need nf in {1,2,4}, nwaves in 1..4, "
-                    "k in {1,2,4,8}, 64*nwaves*k >= %d latitude pairs",
-                    r.nf, r.nwaves, r.k, p->npair);
+                    "FP32 tile impl=%d nf=%d nwaves=%d k=%d: need nf in {1,2,4}, nwaves in "
+                    "1..4, k in {1,2,4,8} (VALU) or {1,2,4} (matrix cores), "
+                    "64*nwaves*k >= %d latitude pairs",
+                    r.impl, r.nf, r.nwaves, r.k, p->npair);
     }
   }
   p->tile[stage] = t;
@@ -816,19 +821,21 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
     const int ct = (2 * nfield + 15) / 16;
     for (int ctc : {1, 2, 4})
-      if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc});
+      if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0});
     return c;
   }
-  for (int nf : {1, 2, 4}) {
-    if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
-    for (int nw = 1; nw <= 4; nw++) {
-      int k = 1;
-      while (k <= 8 && 64 * nw * k < p->npair) k *= 2;
-      if (k > 8) continue;
-      if (nw > 1 && 64 * (nw - 1) * k >= p->npair) continue;  // a wave with no pairs
-      c.push_back(LegTile{nf, nw, k, 0});
+  for (int impl : {1, 2})
+    for (int nf : {1, 2, 4}) {
+      if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
+      for (int nw = 1; nw <= 4; nw++) {
+        const int kmax = impl == 2 ? 4 : 8;
+        int k = 1;
+        while (k <= kmax && 64 * nw * k < p->npair) k *= 2;
+        if (k > kmax) continue;
+        if (nw > 1 && 64 * (nw - 1) * k >= p->npair) continue;  // a wave with no pairs
+        c.push_back(LegTile{nf, nw, k, 0, impl});
+      }
     }
-  }
   return c;
 }
 

@@ -44,7 +44,8 @@ class PlanInfoC(C.Structure):
 
 
 class LegTileC(C.Structure):
-    _fields_ = [("nf", C.c_int), ("nwaves", C.c_int), ("k", C.c_int), ("ctc", C.c_int)]
+    _fields_ = [("nf", C.c_int), ("nwaves", C.c_int), ("k", C.c_int), ("ctc", C.c_int),
+                ("impl", C.c_int)]
 
 
 _vp = C.c_void_p
