@@ -554,8 +554,8 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
   const int r16 = lane & 15, kg = lane >> 4;
   float2* sab = reinterpret_cast<float2*>(smem);
   float* tiles = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
-  float* tl = tiles + wv * (MF_TL * TROW);
-  float* red = tiles + 4 * MF_TL * TROW;  // [4][256]
+  float* tl = tiles + wv * (2 * MF_TL * TROW);  // two tiles per wave
+  float* red = tiles + 4 * 2 * MF_TL * TROW;     // [4][256]
   const int f0 = blockIdx.y * NF;
   const int wp0 = wv * QW;
   const int par = r16 / (2 * NF), cf = (r16 % (2 * NF)) >> 1, ri = r16 & 1;
@@ -613,40 +613,53 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
           reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
               make_float2(0.f, 0.f);
 
+    // Software pipeline: while the matrix core contracts tile i, the VALU runs
+    // the recurrence of tile i + 1 into the other half of the wave's double
+    // tile (one step of two l per four products).
+    auto rec_step = [&](float* dst, int lt, int s) {
+      const int t = 2 * s;
+      if (t % kTile == 0) {
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+          if (c[k].ls == lt + t) {
+            c[k].y0 = c[k].s0;
+            c[k].y1 = c[k].s1;
+          }
+      }
+      const int i = lt + t - m;
+      const float2 a0 = sab[i + 2], a1 = sab[i + 3];
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        dst[t * TROW + lane + 64 * k] = c[k].y0;
+        dst[(t + 1) * TROW + lane + 64 * k] = c[k].y1;
+        advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+      }
+    };
+    constexpr int G = NKB / 4;  // groups of four products per tile
+    if (lblk <= L) {
+#pragma unroll
+      for (int st = 0; st < MF_TL / 2; ++st) rec_step(tl, lblk, st);
+    }
+    int cur = 0;
     for (int lt = lblk; lt <= L; lt += MF_TL) {
-      // 1) 16 values of each chain -> row t of the tile, column = pair
-#pragma unroll
-      for (int t = 0; t < MF_TL; t += 2) {
-        if (t % kTile == 0) {
-#pragma unroll
-          for (int k = 0; k < KC; ++k)
-            if (c[k].ls == lt + t) {
-              c[k].y0 = c[k].s0;
-              c[k].y1 = c[k].s1;
-            }
-        }
-        const int i = lt + t - m;
-        const float2 a0 = sab[i + 2], a1 = sab[i + 3];
-#pragma unroll
-        for (int k = 0; k < KC; ++k) {
-          tl[t * TROW + lane + 64 * k] = c[k].y0;
-          tl[(t + 1) * TROW + lane + 64 * k] = c[k].y1;
-          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
-        }
-      }
-      wave_sync_lds();
-      // 2) this wave's pairs: NKB products
+      wave_sync_lds();  // tile `cur` complete; the other half is free
+      const float* tc = tl + cur * (MF_TL * TROW);
+      float* tn = tl + (cur ^ 1) * (MF_TL * TROW);
       f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
-      const float* arow = tl + r16 * TROW + kg * NKB;
+      const float* arow = tc + r16 * TROW + kg * NKB;
 #pragma unroll
-      for (int kb = 0; kb < NKB; kb += 4) {
-        const float4 a4 = *reinterpret_cast<const float4*>(arow + kb);
-        acc = mf4(a4.x, wB[kb], acc);
-        acc = mf4(a4.y, wB[kb + 1], acc);
-        acc = mf4(a4.z, wB[kb + 2], acc);
-        acc = mf4(a4.w, wB[kb + 3], acc);
+      for (int g = 0; g < G; ++g) {
+        const float4 a4 = *reinterpret_cast<const float4*>(arow + 4 * g);
+        acc = mf4(a4.x, wB[4 * g], acc);
+        acc = mf4(a4.y, wB[4 * g + 1], acc);
+        acc = mf4(a4.z, wB[4 * g + 2], acc);
+        acc = mf4(a4.w, wB[4 * g + 3], acc);
+#pragma unroll
+        for (int st = (MF_TL / 2) * g / G; st < (MF_TL / 2) * (g + 1) / G; ++st)
+          rec_step(tn, lt + MF_TL, st);
       }
-      // 3) the waves' partial sums meet in LDS; one thread per (l, column)
+      cur ^= 1;
+      // the waves' partial sums meet in LDS; one thread per (l, column)
 #pragma unroll
       for (int q = 0; q < 4; ++q) red[wv * 256 + (4 * kg + q) * 16 + r16] = acc[q];
       __syncthreads();
@@ -663,7 +676,7 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
                                                                         (col & 1)] = s;
         }
       }
-      __syncthreads();  // red and the tiles are rewritten next iteration
+      __syncthreads();  // red is rewritten next iteration
     }
   }
 }
@@ -683,7 +696,7 @@ hipError_t analys_mf(const LegTables& T, int nfield, int nwaves, const float2* f
                      const Fields& q, hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   const size_t lds = sizeof(float2) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
-                     sizeof(float) * (4 * MF_TL * (size_t)(64 * KC + 4) + 4 * 256);
+                     sizeof(float) * (4 * 2 * MF_TL * (size_t)(64 * KC + 4) + 4 * 256);
   hipLaunchKernelGGL((k_leg_analys_mf<NF, KC>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q);
   return hipGetLastError();
