@@ -181,3 +181,72 @@ def test_round_trip_large(gpu, lmax):
     assert np.all(np.isfinite(back))
     assert e_rt < leg_tol(lmax) and e_s < leg_tol(lmax)
     plan.destroy()
+
+
+@pytest.mark.parametrize("legendre", ["fp32", "bf16x3"])
+def test_sixty_four_fields_lmax511(gpu, legendre):
+    """The batched SHT configuration: 64 fields at lmax 511 on 512 x 1024,
+    a_lm with sigma_l = 1/(l+1), seed = field index, synthesised and analysed
+    in one launch per stage.  Every field's round trip, and the synthesis /
+    analysis of three fields against the f64 oracle, within the form's
+    tolerance (FP32: the plain-FP32 bound of this file; split-BF16: 1e-5 per
+    transform, 2e-5 per round trip)."""
+    import tsphere
+
+    nf, lmax = 64, 511
+    cfg = O.Cfg(lmax, lmax, 512, 1024)
+    emit(f"64 fields lmax 511 {legendre}")
+    plan = tsphere.ShtPlan.create(tsphere.ShtConfig(lmax, lmax, 512, 1024), legendre=legendre,
+                                  max_fields=nf)
+    qs = np.stack([_random_field_spectrum(cfg, f) for f in range(nf)])
+    qb = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
+    sb = tsphere.DeviceBuffer(nf * cfg.npts)
+    ob = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
+    s = plan.stream
+    qb.upload(qs, s)
+    plan.encode_synth_into(s, plan.create_synth_binding(qb, sb, nf, 2 * cfg.nlm, cfg.npts))
+    plan.encode_analys_into(s, plan.create_analys_binding(sb, ob, nf, cfg.npts, 2 * cfg.nlm))
+    spat = sb.download(s).reshape(nf, -1)
+    back = ob.download(s).reshape(nf, -1)
+    tol, tol_rt = (leg_tol(lmax), leg_tol(lmax)) if legendre == "fp32" else (1e-5, 2e-5)
+    rts = [O.rel_l2(back[f], qs[f]) for f in range(nf)]
+    emit(f"  round trips: max {max(rts):.2e} mean {np.mean(rts):.2e}")
+    assert max(rts) < tol_rt
+    ref = O.ShtReference(cfg)
+    for f in (0, 31, 63):
+        es = O.rel_l2(spat[f], ref.synth_fast(qs[f].astype(np.float64)))
+        ea = O.rel_l2(back[f], ref.analys_fast(spat[f].astype(np.float64)))
+        emit(f"  field {f}: synth {es:.2e} analys {ea:.2e} vs f64")
+        assert es < tol and ea < tol
+    plan.destroy()
+
+
+def test_napi_binding_transform(gpu, tmp_path):
+    """A node process drives the transforms through the N-API addon exactly as
+    the reference's ShtPlan would (create, device buffers, encode, read
+    back); its synthesis matches the f64 oracle and its round trip the input."""
+    import os
+    import shutil
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = os.path.join(root, "turing-sphere_amd", "napi", "synth_file.js")
+    node = shutil.which("node")
+    if node is None or not os.path.exists(os.path.join(os.path.dirname(script),
+                                                       "tsphere_napi.node")):
+        pytest.skip("node or the built addon is not available")
+    cfg = O.Cfg(63, 63, 64, 128)
+    emit("N-API synth/analys lmax63 64x128 (node child process)")
+    q = _random_field_spectrum(cfg, 17)
+    fin, fs, fq = (str(tmp_path / n) for n in ("q.f32", "spat.f32", "back.f32"))
+    q.astype(np.float32).tofile(fin)
+    r = subprocess.run([node, script, "63", "64", "128", fin, fs, fq], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    spat = np.fromfile(fs, np.float32)
+    back = np.fromfile(fq, np.float32)
+    ref = O.ShtReference(cfg)
+    e_s = O.rel_l2(spat, ref.synth_fast(q.astype(np.float64)))
+    e_rt = O.rel_l2(back, q)
+    emit(f"  {r.stdout.strip()} synth vs f64 {e_s:.2e} round trip {e_rt:.2e}")
+    assert e_s < leg_tol(63) and e_rt < leg_tol(63)

@@ -1,0 +1,52 @@
+"""The N-API binding (turing-sphere_amd/napi) that a node host -- the
+reference's TypeScript ShtPlan / ModelPlan -- would load: it loads in node,
+exports the calls, and reports the reference's configuration errors with its
+messages (layout.ts:34-43).  CPU only; the GPU test drives a transform."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "turing-sphere_amd", "napi", "tsphere_napi.node")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is not available")
+
+
+def node(js: str) -> dict:
+    r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def test_addon_loads_and_exports():
+    from tsphere import _lib
+
+    out = node(f"""const a = require({json.dumps(ADDON)});
+      console.log(JSON.stringify({{version: a.version(), names: Object.getOwnPropertyNames(a).sort()}}));""")
+    assert out["version"] == _lib.load().tsphere_version()
+    for name in ("planCreate", "planQuery", "planDestroy", "synth", "analys", "malloc", "free",
+                 "upload", "download", "moduleCompile", "moduleKernel", "kernelLaunch",
+                 "graphBegin", "graphEnd", "graphLaunch", "streamCreate", "streamSync"):
+        assert name in out["names"]
+
+
+def test_addon_config_errors():
+    out = node(f"""const a = require({json.dumps(ADDON)});
+      const r = [];
+      for (const cfg of [{{lmax: 31, mmax: 31, nlat: 31, nphi: 64}},
+                         {{lmax: 31, mmax: 31, nlat: 32, nphi: 62}}]) {{
+        try {{ a.planCreate(cfg); r.push('created'); }}
+        catch (e) {{ r.push([e.name, e.message, e.code]); }}
+      }}
+      let te = null;
+      try {{ a.synth(1, 2, 3); }} catch (e) {{ te = e.name; }}
+      console.log(JSON.stringify({{r, te}}));""")
+    (n1, m1, c1), (n2, m2, c2) = out["r"]
+    assert n1 == n2 == "ShtConfigError" and c1 == c2 == 1
+    assert "nlat must be an integer > lmax" in m1 and "nphi must be an integer >= 2*mmax+1" in m2
+    assert out["te"] == "TypeError"   # plain numbers are not handles

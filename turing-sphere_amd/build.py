@@ -42,7 +42,33 @@ def build(verbose: bool = False) -> str:
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(tmp, OUT)
+    build_napi(verbose)
     return OUT
+
+
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+NAPI_OUT = os.path.join(HERE, "napi", "tsphere_napi.node")
+
+
+def build_napi(verbose: bool = False):
+    """The N-API addon (napi/tsphere_napi.cc) for a node host, when node's
+    headers are installed; returns its path, or None when they are not."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print(f"node headers not found under {NODE_INCLUDE}: N-API addon not built",
+                  file=sys.stderr)
+        return None
+    tmp = NAPI_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-Wall", "-I", NODE_INCLUDE,
+           "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "napi", "tsphere_napi.cc"),
+           "-L", HERE, "-ltsphere", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"N-API addon build failed ({res.returncode}):\n{res.stderr}")
+    os.replace(tmp, NAPI_OUT)
+    return NAPI_OUT
 
 
 if __name__ == "__main__":
