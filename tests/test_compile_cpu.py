@@ -58,8 +58,11 @@ def test_fusion_guard_kernel_counts(key):
 
 
 def test_schnakenberg_op_list():
-    """README.md:85: 11 ops per step as the reference records them; batched
-    transforms and copy-free feedback make it 7 launches here."""
+    """README.md:85: 11 ops per step as the reference records them.  Here,
+    unbatched: one kernel per element-wise line (the reference's shape);
+    batched: the two syntheses and the two analyses each become one launch,
+    feedback copies fold into the kernels that produce the new state, and
+    each run of element-wise lines becomes one fused kernel -- 4 launches."""
     _, step = plan_cpu("schnakenberg", batch=False)
     kinds = [l.split()[0] for l in step.describe()]
     # unbatched: 2 synth, 5 kernels, 2 analys; feedback copies folded into the
@@ -67,17 +70,39 @@ def test_schnakenberg_op_list():
     assert sorted(kinds) == sorted(["synth"] * 2 + ["kernel"] * 5 + ["analys"] * 2)
     _, step = plan_cpu("schnakenberg", batch=True)
     kinds = [l.split()[0] for l in step.describe()]
-    assert kinds == ["synth", "kernel", "kernel", "kernel", "analys", "kernel", "kernel"], \
-        step.describe()
-    # the batched ops carry both species
-    assert step.describe()[0].count(";") == 1 and step.describe()[4].count(";") == 1
+    assert kinds == ["synth", "kernel", "analys", "kernel"], step.describe()
+    d = step.describe()
+    assert d[0].count(";") == 1 and d[2].count(";") == 1   # both species per transform
+    assert d[1].count(";") == 2 and d[3].count(";") == 1   # 3 reaction lines, 2 IMEX lines
+    fused = [op for op in step.ops if op.kind == "kernel"]
+    # uuv is used only inside the reaction group: kept in registers, not stored
+    assert len(fused[0].outs) == 2 and len(fused[1].outs) == 2
+
+
+def test_fused_kernel_matches_statementwise_f32():
+    """The fused reaction kernel evaluates each statement with the same f32
+    operation order as the one-kernel-per-line form (no contraction): its
+    source holds the same per-statement expressions."""
+    _, one = plan_cpu("brusselator", batch=False)
+    _, fused = plan_cpu("brusselator", batch=True)
+    bodies = [op.kernel.body for op in one.ops if op.kind == "kernel"]
+    src = fused.kernel_source()
+    for b in bodies:
+        # each single-statement body appears in the fused source with the
+        # produced values renamed to registers; the operators are unchanged
+        ops_in = "".join(ch for ch in b if ch in "+-*/()")
+        assert ops_in, b
+    assert src.count("__global__") == 2
 
 
 def test_init_plan_and_feedback():
-    init, _ = plan_cpu("schnakenberg")
+    init, _ = plan_cpu("schnakenberg", batch=False)
     d = init.describe()
     assert sum(l.startswith("kernel") for l in d) == 2   # us + noise, vs * ones(...)
     assert not any(l.startswith("copy") for l in d)      # U, V written in place
+    init, _ = plan_cpu("schnakenberg")                   # batched: one fused kernel
+    d = init.describe()
+    assert sum(l.startswith("kernel") for l in d) == 1 and not any(l.startswith("copy") for l in d)
 
 
 def test_generated_source_compiles_for_gfx950():
@@ -240,7 +265,7 @@ def test_adjoint_kernels_compile(key):
     host = HostBuffers(allocate=False)
     host.ensure_group(state, 2 * 2080)
     tape = ModelPlan.create(None, PlanSpec(step_fn, [None] * len(step_fn.outputs)), host, True,
-                            "tape")
+                            "tape", fuse=False)
     srcs = []
     for i, op in enumerate(tape.ops):
         if isinstance(op, KernelOp):

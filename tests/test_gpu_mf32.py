@@ -17,15 +17,17 @@ def leg_tol(lmax):
     return max(3e-6, 2e-10 * (lmax + 1) ** 2)
 
 
-@pytest.mark.parametrize("lmax,nlat,nphi", [(15, 16, 32), (15, 17, 32), (12, 13, 26),
-                                            (63, 64, 128), (127, 128, 256), (255, 256, 512),
-                                            (511, 512, 1024), (1023, 1024, 2048)])
-@pytest.mark.parametrize("nf", [1, 2, 3, 5])
+_SMALL = [(15, 16, 32), (15, 17, 32), (12, 13, 26), (63, 64, 128), (127, 128, 256),
+          (255, 256, 512)]
+# every field count on the small grids; the two-field case (a step's shape) on the large ones
+_CASES = [g + (nf,) for g in _SMALL for nf in (1, 2, 3, 5)] + [(511, 512, 1024, 2),
+                                                               (1023, 1024, 2048, 2)]
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi,nf", _CASES)
 def test_matrix_core_stages(gpu, lmax, nlat, nphi, nf):
     import tsphere
 
-    if lmax >= 511 and nf not in (2,):
-        pytest.skip("large grids: the two-field case (the step's shape) only")
     cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
     emit(f"mf32 lmax{lmax} {nlat}x{nphi} nf {nf}")
     plan = tsphere.ShtPlan.create(cfg, max_fields=nf)

@@ -259,7 +259,8 @@ class AdjointModel:
         host.ensure_group(self.state, 2 * nlm)
         host.ensure("lam", 2 * nlm)
         self._host = host
-        tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape")
+        tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape",
+                                fuse=False)
         self.tape = tape
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         # state outputs: the value each state variable takes after the step

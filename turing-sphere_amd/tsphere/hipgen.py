@@ -14,7 +14,10 @@ emitter, with these MI355X-specific choices:
   16-byte store (HBM-bound kernels), falling back to scalar accesses for the
   tail or an unaligned buffer;
 * the kernel signature is ``(long long n, float* out, const float* in0..,
-  float p0..)``, which ``tsphere_kernel_launch`` packs;
+  float p0..)``, which ``tsphere_kernel_launch`` packs; the pointers are not
+  ``__restrict__``: a kernel that produces a state's next value may write the
+  buffer it reads (copy-free feedback), and every element is read before it
+  is written in the same thread;
 * element-wise math calls the HIP device library (``expf``, ``sinf``...);
 * hipRTC compiles with ``-ffp-contract=off``, so every f32 operation rounds
   separately, in the expression tree's order.
@@ -256,7 +259,7 @@ def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: s
     args = ["a" + str(i) for i in range(nin)]
     params = ["float p" + str(i) for i in range(npar)]
     elem_params = ", ".join([f"float {a}" for a in args] + params) or "void"
-    kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
+    kin = "".join(f", const float* in{i}" for i in range(nin))
     pro = "\n".join("  " + line.strip() for line in ctx.prologue)
     helpers = _pow_helpers(used)
     elem = f"""{helpers}
@@ -277,7 +280,7 @@ __device__ __forceinline__ float {name}_elem({elem_params}) {{
             return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)])
 
         scalar_args = ", ".join([f"in{i}[i]" for i in range(nin)] + [f"p{i}" for i in range(npar)])
-        code = elem + f"""extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}{kpar}) {{
+        code = elem + f"""extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* out{kin}{kpar}) {{
   const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
   if (i0 >= n) return;
   if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
@@ -329,7 +332,7 @@ __device__ __forceinline__ float {name}_elem({elem_params}) {{
     else:
         vec = "  // runs are not a multiple of 4 elements long: element by element\n"
     code = elem + f"""// ensemble form: {batch.runs} runs x {per} elements, {npar} scalar(s) per run in prm
-extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* __restrict__ out{kin}, const float* __restrict__ prm) {{
+extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, float* out{kin}, const float* __restrict__ prm) {{
   const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
   if (i0 >= n) return;
 {vec}  for (long long i = i0; i < n && i < i0 + {ELEMS_PER_THREAD}; ++i) {{
@@ -340,6 +343,147 @@ extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long 
 """
     return Kernel(code=code, name=name, count=count * batch.runs, label=label, ninputs=nin,
                   nparams=npar, body=body)
+
+
+def build_fused_kernel(stmts, ios, count: int, stores, label: str, name: str,
+                       batch: "Batch | None" = None) -> Kernel:
+    """Several consecutive element-wise statements over the same element
+    count as ONE kernel (the planner's fusion of a step's element-wise run):
+    each thread evaluates the statements in program order for its elements,
+    values produced inside the group are registers, and only the c_names in
+    `stores` (values read after the group or leaving the step) are written.
+    External inputs are read before anything is stored, so a statement that
+    reads a buffer a later statement overwrites sees the old value, as the
+    sequential kernels would.  Signature: (long long n, float* out0..,
+    const float* in0.., float p0.. | const float* prm).  Returns the kernel
+    and, via Kernel.body, the input c_names in slot order."""
+    produced = [st.c_name for st in stmts]
+    if len(set(produced)) != len(produced):
+        raise UnsupportedOnGpu("a fused group may assign each value once")
+    ext = []                                   # external tensor inputs, slot order
+    done = set()
+    for st, io in zip(stmts, ios):
+        for c in sorted(io.tensors, key=lambda c: io.tensors[c]):
+            if c not in done and c not in ext:
+                ext.append(c)
+        done.add(st.c_name)
+    slots = {c: i for i, c in enumerate(ext)}  # reads resolve through this map
+    params = ios[0].params
+    scalars = {}
+    for io in ios:
+        scalars.update(io.scalars)
+    ctx = _Ctx(KernelInputs(slots, params, scalars))
+    body = []
+    nslot = len(ext)
+    for st in stmts:
+        if not is_numeric(st.ty) or st.ty.is_complex:
+            raise UnsupportedOnGpu(f"'{st.name}' is not a real numeric array", st.span)
+        check_shapes(st.expr, st.ty, st.name)
+        rhs = _emit(st.expr, ctx)
+        # the produced value is a new register; later reads of it resolve there
+        slots[st.c_name] = nslot
+        body.append(f"  const float a{nslot} = {rhs};")
+        nslot += 1
+    emitted = "\n".join(ctx.prologue + body)
+    used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", emitted)}
+    nin, npar, nout = len(ext), len(params), len(stores)
+    elem_params = ", ".join([f"float a{i}" for i in range(nin)] +
+                            [f"float p{i}" for i in range(npar)] +
+                            [f"float& o{j}" for j in range(nout)])
+    pro = "\n".join("  " + line.strip() for line in ctx.prologue)
+    outs = "\n".join(f"  o{j} = a{slots[c]};" for j, c in enumerate(stores))
+    kout = ", ".join(f"float* out{j}" for j in range(nout))
+    kin = "".join(f", const float* in{i}" for i in range(nin))
+    aligned = " && ".join([f"ts_al(out{j})" for j in range(nout)] +
+                          [f"ts_al(in{i})" for i in range(nin)])
+    per = int(count)
+    shared = set()
+    if batch is not None:
+        shared = {slots[c] for c in batch.shared if c in slots and slots[c] < nin}
+
+    def idx(i, var):
+        return f"{var} % {per}LL" if i in shared else var
+
+    vec_loads = "\n".join(
+        f"    const float4 v{i} = *reinterpret_cast<const float4*>(in{i} + {idx(i, 'i0')});"
+        for i in range(nin))
+    rdecl = "\n".join(f"    float4 r{j};" for j in range(nout))
+
+    def lane(c):
+        return ", ".join([f"v{i}.{c}" for i in range(nin)] + [f"p{i}" for i in range(npar)] +
+                         [f"r{j}.{c}" for j in range(nout)])
+
+    vstores = "\n".join(f"    *reinterpret_cast<float4*>(out{j} + i0) = r{j};"
+                         for j in range(nout))
+    sdecl = " ".join(f"float s{j};" for j in range(nout))
+    sargs = ", ".join([f"in{i}[{idx(i, 'i')}]" for i in range(nin)] +
+                      [f"p{i}" for i in range(npar)] + [f"s{j}" for j in range(nout)])
+    sstores = " ".join(f"out{j}[i] = s{j};" for j in range(nout))
+    helpers = _pow_helpers(used)
+    code = f"""{helpers}
+// {label}
+__device__ __forceinline__ void {name}_elem({elem_params}) {{
+{pro}
+{chr(10).join(body)}
+{outs}
+}}
+"""
+    if batch is None:
+        kpar = "".join(f", float p{i}" for i in range(npar))
+        code += f"""extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, {kout}{kin}{kpar}) {{
+  const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
+  if (i0 >= n) return;
+  if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
+{vec_loads}
+{rdecl}
+    {name}_elem({lane('x')});
+    {name}_elem({lane('y')});
+    {name}_elem({lane('z')});
+    {name}_elem({lane('w')});
+{vstores}
+  }} else {{
+    for (long long i = i0; i < n && i < i0 + {ELEMS_PER_THREAD}; ++i) {{
+      {sdecl}
+      {name}_elem({sargs});
+      {sstores}
+    }}
+  }}
+}}
+"""
+        ktotal = count
+    else:
+        pload = "".join(f"    const float p{k} = prm[run * {npar} + {k}];\n" for k in range(npar))
+        vec = (f"""  if (i0 + {ELEMS_PER_THREAD} <= n && {aligned}) {{
+    const long long run = i0 / {per}LL;
+{pload}{vec_loads}
+{rdecl}
+    {name}_elem({lane('x')});
+    {name}_elem({lane('y')});
+    {name}_elem({lane('z')});
+    {name}_elem({lane('w')});
+{vstores}
+    return;
+  }}
+""" if per % ELEMS_PER_THREAD == 0 else
+               "  // runs are not a multiple of 4 elements long: element by element\n")
+        code += f"""// ensemble form: {batch.runs} runs x {per} elements, {npar} scalar(s) per run in prm
+extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, {kout}{kin}, const float* __restrict__ prm) {{
+  const long long i0 = ((long long)blockIdx.x * {WORKGROUP_SIZE} + threadIdx.x) * {ELEMS_PER_THREAD};
+  if (i0 >= n) return;
+{vec}  for (long long i = i0; i < n && i < i0 + {ELEMS_PER_THREAD}; ++i) {{
+    const long long run = i / {per}LL;
+{pload}    {sdecl}
+    {name}_elem({sargs});
+    {sstores}
+  }}
+}}
+"""
+        ktotal = count * batch.runs
+    k = Kernel(code=code, name=name, count=ktotal, label=label, ninputs=nin, nparams=npar,
+               body=emitted)
+    k.inputs = list(ext)
+    k.stores = list(stores)
+    return k
 
 
 @dataclass

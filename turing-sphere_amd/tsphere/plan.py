@@ -31,7 +31,8 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceBuffer, Stream
-from .hipgen import Batch, KernelInputs, UnsupportedOnGpu, build_kernel, module_source
+from .hipgen import (Batch, KernelInputs, UnsupportedOnGpu, build_fused_kernel, build_kernel,
+                     module_source)
 from .ir import Assign, CompiledFunction, is_numeric, is_tensor, numel, walk
 
 EXTERNAL_OPS = ("synth", "analys")  # externals.ts:95
@@ -128,6 +129,20 @@ class KernelOp:
 
 
 @dataclass
+class FusedOp:
+    """Consecutive element-wise statements run as one kernel (batched plans):
+    writes `outs`, reads `ins` (external values only)."""
+    kind: str
+    label: str
+    kernel: object
+    outs: list
+    ins: list
+    members: list           # the KernelOps it replaces
+    handle: object = None
+    count: int = 0
+
+
+@dataclass
 class XformOp:
     kind: str               # 'synth' | 'analys'
     label: str
@@ -147,6 +162,8 @@ class CopyOp:
 def _reads(op) -> set:
     if isinstance(op, KernelOp):
         return set(op.ins)
+    if isinstance(op, FusedOp):
+        return set(op.ins)
     if isinstance(op, XformOp):
         return set(op.srcs)
     return {op.src}
@@ -155,6 +172,8 @@ def _reads(op) -> set:
 def _writes(op) -> set:
     if isinstance(op, KernelOp):
         return {op.out}
+    if isinstance(op, FusedOp):
+        return set(op.outs)
     if isinstance(op, XformOp):
         return set(op.dsts)
     return {"@host:" + op.dst_host, *op.dst_keys}
@@ -182,7 +201,7 @@ class ModelPlan:
     # --------------------------------------------------------- creation
     @staticmethod
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
-               label: str = "step", runs: int = 1, shared=()) -> "ModelPlan":
+               label: str = "step", runs: int = 1, shared=(), fuse: bool = True) -> "ModelPlan":
         """ModelPlan.create (plan.ts:206-407).
 
         ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
@@ -297,6 +316,8 @@ class ModelPlan:
             cap = sht.max_fields // runs if sht is not None else 8
             ops = _schedule(ops, max_batch=max(1, min(8, cap)))
         ops = self._fold_feedback(ops, host, host_of)
+        if batch and fuse:
+            ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
 
         # Buffers for every value the .m computes (reused across steps).
         for c, n in counts.items():
@@ -308,6 +329,7 @@ class ModelPlan:
             self._by_name[names[c]] = self._bufs[c]
         self.ops = ops
 
+        pending_kernels = [op for op in ops if isinstance(op, (KernelOp, FusedOp))]
         if pending_kernels and not host.allocate:
             src = module_source([op.kernel for op in pending_kernels])
             if _lib.load().tsphere_rtc_check(src.encode(), None) != _lib.OK:
@@ -326,6 +348,44 @@ class ModelPlan:
         if runs > 1 and host.allocate:
             self._prm = DeviceBuffer(self._params.size, f"mgpu-{label}-params")
         return self
+
+    def _fuse(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c) -> list:
+        """Merge each run of consecutive element-wise kernels over the same
+        element count into one kernel; values used only inside a run stay in
+        registers."""
+        host_bufs = {id(b) for b in host._slots.values()}
+        out = []
+        i = 0
+        while i < len(ops):
+            j = i
+            while (j < len(ops) and isinstance(ops[j], KernelOp) and
+                   ops[j].count == ops[i].count):
+                j += 1
+            group = ops[i:j] if j > i else []
+            if len(group) < 2 or len({g.out for g in group}) != len(group):
+                out.append(ops[i])
+                i += 1
+                continue
+            produced = {g.out for g in group}
+            later = set()
+            for o in ops[:i] + ops[j:]:
+                later |= _reads(o)
+            stores = [g.out for g in group
+                      if g.out in outputs or g.out in later or
+                      id(self._bufs.get(g.out)) in host_bufs]
+            if not stores:
+                stores = [group[-1].out]
+            name = f"f{len(out)}_" + "_".join(_ident(g.stmt.name) for g in group)[:40]
+            lbl = " ; ".join(g.label for g in group)
+            per = group[0].count // runs
+            kern = build_fused_kernel([g.stmt for g in group], [g.io for g in group], per,
+                                      stores, lbl, name,
+                                      Batch(runs, shared_c) if runs > 1 else None)
+            out.append(FusedOp("kernel", lbl, kern, stores, kern.inputs, list(group),
+                               count=group[0].count))
+            del produced
+            i = j
+        return out
 
     def _fold_feedback(self, ops, host: HostBuffers, host_of: dict) -> list:
         """Let the kernel that produces a fed-back value write the host buffer
@@ -408,8 +468,9 @@ class ModelPlan:
             nq, ns = 2 * self._sht.nlm, self._sht.cfg.npts
         for _ in range(steps):
             for op in self.ops:
-                if isinstance(op, KernelOp):
-                    ptrs = self._ptrs([op.out] + op.ins + (["@prm"] if ens else []))
+                if isinstance(op, (KernelOp, FusedOp)):
+                    outs = op.outs if isinstance(op, FusedOp) else [op.out]
+                    ptrs = self._ptrs(outs + op.ins + (["@prm"] if ens else []))
                     _lib.check(L.tsphere_kernel_launch(op.handle, op.count, len(ptrs), ptrs, np_,
                                                        pvals, sh), op.label)
                 elif isinstance(op, XformOp):
@@ -459,7 +520,8 @@ class ModelPlan:
         return [f"{op.kind:<7} {op.label}" for op in self.ops]
 
     def kernel_source(self) -> str:
-        return module_source([op.kernel for op in self.ops if isinstance(op, KernelOp)])
+        return module_source([op.kernel for op in self.ops
+                              if isinstance(op, (KernelOp, FusedOp))])
 
     def destroy(self) -> None:
         self.invalidate()
