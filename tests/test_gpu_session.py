@@ -68,7 +68,9 @@ def test_run_record_and_dump(gpu, tmp_path):
     for k in ("command", "spec", "model", "backend", "grid", "compiled", "digest", "throughput",
               "latency", "state"):
         assert k in rec
-    assert rec["compiled"]["kernels"] == 5 and rec["state"]["finite"]
+    # 5 element-wise lines, fused into 2 launches (reaction run, IMEX run)
+    assert rec["compiled"]["elementwiseStatements"] == 5 and rec["compiled"]["kernels"] == 2
+    assert rec["state"]["finite"]
     d = json.loads(dump.read_text())
     assert d["digest"]["n"] == len(d["state"]) == 2 * rec["grid"]["nlm"]
     rec2 = run(spec, batch=8, latency_steps=10)

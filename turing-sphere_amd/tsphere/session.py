@@ -473,7 +473,9 @@ def run(spec: RunSpec, batch: int = 16, latency_steps: int = 200, digest: bool =
     try:
         s.seed(spec.seed)
         plan = s.describe()
-        kernels = sum(1 for l in plan["step"] if l.startswith("kernel"))
+        klines = [l for l in plan["step"] if l.startswith("kernel")]
+        kernels = len(klines)                                   # generated kernels launched
+        statements = sum(1 + l.count(" ; ") for l in klines)    # element-wise .m lines
         s.step(spec.warmup)
         s.sync()
         batches = max(1, math.ceil(spec.steps / batch))
@@ -517,7 +519,8 @@ def run(spec: RunSpec, batch: int = 16, latency_steps: int = 200, digest: bool =
                         "precision": "fp32" if legendre == "fp32" else "bf16x3"},
             "grid": {"lmax": s.cfg.lmax, "nlat": s.cfg.nlat, "nphi": s.cfg.nphi,
                      "nlm": s.sht.nlm},
-            "compiled": {"opsPerStep": len(plan["step"]), "kernels": kernels},
+            "compiled": {"opsPerStep": len(plan["step"]), "kernels": kernels,
+                         "elementwiseStatements": statements},
             "digest": dg.as_dict() if dg else None,
             "throughput": {"batch": batch, "msPerStep": thr, "stepsPerSec": 1e3 / thr,
                            "encodeMsPerStep": enc * 1e3 / run_n},
