@@ -188,6 +188,7 @@ def batched_sht(lmax: int, nf: int, legendre: str, reps: int = 10):
 
     cfg = tsphere.ShtConfig(lmax, lmax, lmax + 1, 2 * (lmax + 1))
     p = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=nf)
+    tiles = p.tune(nf, reps=3)
     q = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
     sb = tsphere.DeviceBuffer(nf * cfg.npts)
     q.fill(0.01, p.stream)
@@ -198,7 +199,8 @@ def batched_sht(lmax: int, nf: int, legendre: str, reps: int = 10):
     p.destroy()
     f_leg, f_fft = flops_per_transform(lmax, lmax + 1, 2 * (lmax + 1))
     return {"round_trips_per_s": nf / (ms * 1e-3), "ms_per_batch": ms,
-            "sht_gflops": 2 * nf * (f_leg + f_fft) / (ms * 1e-3) / 1e9}
+            "sht_gflops": 2 * nf * (f_leg + f_fft) / (ms * 1e-3) / 1e9,
+            "tiles": {k: tiles[k] for k in ("synth", "analys")}}
 
 
 def ensemble_bench(steps: int = 200, legendre: str = "fp32"):
@@ -294,7 +296,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--legendre", default="fp32", choices=["fp32", "bf16x3"],
+    ap.add_argument("--legendre", default="fp32", choices=["fp32", "f64rec", "bf16x3"],
                     help="the variant reported as `value` (both are measured)")
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--cpu-rows", type=int, default=1024)
@@ -308,7 +310,7 @@ def main() -> int:
         return 2
 
     variants = {}
-    for leg in ("fp32", "bf16x3"):
+    for leg in ("fp32", "f64rec", "bf16x3"):
         variants[leg] = run_variant(leg, args.steps, args.warmup, args.graph_steps,
                                     not args.quick)
     main_v = variants[args.legendre]
@@ -331,7 +333,7 @@ def main() -> int:
 
     extras = {}
     if not args.quick:
-        for leg in ("fp32", "bf16x3"):
+        for leg in ("fp32", "f64rec", "bf16x3"):
             extras[f"sht_64fields_l511_{leg}"] = batched_sht(511, 64, leg)
             log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
         extras["ensemble_allencahn_256runs"] = ensemble_bench()
@@ -355,7 +357,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.legendre == "fp32" else "bf16x3",
+        "dtype": {"fp32": "f32", "f64rec": "f32 (f64 recurrence)", "bf16x3": "bf16x3"}[args.legendre],
         "data": "synthetic",
         "config": {"workload": "brusselator IMEX Euler, lmax 1023, Gauss 1024x2048, 2 species, "
                                "FP32 state, dt 0.002", "model": "brusselator", "lmax": LMAX,

@@ -88,6 +88,7 @@ int tsphere_event_elapsed(tsphere_event start, tsphere_event end, float *ms);
 
 #define TSPHERE_LEG_FP32 0   /* Legendre stage on the FP32 VALU                */
 #define TSPHERE_LEG_BF16X3 1 /* Legendre stage on MFMA, split-BF16 x3, FP32 acc */
+#define TSPHERE_LEG_F64REC 2 /* FP64 recurrence, FP32 sums on the VALU          */
 
 typedef struct {
   int lmax, mmax, nlat, nphi; /* validated as layout.ts:34-43               */

@@ -95,6 +95,81 @@ __device__ __forceinline__ void load_chain(const LegTables& T, int m, int j, Cha
   c.y0 = c.y1 = 0.f;
 }
 
+// Recurrence precision of the VALU kernels.  RecF: f32 state, cos(theta) as an
+// f32 hi + lo pair (4 VALU ops per l).  RecD: f64 state from the f64 tables
+// and seeds (3 f64 ops per l; on gfx950 an f64 FMA issues at ~88% of the f32
+// rate), rounded to f32 once per value before the FP32 sums: the FP32 form's
+// error at high degree comes from rounding the recurrence state, which this
+// removes, at about the FP32 form's cost.
+struct RecF {
+  using Tables = LegTables;
+  using ab_t = float2;
+  using ct_t = float2;
+  using y_t = float;
+  static __device__ __forceinline__ ab_t zero_ab() { return make_float2(0.f, 0.f); }
+  static __device__ __forceinline__ void advance(ab_t a0, ab_t a1, ct_t ct, y_t& y0, y_t& y1) {
+    const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
+    y1 = act_fma(a1.x, t0, ct, a1.y * y1);
+    y0 = t0;
+  }
+};
+struct RecD {
+  using Tables = LegTables64;
+  using ab_t = double2;
+  using ct_t = double;
+  using y_t = double;
+  static __device__ __forceinline__ ab_t zero_ab() { return make_double2(0.0, 0.0); }
+  static __device__ __forceinline__ void advance(ab_t a0, ab_t a1, ct_t ct, y_t& y0, y_t& y1) {
+    const double t0 = fma(a0.x * y1, ct, a0.y * y0);
+    y1 = fma(a1.x * t0, ct, a1.y * y1);
+    y0 = t0;
+  }
+};
+
+template <class P>
+struct ChainT {
+  typename P::ct_t ct;
+  typename P::y_t y0, y1, s0, s1;
+  int ls;  // seed position; INT_MAX = never significant up to lmax
+};
+
+template <class P>
+__device__ __forceinline__ void load_chain_t(const typename P::Tables& T, int m, int j,
+                                             ChainT<P>& c) {
+  if (j < T.npair) {
+    c.ct = T.ct[j];
+    const auto sd = T.seed[(long long)m * T.npair + j];
+    c.s0 = sd.y0;
+    c.s1 = sd.y1;
+    c.ls = sd.l;
+  } else {
+    c.ct = typename P::ct_t{};
+    c.s0 = c.s1 = 0;
+    c.ls = 0x7fffffff;
+  }
+  c.y0 = c.y1 = 0;
+}
+
+// Stage a row's (a, b) (in the recurrence's precision) and the fields' Q.
+template <int NF, class P>
+__device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, int f0,
+                                            int nfield, const Fields& Fq, bool with_q,
+                                            typename P::ab_t* sab, float2* sq, int pad) {
+  const int n = T.lmax - m + 1;
+  const long long base = lm_base(T.lmax, m);
+  for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
+    sab[i] = i < n ? T.ab[base + i] : P::zero_ab();
+    if (with_q) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int fld = min(f0 + f, nfield - 1);
+        sq[i * NF + f] =
+            i < n ? reinterpret_cast<const float2*>(Fq.at(fld))[base + i] : make_float2(0.f, 0.f);
+      }
+    }
+  }
+}
+
 // Stage a row's (a, b) and the fields' Q into LDS, zero-padded.
 template <int NF>
 __device__ __forceinline__ void stage_row(const LegTables& T, int m, int f0, int nfield,
@@ -116,13 +191,13 @@ __device__ __forceinline__ void stage_row(const LegTables& T, int m, int f0, int
 }
 
 // ------------------------------------------------------------- synthesis
-template <int NF, int K>
-__global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fields Fq,
+template <int NF, int K, class P>
+__global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfield, Fields Fq,
                                                    float2* __restrict__ fm) {
-  extern __shared__ float2 lds[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int L = T.lmax, M = T.mmax;
-  float2* sab = lds;
-  float2* sq = lds + (L + 1 + kPadQ);
+  auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  float2* sq = reinterpret_cast<float2*>(sab + (L + 1 + kPadQ));
   const int f0 = blockIdx.y * NF;
   const int nthr = blockDim.x;
 
@@ -131,14 +206,14 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fiel
     const int m = r == 0 ? mp : M - mp;
     if (r == 1 && m <= mp) break;
     __syncthreads();  // the previous row's readers are done
-    stage_row<NF>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
     __syncthreads();
 
-    Chain c[K];
+    ChainT<P> c[K];
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      load_chain(T, m, threadIdx.x + k * nthr, c[k]);
+      load_chain_t<P>(T, m, threadIdx.x + k * nthr, c[k]);
       lmin = min(lmin, c[k].ls);
     }
     lmin = wave_min(lmin);
@@ -165,16 +240,16 @@ __global__ __launch_bounds__(256) void k_leg_synth(LegTables T, int nfield, Fiel
           q0[f] = sq[(i + p) * NF + f];
           q1[f] = sq[(i + p + 1) * NF + f];
         }
-        const float2 a0 = sab[i + p + 2];
-        const float2 a1 = sab[i + p + 3];
+        const auto a0 = sab[i + p + 2];
+        const auto a1 = sab[i + p + 3];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            cfma(accE[k][f], c[k].y0, q0[f]);
-            cfma(accO[k][f], c[k].y1, q1[f]);
+            cfma(accE[k][f], (float)c[k].y0, q0[f]);
+            cfma(accO[k][f], (float)c[k].y1, q1[f]);
           }
-          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+          P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
         }
       }
     }
@@ -237,16 +312,16 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 }
 
 // -------------------------------------------------------------- analysis
-template <int NF, int K>
-__global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
+template <int NF, int K, class P>
+__global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nfield,
                                                     const float2* __restrict__ fm, Fields Fq) {
   constexpr int SPAN = 32 / NF;  // l-values per reduced span
   static_assert(SPAN % kTile == 0 && SPAN * NF * 2 == 64, "64 partials per lane");
-  extern __shared__ float2 lds[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][4][64];
   __shared__ int smin[4];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
-  float2* sab = lds;
+  auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
   const int f0 = blockIdx.y * NF;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nthr = blockDim.x, nwaves = nthr >> 6;
@@ -257,15 +332,15 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
     if (r == 1 && m <= mp) break;
     const long long base = lm_base(L, m);
     __syncthreads();
-    stage_row<NF>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
 
-    Chain c[K];
+    ChainT<P> c[K];
     float2 wp[K][NF], wm[K][NF];
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = tid + k * nthr;
-      load_chain(T, m, j, c[k]);
+      load_chain_t<P>(T, m, j, c[k]);
       lmin = min(lmin, c[k].ls);
       const int jc = min(j, T.npair - 1);
       const int js = nlat - 1 - jc;
@@ -318,21 +393,22 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
+          const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
             float* e = &acc[(p * NF + f) * 2];
             float* o = &acc[((p + 1) * NF + f) * 2];
-            e[0] = fmaf(c[k].y0, wp[k][f].x, e[0]);
-            e[1] = fmaf(c[k].y0, wp[k][f].y, e[1]);
-            o[0] = fmaf(c[k].y1, wm[k][f].x, o[0]);
-            o[1] = fmaf(c[k].y1, wm[k][f].y, o[1]);
+            e[0] = fmaf(y0, wp[k][f].x, e[0]);
+            e[1] = fmaf(y0, wp[k][f].y, e[1]);
+            o[0] = fmaf(y1, wm[k][f].x, o[0]);
+            o[1] = fmaf(y1, wm[k][f].y, o[1]);
           }
         }
         const int i = l - m;
-        const float2 a0 = sab[i + 2];
-        const float2 a1 = sab[i + 3];
+        const auto a0 = sab[i + 2];
+        const auto a1 = sab[i + 3];
 #pragma unroll
-        for (int k = 0; k < K; ++k) advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+        for (int k = 0; k < K; ++k) P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
       }
       const float v = transpose_reduce64(acc);
       red[buf][wv][lane] = v;
@@ -355,46 +431,49 @@ __global__ __launch_bounds__(256) void k_leg_analys(LegTables T, int nfield,
   }
 }
 
-size_t synth_lds(int lmax, int nf) { return sizeof(float2) * (lmax + 1 + kPadQ) * (1 + nf); }
+template <class P>
+size_t synth_lds(int lmax, int nf) {
+  return (sizeof(typename P::ab_t) + sizeof(float2) * nf) * (lmax + 1 + kPadQ);
+}
 
-template <int NF, int K>
-hipError_t synth_nfk(const LegTables& T, int nfield, int nwaves, const Fields& q, float2* fm,
-                     hipStream_t s) {
+template <int NF, int K, class P>
+hipError_t synth_nfk(const typename P::Tables& T, int nfield, int nwaves, const Fields& q,
+                     float2* fm, hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL((k_leg_synth<NF, K>), grid, dim3(64 * nwaves), synth_lds(T.lmax, NF), s, T,
-                     nfield, q, fm);
+  hipLaunchKernelGGL((k_leg_synth<NF, K, P>), grid, dim3(64 * nwaves), synth_lds<P>(T.lmax, NF),
+                     s, T, nfield, q, fm);
   return hipGetLastError();
 }
 
-template <int NF, int K>
-hipError_t analys_nfk(const LegTables& T, int nfield, int nwaves, const float2* fm,
+template <int NF, int K, class P>
+hipError_t analys_nfk(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
                       const Fields& q, hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL((k_leg_analys<NF, K>), grid, dim3(64 * nwaves),
-                     sizeof(float2) * (T.lmax + 1 + kPadAnalys), s, T, nfield, fm, q);
+  hipLaunchKernelGGL((k_leg_analys<NF, K, P>), grid, dim3(64 * nwaves),
+                     sizeof(typename P::ab_t) * (T.lmax + 1 + kPadAnalys), s, T, nfield, fm, q);
   return hipGetLastError();
 }
 
-template <int NF>
-hipError_t synth_nf(const LegTables& T, int nfield, const LegTile& t, const Fields& q, float2* fm,
-                    hipStream_t s) {
+template <int NF, class P>
+hipError_t synth_nf(const typename P::Tables& T, int nfield, const LegTile& t, const Fields& q,
+                    float2* fm, hipStream_t s) {
   switch (t.k) {
-    case 1: return synth_nfk<NF, 1>(T, nfield, t.nwaves, q, fm, s);
-    case 2: return synth_nfk<NF, 2>(T, nfield, t.nwaves, q, fm, s);
-    case 4: return synth_nfk<NF, 4>(T, nfield, t.nwaves, q, fm, s);
-    case 8: return synth_nfk<NF, 8>(T, nfield, t.nwaves, q, fm, s);
+    case 1: return synth_nfk<NF, 1, P>(T, nfield, t.nwaves, q, fm, s);
+    case 2: return synth_nfk<NF, 2, P>(T, nfield, t.nwaves, q, fm, s);
+    case 4: return synth_nfk<NF, 4, P>(T, nfield, t.nwaves, q, fm, s);
+    case 8: return synth_nfk<NF, 8, P>(T, nfield, t.nwaves, q, fm, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int NF>
-hipError_t analys_nf(const LegTables& T, int nfield, const LegTile& t, const float2* fm,
+template <int NF, class P>
+hipError_t analys_nf(const typename P::Tables& T, int nfield, const LegTile& t, const float2* fm,
                      const Fields& q, hipStream_t s) {
   switch (t.k) {
-    case 1: return analys_nfk<NF, 1>(T, nfield, t.nwaves, fm, q, s);
-    case 2: return analys_nfk<NF, 2>(T, nfield, t.nwaves, fm, q, s);
-    case 4: return analys_nfk<NF, 4>(T, nfield, t.nwaves, fm, q, s);
-    case 8: return analys_nfk<NF, 8>(T, nfield, t.nwaves, fm, q, s);
+    case 1: return analys_nfk<NF, 1, P>(T, nfield, t.nwaves, fm, q, s);
+    case 2: return analys_nfk<NF, 2, P>(T, nfield, t.nwaves, fm, q, s);
+    case 4: return analys_nfk<NF, 4, P>(T, nfield, t.nwaves, fm, q, s);
+    case 8: return analys_nfk<NF, 8, P>(T, nfield, t.nwaves, fm, q, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -740,6 +819,7 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
     return t;
   }
+  if (legendre == TSPHERE_LEG_F64REC) t.impl = 1;  // the f64 state runs on the VALU only
   if (t.impl == 0) t.impl = kDefaultImpl;
   if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;  // beyond the matrix-core kernels' reach
   if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
@@ -770,9 +850,9 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
       default: return synth_mf_nf<4>(T, nfield, t, q, fm, s);
     }
   switch (t.nf) {
-    case 1: return synth_nf<1>(T, nfield, t, q, fm, s);
-    case 2: return synth_nf<2>(T, nfield, t, q, fm, s);
-    default: return synth_nf<4>(T, nfield, t, q, fm, s);
+    case 1: return synth_nf<1, RecF>(T, nfield, t, q, fm, s);
+    case 2: return synth_nf<2, RecF>(T, nfield, t, q, fm, s);
+    default: return synth_nf<4, RecF>(T, nfield, t, q, fm, s);
   }
 }
 
@@ -787,9 +867,35 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
       default: return analys_mf_nf<4>(T, nfield, t, fm, q, s);
     }
   switch (t.nf) {
-    case 1: return analys_nf<1>(T, nfield, t, fm, q, s);
-    case 2: return analys_nf<2>(T, nfield, t, fm, q, s);
-    default: return analys_nf<4>(T, nfield, t, fm, q, s);
+    case 1: return analys_nf<1, RecF>(T, nfield, t, fm, q, s);
+    case 2: return analys_nf<2, RecF>(T, nfield, t, fm, q, s);
+    default: return analys_nf<4, RecF>(T, nfield, t, fm, q, s);
+  }
+}
+
+hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields& q, float2* fm,
+                                 LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  tile.impl = 1;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return synth_nf<1, RecD>(T, nfield, t, q, fm, s);
+    case 2: return synth_nf<2, RecD>(T, nfield, t, q, fm, s);
+    default: return synth_nf<4, RecD>(T, nfield, t, q, fm, s);
+  }
+}
+
+hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2* fm,
+                                  const Fields& q, LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  tile.impl = 1;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return analys_nf<1, RecD>(T, nfield, t, fm, q, s);
+    case 2: return analys_nf<2, RecD>(T, nfield, t, fm, q, s);
+    default: return analys_nf<4, RecD>(T, nfield, t, fm, q, s);
   }
 }
 

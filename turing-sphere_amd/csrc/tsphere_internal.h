@@ -157,6 +157,11 @@ hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields&
                                  LegTile tile, hipStream_t s);
 hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s);
+// The FP64-recurrence form: the VALU kernels with an f64 recurrence.
+hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields& q, float2* fm,
+                                 LegTile tile, hipStream_t s);
+hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2* fm,
+                                  const Fields& q, LegTile tile, hipStream_t s);
 // The shape a launch of `nfield` fields would use (tile fields resolved).
 LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile tile);
 // Scaling hooks of the Fourier stage (identity by default): mscale multiplies

@@ -364,7 +364,8 @@ static int validate(const tsphere_sht_config* c) {
     return fail(TSPHERE_ERR_UNSUPPORTED, "nlat=%d exceeds this build's limit of 4096", c->nlat);
   if (c->fourier < TSPHERE_FOURIER_AUTO || c->fourier > TSPHERE_FOURIER_DFT)
     return fail(TSPHERE_ERR_INVALID, "unknown fourier mode %d", c->fourier);
-  if (c->legendre != TSPHERE_LEG_FP32 && c->legendre != TSPHERE_LEG_BF16X3)
+  if (c->legendre != TSPHERE_LEG_FP32 && c->legendre != TSPHERE_LEG_BF16X3 &&
+      c->legendre != TSPHERE_LEG_F64REC)
     return fail(TSPHERE_ERR_INVALID, "unknown legendre mode %d", c->legendre);
   if (c->max_fields < 1) return fail(TSPHERE_ERR_INVALID, "max_fields must be >= 1");
   return TSPHERE_OK;
@@ -460,7 +461,7 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   if (e == hipSuccess) e = upload(&p->d_ones, ones.data(), (size_t)cfg->nlat);
   if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
   size_t bytes64 = 0;
-  if (cfg->legendre == TSPHERE_LEG_BF16X3) {
+  if (cfg->legendre == TSPHERE_LEG_BF16X3 || cfg->legendre == TSPHERE_LEG_F64REC) {
     if (e == hipSuccess)
       e = upload(&p->d_ab64, reinterpret_cast<const double2*>(t.ab64.data()),
                  (size_t)(p->nlm + kAbPad));
@@ -554,6 +555,8 @@ static Fields listed(const void* const* v, int n) {
 }
 
 static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hipStream_t s) {
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+    return launch_leg_synth_f64r(p->tables64(), nfield, q, fm, p->tile[0], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->tile[0], s);
   return launch_leg_synth(p->tables(), nfield, q, fm, p->tile[0], s);
@@ -561,6 +564,8 @@ static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hi
 
 static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, const Fields& q,
                              hipStream_t s, bool unit_w = false) {
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+    return launch_leg_analys_f64r(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_analys_mfma(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
   return launch_leg_analys(p->tables(unit_w), nfield, fm, q, p->tile[1], s);
@@ -824,7 +829,8 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
       if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0});
     return c;
   }
-  for (int impl : {1, 2})
+  const int nimpl = p->cfg.legendre == TSPHERE_LEG_F64REC ? 1 : 2;  // f64 state: VALU only
+  for (int impl = 1; impl <= nimpl; impl++)
     for (int nf : {1, 2, 4}) {
       if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
       for (int nw = 1; nw <= 4; nw++) {
@@ -866,6 +872,9 @@ int tsphere_plan_tune(tsphere_plan p, int nfield, int reps, tsphere_stream strea
         if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
           return stage == 0 ? launch_leg_synth_mfma(p->tables64(), nfield, F, p->d_fm, t, s)
                             : launch_leg_analys_mfma(p->tables64(), nfield, p->d_fm, F, t, s);
+        if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+          return stage == 0 ? launch_leg_synth_f64r(p->tables64(), nfield, F, p->d_fm, t, s)
+                            : launch_leg_analys_f64r(p->tables64(), nfield, p->d_fm, F, t, s);
         return stage == 0 ? launch_leg_synth(p->tables(), nfield, F, p->d_fm, t, s)
                           : launch_leg_analys(p->tables(), nfield, p->d_fm, F, t, s);
       };

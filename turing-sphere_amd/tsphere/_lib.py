@@ -21,7 +21,7 @@ ERR_UNSUPPORTED = 4
 ERR_NODEVICE = 5
 
 FOURIER_AUTO, FOURIER_FFT, FOURIER_DFT = 0, 1, 2
-LEG_FP32, LEG_BF16X3 = 0, 1
+LEG_FP32, LEG_BF16X3, LEG_F64REC = 0, 1, 2
 
 
 class TsphereError(RuntimeError):

@@ -19,7 +19,7 @@ from .device import DeviceBuffer, Stream
 from .layout import ShtConfig, nlm_calc, validate_config
 
 FOURIER_MODES = {"auto": _lib.FOURIER_AUTO, "fft": _lib.FOURIER_FFT, "dft": _lib.FOURIER_DFT}
-LEGENDRE_MODES = {"fp32": _lib.LEG_FP32, "bf16x3": _lib.LEG_BF16X3}
+LEGENDRE_MODES = {"fp32": _lib.LEG_FP32, "bf16x3": _lib.LEG_BF16X3, "f64rec": _lib.LEG_F64REC}
 
 
 @dataclass(frozen=True)
@@ -72,7 +72,7 @@ class ShtPlan:
         self.cfg = cfg
         self.nlm = int(info.nlm)
         self.fourier_mode = "fft" if info.fourier == _lib.FOURIER_FFT else "dft"
-        self.legendre = "bf16x3" if info.legendre == _lib.LEG_BF16X3 else "fp32"
+        self.legendre = {v: k for k, v in LEGENDRE_MODES.items()}[info.legendre]
         self.leg_lat = int(info.leg_lat)
         self.max_fields = int(info.max_fields)
         self.scratch_bytes = int(info.scratch_bytes)
