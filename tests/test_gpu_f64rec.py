@@ -68,3 +68,41 @@ def test_f64rec_model_step(gpu):
     assert eu < 1e-5 and ev < 1e-5
     gm.destroy()
     sht.destroy()
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi,nf", [(63, 64, 128, 5), (255, 256, 512, 8),
+                                               (1023, 1024, 2048, 2)])
+def test_f64rec_on_matrix_cores(gpu, lmax, nlat, nphi, nf):
+    """The FP64-recurrence form with its sums on the FP32 matrix cores
+    (tile impl 2): same tolerance against FP64, and within 1e-6 of the VALU
+    sums."""
+    import tsphere
+
+    cfg = O.Cfg(lmax, lmax, nlat, nphi)
+    emit(f"f64rec matrix cores lmax{lmax} nf {nf}")
+    plan = tsphere.ShtPlan.create(tsphere.ShtConfig(lmax, lmax, nlat, nphi), legendre="f64rec",
+                                  max_fields=nf)
+    qs = np.concatenate([_random_field_spectrum(cfg, 60 + f) for f in range(nf)])
+    q = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
+    sp = tsphere.DeviceBuffer(nf * cfg.npts)
+    back = tsphere.DeviceBuffer(nf * 2 * cfg.nlm)
+    q.upload(qs, plan.stream)
+    res = []
+    for impl in (1, 2):
+        for st in ("synth", "analys"):
+            plan.set_tile(st, impl=impl)
+        plan.encode_synth_into(plan.stream, plan.create_synth_binding(q, sp, nf, 2 * cfg.nlm,
+                                                                      cfg.npts))
+        plan.encode_analys_into(plan.stream, plan.create_analys_binding(sp, back, nf, cfg.npts,
+                                                                        2 * cfg.nlm))
+        res.append((sp.download(plan.stream).reshape(nf, -1),
+                    back.download(plan.stream).reshape(nf, -1)))
+    ref = O.ShtReference(cfg)
+    for f in (0, nf - 1):
+        qf = qs[f * 2 * cfg.nlm:(f + 1) * 2 * cfg.nlm]
+        e_s = O.rel_l2(res[1][0][f], ref.synth_fast(qf.astype(np.float64)))
+        e_a = O.rel_l2(res[1][1][f], ref.analys_fast(res[1][0][f].astype(np.float64)))
+        d = max(O.rel_l2(res[1][0][f], res[0][0][f]), O.rel_l2(res[1][1][f], res[0][1][f]))
+        emit(f"  field {f}: synth {e_s:.2e} analys {e_a:.2e} vs f64; vs VALU {d:.1e}")
+        assert e_s < TOL and e_a < TOL and d < 1e-6
+    plan.destroy()
