@@ -280,6 +280,22 @@ def adjoint_bench(steps: int = 200, every: int = 20):
     return out
 
 
+def default_run():
+    """The reference's default benchmark run (runSpec.ts defaults: preset
+    schnak-spots, lmax 63 on its dealiased 128 x 256 grid, seed 1, 100 warmup
+    + 2000 steps in batches of 16 with one sync each, then 200 one-step
+    latencies, then the state digest after exactly 2000 steps from the seed),
+    through the session layer -- the number the reference's own benchmark
+    prints for the same command."""
+    from tsphere.session import parse_args, run
+
+    rec = run(parse_args([]), batch=16, latency_steps=200)
+    return {"command": rec["command"], "steps_per_s": rec["throughput"]["stepsPerSec"],
+            "ms_per_step": rec["throughput"]["msPerStep"],
+            "latency_median_ms": rec["latency"]["medianMs"], "digest": rec["digest"],
+            "kernels": rec["compiled"]["kernels"], "ops_per_step": rec["compiled"]["opsPerStep"]}
+
+
 def mulberry_uniform(seed: int, n: int) -> np.ndarray:
     """n mulberry32 deviates in [0, 1) (src/mgpu/noise.ts:15-24), vectorised."""
     k = np.arange(1, n + 1, dtype=np.uint64)
@@ -336,6 +352,8 @@ def main() -> int:
         for leg in ("fp32", "f64rec", "bf16x3"):
             extras[f"sht_64fields_l511_{leg}"] = batched_sht(511, 64, leg)
             log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
+        extras["reference_default_run"] = default_run()
+        log(f"[bench] default run: {extras['reference_default_run']}")
         extras["ensemble_allencahn_256runs"] = ensemble_bench()
         log(f"[bench] ensemble: {extras['ensemble_allencahn_256runs']}")
         extras["adjoint_schnakenberg_l255"] = adjoint_bench()

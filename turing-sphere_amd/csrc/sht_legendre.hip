@@ -56,13 +56,6 @@ __device__ __forceinline__ float act_fma(float a, float y, float2 ct, float c) {
   return fmaf(u, ct.x, fmaf(u, ct.y, c));
 }
 
-// (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}):  ytilde_l = a_l cos(theta) ytilde_{l-1} + b_l ytilde_{l-2}
-__device__ __forceinline__ void advance(float2 a0, float2 a1, float2 ct, float& y0, float& y1) {
-  const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
-  y1 = act_fma(a1.x, t0, ct, a1.y * y1);
-  y0 = t0;
-}
-
 __device__ __forceinline__ void cfma(float2& acc, float y, float2 q) {
   acc.x = fmaf(y, q.x, acc.x);
   acc.y = fmaf(y, q.y, acc.y);
@@ -74,33 +67,13 @@ __device__ __forceinline__ int wave_min(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-struct Chain {
-  float2 ct;
-  float y0, y1, s0, s1;
-  int ls;  // seed position; INT_MAX = never significant up to lmax
-};
-
-__device__ __forceinline__ void load_chain(const LegTables& T, int m, int j, Chain& c) {
-  if (j < T.npair) {
-    c.ct = T.ct[j];
-    const SeedL sd = T.seed[(long long)m * T.npair + j];
-    c.s0 = sd.y0;
-    c.s1 = sd.y1;
-    c.ls = sd.l;
-  } else {
-    c.ct = make_float2(0.f, 0.f);
-    c.s0 = c.s1 = 0.f;
-    c.ls = 0x7fffffff;
-  }
-  c.y0 = c.y1 = 0.f;
-}
-
-// Recurrence precision of the VALU kernels.  RecF: f32 state, cos(theta) as an
+// Recurrence precision of the Legendre kernels.  RecF: f32 state, cos(theta) as an
 // f32 hi + lo pair (4 VALU ops per l).  RecD: f64 state from the f64 tables
 // and seeds (3 f64 ops per l; on gfx950 an f64 FMA issues at ~88% of the f32
 // rate), rounded to f32 once per value before the FP32 sums: the FP32 form's
 // error at high degree comes from rounding the recurrence state, which this
 // removes, at about the FP32 form's cost.
+// (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}): ytilde_l = a_l cos(theta) ytilde_{l-1} + b_l ytilde_{l-2}
 struct RecF {
   using Tables = LegTables;
   using ab_t = float2;
@@ -159,26 +132,6 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
   const long long base = lm_base(T.lmax, m);
   for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
     sab[i] = i < n ? T.ab[base + i] : P::zero_ab();
-    if (with_q) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const int fld = min(f0 + f, nfield - 1);
-        sq[i * NF + f] =
-            i < n ? reinterpret_cast<const float2*>(Fq.at(fld))[base + i] : make_float2(0.f, 0.f);
-      }
-    }
-  }
-}
-
-// Stage a row's (a, b) and the fields' Q into LDS, zero-padded.
-template <int NF>
-__device__ __forceinline__ void stage_row(const LegTables& T, int m, int f0, int nfield,
-                                          const Fields& Fq, bool with_q, float2* sab,
-                                          float2* sq, int pad) {
-  const int n = T.lmax - m + 1;
-  const long long base = lm_base(T.lmax, m);
-  for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
-    sab[i] = i < n ? T.ab[base + i] : make_float2(0.f, 0.f);
     if (with_q) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -506,8 +459,8 @@ __device__ __forceinline__ f4v mf4(float a, float b, f4v c) {
 // Synthesis: D[pair][col] = sum_l P[pair][l] B[l][col], B = Q masked by parity.
 // A row tile is 16 pairs; K-block kb of a 16-l tile holds l = lt + 4 (L/16) + kb
 // for lane L, so each lane reads its 4 K values as one 16-byte LDS load.
-template <int NF, int KC>
-__global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, Fields Fq,
+template <int NF, int KC, class P>
+__global__ __launch_bounds__(256) void k_leg_synth_mf(typename P::Tables T, int nfield, Fields Fq,
                                                       float2* __restrict__ fm) {
   constexpr int NCOL = 4 * NF;
   constexpr int QW = 64 * KC;        // pairs per wave
@@ -518,8 +471,8 @@ __global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, F
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int r16 = lane & 15, kg = lane >> 4;
   const int nrow = (L + 1 + MF_PAD + 1) & ~1;  // even: keeps the tile 16-byte aligned
-  float2* sab = reinterpret_cast<float2*>(smem);
-  float2* sq = sab + nrow;
+  auto* sab = reinterpret_cast<typename P::ab_t*>(smem);
+  float2* sq = reinterpret_cast<float2*>(sab + nrow);
   float* tl = reinterpret_cast<float*>(sq + nrow * NF) + wv * (QW * PROW);
   const int f0 = blockIdx.y * NF;
   const int wp0 = wv * QW;  // this wave's first pair
@@ -531,14 +484,14 @@ __global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, F
     const int m = r == 0 ? mp : M - mp;
     if (r == 1 && m <= mp) break;
     __syncthreads();
-    stage_row<NF>(T, m, f0, nfield, Fq, true, sab, sq, MF_PAD);
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, MF_PAD);
     __syncthreads();
 
-    Chain c[KC];
+    ChainT<P> c[KC];
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
-      load_chain(T, m, wp0 + lane + 64 * k, c[k]);
+      load_chain_t<P>(T, m, wp0 + lane + 64 * k, c[k]);
       lmin = min(lmin, c[k].ls);
     }
     lmin = wave_min(lmin);
@@ -561,12 +514,12 @@ __global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, F
             }
         }
         const int i = lt + t - m;
-        const float2 a0 = sab[i + 2], a1 = sab[i + 3];
+        const auto a0 = sab[i + 2], a1 = sab[i + 3];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-          v[k][t] = c[k].y0;
-          v[k][t + 1] = c[k].y1;
-          advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+          v[k][t] = (float)c[k].y0;
+          v[k][t + 1] = (float)c[k].y1;
+          P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
         }
       }
 #pragma unroll
@@ -618,8 +571,8 @@ __global__ __launch_bounds__(256) void k_leg_synth_mf(LegTables T, int nfield, F
 // (L / 16) * NKB + kb for lane L, so a lane's A values for four K-blocks are one
 // 16-byte load; W stays in registers for the whole row.  The four waves' partial
 // sums of a tile meet in LDS.
-template <int NF, int KC>
-__global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
+template <int NF, int KC, class P>
+__global__ __launch_bounds__(256) void k_leg_analys_mf(typename P::Tables T, int nfield,
                                                        const float2* __restrict__ fm, Fields Fq) {
   constexpr int NCOL = 4 * NF;
   constexpr int QW = 64 * KC;   // pairs per wave
@@ -631,7 +584,7 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nthr = blockDim.x, nwaves = nthr >> 6;
   const int r16 = lane & 15, kg = lane >> 4;
-  float2* sab = reinterpret_cast<float2*>(smem);
+  auto* sab = reinterpret_cast<typename P::ab_t*>(smem);
   float* tiles = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
   float* tl = tiles + wv * (2 * MF_TL * TROW);  // two tiles per wave
   float* red = tiles + 4 * 2 * MF_TL * TROW;     // [4][256]
@@ -645,13 +598,13 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
     if (r == 1 && m <= mp) break;
     const long long base = lm_base(L, m);
     __syncthreads();
-    stage_row<NF>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
 
-    Chain c[KC];
+    ChainT<P> c[KC];
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
-      load_chain(T, m, wp0 + lane + 64 * k, c[k]);
+      load_chain_t<P>(T, m, wp0 + lane + 64 * k, c[k]);
       lmin = min(lmin, c[k].ls);
     }
     // B operands for the whole row: pair wp0 + kg NKB + kb, this lane's column
@@ -706,12 +659,12 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
           }
       }
       const int i = lt + t - m;
-      const float2 a0 = sab[i + 2], a1 = sab[i + 3];
+      const auto a0 = sab[i + 2], a1 = sab[i + 3];
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
-        dst[t * TROW + lane + 64 * k] = c[k].y0;
-        dst[(t + 1) * TROW + lane + 64 * k] = c[k].y1;
-        advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+        dst[t * TROW + lane + 64 * k] = (float)c[k].y0;
+        dst[(t + 1) * TROW + lane + 64 * k] = (float)c[k].y1;
+        P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
       }
     };
     constexpr int G = NKB / 4;  // groups of four products per tile
@@ -760,45 +713,45 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(LegTables T, int nfield,
   }
 }
 
-template <int NF, int KC>
-hipError_t synth_mf(const LegTables& T, int nfield, int nwaves, const Fields& q, float2* fm,
+template <int NF, int KC, class P>
+hipError_t synth_mf(const typename P::Tables& T, int nfield, int nwaves, const Fields& q, float2* fm,
                     hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  const size_t lds = sizeof(float2) * ((T.lmax + 1 + MF_PAD + 1) & ~1) * (1 + NF) +
+  const size_t lds = (sizeof(typename P::ab_t) + sizeof(float2) * NF) * ((T.lmax + 1 + MF_PAD + 1) & ~1) +
                      sizeof(float) * (size_t)nwaves * 64 * KC * (MF_TL + 4);
-  hipLaunchKernelGGL((k_leg_synth_mf<NF, KC>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm);
+  hipLaunchKernelGGL((k_leg_synth_mf<NF, KC, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm);
   return hipGetLastError();
 }
 
-template <int NF, int KC>
-hipError_t analys_mf(const LegTables& T, int nfield, int nwaves, const float2* fm,
+template <int NF, int KC, class P>
+hipError_t analys_mf(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
                      const Fields& q, hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  const size_t lds = sizeof(float2) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
                      sizeof(float) * (4 * 2 * MF_TL * (size_t)(64 * KC + 4) + 4 * 256);
-  hipLaunchKernelGGL((k_leg_analys_mf<NF, KC>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
+  hipLaunchKernelGGL((k_leg_analys_mf<NF, KC, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q);
   return hipGetLastError();
 }
 
-template <int NF>
-hipError_t synth_mf_nf(const LegTables& T, int nfield, const LegTile& t, const Fields& q,
+template <int NF, class P>
+hipError_t synth_mf_nf(const typename P::Tables& T, int nfield, const LegTile& t, const Fields& q,
                        float2* fm, hipStream_t s) {
   switch (t.k) {
-    case 1: return synth_mf<NF, 1>(T, nfield, t.nwaves, q, fm, s);
-    case 2: return synth_mf<NF, 2>(T, nfield, t.nwaves, q, fm, s);
-    case 4: return synth_mf<NF, 4>(T, nfield, t.nwaves, q, fm, s);
+    case 1: return synth_mf<NF, 1, P>(T, nfield, t.nwaves, q, fm, s);
+    case 2: return synth_mf<NF, 2, P>(T, nfield, t.nwaves, q, fm, s);
+    case 4: return synth_mf<NF, 4, P>(T, nfield, t.nwaves, q, fm, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int NF>
-hipError_t analys_mf_nf(const LegTables& T, int nfield, const LegTile& t, const float2* fm,
+template <int NF, class P>
+hipError_t analys_mf_nf(const typename P::Tables& T, int nfield, const LegTile& t, const float2* fm,
                         const Fields& q, hipStream_t s) {
   switch (t.k) {
-    case 1: return analys_mf<NF, 1>(T, nfield, t.nwaves, fm, q, s);
-    case 2: return analys_mf<NF, 2>(T, nfield, t.nwaves, fm, q, s);
-    case 4: return analys_mf<NF, 4>(T, nfield, t.nwaves, fm, q, s);
+    case 1: return analys_mf<NF, 1, P>(T, nfield, t.nwaves, fm, q, s);
+    case 2: return analys_mf<NF, 2, P>(T, nfield, t.nwaves, fm, q, s);
+    case 4: return analys_mf<NF, 4, P>(T, nfield, t.nwaves, fm, q, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -819,7 +772,6 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
     return t;
   }
-  if (legendre == TSPHERE_LEG_F64REC) t.impl = 1;  // the f64 state runs on the VALU only
   if (t.impl == 0) t.impl = kDefaultImpl;
   if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;  // beyond the matrix-core kernels' reach
   if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
@@ -845,9 +797,9 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
   if (t.impl == 2) switch (t.nf) {
-      case 1: return synth_mf_nf<1>(T, nfield, t, q, fm, s);
-      case 2: return synth_mf_nf<2>(T, nfield, t, q, fm, s);
-      default: return synth_mf_nf<4>(T, nfield, t, q, fm, s);
+      case 1: return synth_mf_nf<1, RecF>(T, nfield, t, q, fm, s);
+      case 2: return synth_mf_nf<2, RecF>(T, nfield, t, q, fm, s);
+      default: return synth_mf_nf<4, RecF>(T, nfield, t, q, fm, s);
     }
   switch (t.nf) {
     case 1: return synth_nf<1, RecF>(T, nfield, t, q, fm, s);
@@ -862,9 +814,9 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
   if (t.impl == 2) switch (t.nf) {
-      case 1: return analys_mf_nf<1>(T, nfield, t, fm, q, s);
-      case 2: return analys_mf_nf<2>(T, nfield, t, fm, q, s);
-      default: return analys_mf_nf<4>(T, nfield, t, fm, q, s);
+      case 1: return analys_mf_nf<1, RecF>(T, nfield, t, fm, q, s);
+      case 2: return analys_mf_nf<2, RecF>(T, nfield, t, fm, q, s);
+      default: return analys_mf_nf<4, RecF>(T, nfield, t, fm, q, s);
     }
   switch (t.nf) {
     case 1: return analys_nf<1, RecF>(T, nfield, t, fm, q, s);
@@ -876,9 +828,13 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
 hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields& q, float2* fm,
                                  LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  tile.impl = 1;
-  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 0, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 2) switch (t.nf) {
+      case 1: return synth_mf_nf<1, RecD>(T, nfield, t, q, fm, s);
+      case 2: return synth_mf_nf<2, RecD>(T, nfield, t, q, fm, s);
+      default: return synth_mf_nf<4, RecD>(T, nfield, t, q, fm, s);
+    }
   switch (t.nf) {
     case 1: return synth_nf<1, RecD>(T, nfield, t, q, fm, s);
     case 2: return synth_nf<2, RecD>(T, nfield, t, q, fm, s);
@@ -889,9 +845,13 @@ hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields&
 hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  tile.impl = 1;
-  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 2) switch (t.nf) {
+      case 1: return analys_mf_nf<1, RecD>(T, nfield, t, fm, q, s);
+      case 2: return analys_mf_nf<2, RecD>(T, nfield, t, fm, q, s);
+      default: return analys_mf_nf<4, RecD>(T, nfield, t, fm, q, s);
+    }
   switch (t.nf) {
     case 1: return analys_nf<1, RecD>(T, nfield, t, fm, q, s);
     case 2: return analys_nf<2, RecD>(T, nfield, t, fm, q, s);

@@ -829,8 +829,7 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
       if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0});
     return c;
   }
-  const int nimpl = p->cfg.legendre == TSPHERE_LEG_F64REC ? 1 : 2;  // f64 state: VALU only
-  for (int impl = 1; impl <= nimpl; impl++)
+  for (int impl = 1; impl <= 2; impl++)
     for (int nf : {1, 2, 4}) {
       if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
       for (int nw = 1; nw <= 4; nw++) {

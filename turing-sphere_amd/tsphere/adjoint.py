@@ -23,12 +23,13 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import re
 
 import numpy as np
 
 from . import _lib
 from .device import DeviceBuffer, Stream
-from .hipgen import CALL_FNS, UnsupportedOnGpu, _Ctx, _emit, f32_lit, module_source
+from .hipgen import UnsupportedOnGpu, _Ctx, _emit, f32_lit
 from .ir import is_numeric, is_tensor
 from .layout import ShtConfig
 from .mfront import CompiledModel, model_bindings
@@ -201,7 +202,7 @@ def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: in
     fin = "\n".join(
         f"    partials[(long long)blockIdx.x * {np_} + {j}] += "
         f"(red[{j}][0] + red[{j}][1]) + (red[{j}][2] + red[{j}][3]);" for j in live)
-    used = {int(m) for m in __import__("re").findall(r"\bpow_i(\d+)\(", upd + acc + pro)}
+    used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", upd + acc + pro)}
     helpers = "\n".join(
         f"__device__ __forceinline__ float pow_i{k}(float x) {{ return {'x' + ' * x' * (k - 1)}; }}"
         for k in sorted(used))

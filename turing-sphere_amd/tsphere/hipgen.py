@@ -30,8 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .ir import (Assign, Binary, Call, IRExpr, NumLit, NumericType, Unary, Var, is_multi_element,
-                 is_numeric, is_tensor)
+from .ir import Assign, IRExpr, NumericType, is_multi_element, is_numeric, is_tensor
 
 
 class UnsupportedOnGpu(Exception):

@@ -25,7 +25,7 @@ Differences from the WebGPU planner, all consequences of the hardware:
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
