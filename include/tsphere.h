@@ -227,10 +227,12 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  *               (v_mfma_f32_16x16x4_f32; the recurrence stays on the VALU);
  *               nf fields per workgroup (1, 2, 4), nwaves waves (1..4),
  *               k latitude pairs per lane (1, 2, 4, 8; matrix cores 1, 2, 4),
- *               64*nwaves*k >= pairs.
+ *               64*nwaves*k >= pairs; rows = m-rows per workgroup on the VALU
+ *               (2: the pair (m, mmax - m), balanced; 1: one row, twice the
+ *               workgroups).
  *   MFMA form:  ctc 16-column tiles per workgroup (1, 2; 4 for synthesis). */
 typedef struct {
-  int nf, nwaves, k, ctc, impl;
+  int nf, nwaves, k, ctc, impl, rows;
 } tsphere_leg_tile;
 
 int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile *tile); /* NULL = built-in */

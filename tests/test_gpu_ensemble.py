@@ -175,12 +175,13 @@ def test_tile_shapes_and_tuning(gpu, tmp_path, monkeypatch):
 
     s0, b0 = run()
     for nfb in (1, 2, 4):
-        for nw, k in ((1, 1), (2, 1), (1, 4), (4, 1)):
+        for nw, k, rows in ((1, 1, 2), (2, 1, 2), (1, 4, 2), (4, 1, 2), (2, 1, 1), (1, 4, 1)):
             for stage in ("synth", "analys"):
-                plan.set_tile(stage, nf=nfb, nwaves=nw, k=k)
+                plan.set_tile(stage, nf=nfb, nwaves=nw, k=k, impl=1, rows=rows)
             s1, b1 = run()
             es, eb = O.rel_l2(s1, s0), O.rel_l2(b1, b0)
-            emit(f"  tile nf={nfb} nwaves={nw} k={k}: synth {es:.1e} analys {eb:.1e}")
+            emit(f"  tile nf={nfb} nwaves={nw} k={k} rows={rows}: synth {es:.1e} "
+                 f"analys {eb:.1e}")
             assert es < 1e-6 and eb < 1e-6
     with pytest.raises(tsphere.TsphereError):
         plan.set_tile("synth", nf=3)
@@ -188,6 +189,8 @@ def test_tile_shapes_and_tuning(gpu, tmp_path, monkeypatch):
         plan.set_tile("analys", k=3)
     with pytest.raises(tsphere.TsphereError):
         plan.set_tile("synth", nwaves=5)
+    with pytest.raises(tsphere.TsphereError):
+        plan.set_tile("synth", rows=3)
     res = plan.tune(nf, reps=3)
     emit(f"  tuned: {res}")
     assert not res["cached"] and res["ms"][0] > 0

@@ -67,6 +67,19 @@ __device__ __forceinline__ int wave_min(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// The m-row this workgroup handles in its pass r (0 or 1), or -1.  Paired
+// (rows = 2): workgroup b takes m = b and M - b, whose l-loops sum to
+// lmax + 2, laid out XCD by XCD.  Single (rows = 1): workgroup b takes m = b
+// alone -- twice the workgroups, each with half the work, so more waves are
+// resident when the field count is small; dispatch order puts the long rows
+// (small m) first.
+__device__ __forceinline__ int row_m(int r, int rows, int M) {
+  if (rows == 1) return r == 0 ? (int)blockIdx.x : -1;
+  const int mp = xcd_chunk(blockIdx.x, gridDim.x);
+  const int m = r == 0 ? mp : M - mp;
+  return (r == 1 && m <= mp) ? -1 : m;
+}
+
 // Recurrence precision of the Legendre kernels.  RecF: f32 state, cos(theta) as an
 // f32 hi + lo pair (4 VALU ops per l).  RecD: f64 state from the f64 tables
 // and seeds (3 f64 ops per l; on gfx950 an f64 FMA issues at ~88% of the f32
@@ -146,7 +159,7 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
 // ------------------------------------------------------------- synthesis
 template <int NF, int K, class P>
 __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfield, Fields Fq,
-                                                   float2* __restrict__ fm) {
+                                                   float2* __restrict__ fm, int rows) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int L = T.lmax, M = T.mmax;
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
@@ -155,9 +168,8 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
   const int nthr = blockDim.x;
 
   for (int r = 0; r < 2; ++r) {
-    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
-    const int m = r == 0 ? mp : M - mp;
-    if (r == 1 && m <= mp) break;
+    const int m = row_m(r, rows, M);
+    if (m < 0) break;
     __syncthreads();  // the previous row's readers are done
     stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
     __syncthreads();
@@ -267,7 +279,8 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 // -------------------------------------------------------------- analysis
 template <int NF, int K, class P>
 __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nfield,
-                                                    const float2* __restrict__ fm, Fields Fq) {
+                                                    const float2* __restrict__ fm, Fields Fq,
+                                                    int rows) {
   constexpr int SPAN = 32 / NF;  // l-values per reduced span
   static_assert(SPAN % kTile == 0 && SPAN * NF * 2 == 64, "64 partials per lane");
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -280,9 +293,8 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
   const int nthr = blockDim.x, nwaves = nthr >> 6;
 
   for (int r = 0; r < 2; ++r) {
-    const int mp = xcd_chunk(blockIdx.x, gridDim.x);
-    const int m = r == 0 ? mp : M - mp;
-    if (r == 1 && m <= mp) break;
+    const int m = row_m(r, rows, M);
+    if (m < 0) break;
     const long long base = lm_base(L, m);
     __syncthreads();
     stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
@@ -390,20 +402,21 @@ size_t synth_lds(int lmax, int nf) {
 }
 
 template <int NF, int K, class P>
-hipError_t synth_nfk(const typename P::Tables& T, int nfield, int nwaves, const Fields& q,
-                     float2* fm, hipStream_t s) {
-  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+hipError_t synth_nfk(const typename P::Tables& T, int nfield, int nwaves, int rows,
+                     const Fields& q, float2* fm, hipStream_t s) {
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   hipLaunchKernelGGL((k_leg_synth<NF, K, P>), grid, dim3(64 * nwaves), synth_lds<P>(T.lmax, NF),
-                     s, T, nfield, q, fm);
+                     s, T, nfield, q, fm, rows);
   return hipGetLastError();
 }
 
 template <int NF, int K, class P>
-hipError_t analys_nfk(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
-                      const Fields& q, hipStream_t s) {
-  dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+hipError_t analys_nfk(const typename P::Tables& T, int nfield, int nwaves, int rows,
+                      const float2* fm, const Fields& q, hipStream_t s) {
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   hipLaunchKernelGGL((k_leg_analys<NF, K, P>), grid, dim3(64 * nwaves),
-                     sizeof(typename P::ab_t) * (T.lmax + 1 + kPadAnalys), s, T, nfield, fm, q);
+                     sizeof(typename P::ab_t) * (T.lmax + 1 + kPadAnalys), s, T, nfield, fm, q,
+                     rows);
   return hipGetLastError();
 }
 
@@ -411,10 +424,10 @@ template <int NF, class P>
 hipError_t synth_nf(const typename P::Tables& T, int nfield, const LegTile& t, const Fields& q,
                     float2* fm, hipStream_t s) {
   switch (t.k) {
-    case 1: return synth_nfk<NF, 1, P>(T, nfield, t.nwaves, q, fm, s);
-    case 2: return synth_nfk<NF, 2, P>(T, nfield, t.nwaves, q, fm, s);
-    case 4: return synth_nfk<NF, 4, P>(T, nfield, t.nwaves, q, fm, s);
-    case 8: return synth_nfk<NF, 8, P>(T, nfield, t.nwaves, q, fm, s);
+    case 1: return synth_nfk<NF, 1, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 2: return synth_nfk<NF, 2, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 4: return synth_nfk<NF, 4, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 8: return synth_nfk<NF, 8, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -423,10 +436,10 @@ template <int NF, class P>
 hipError_t analys_nf(const typename P::Tables& T, int nfield, const LegTile& t, const float2* fm,
                      const Fields& q, hipStream_t s) {
   switch (t.k) {
-    case 1: return analys_nfk<NF, 1, P>(T, nfield, t.nwaves, fm, q, s);
-    case 2: return analys_nfk<NF, 2, P>(T, nfield, t.nwaves, fm, q, s);
-    case 4: return analys_nfk<NF, 4, P>(T, nfield, t.nwaves, fm, q, s);
-    case 8: return analys_nfk<NF, 8, P>(T, nfield, t.nwaves, fm, q, s);
+    case 1: return analys_nfk<NF, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 2: return analys_nfk<NF, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 4: return analys_nfk<NF, 4, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 8: return analys_nfk<NF, 8, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -773,6 +786,10 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
     return t;
   }
   if (t.impl == 0) t.impl = kDefaultImpl;
+  // one row per workgroup when a launch has one or two fields (its workgroups
+  // are few and long); the matrix-core kernels always pair rows
+  if (t.rows == 0) t.rows = nfield <= 2 ? 1 : 2;
+  if (t.impl == 2) t.rows = 2;
   if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;  // beyond the matrix-core kernels' reach
   if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
   if (t.nwaves == 0 && t.k == 0) t.nwaves = std::min(4, (npair + 63) / 64);
@@ -785,7 +802,8 @@ namespace {
 
 bool fp32_tile_ok(const LegTile& t, int npair) {
   const bool kok = t.impl == 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
-  return (t.impl == 1 || t.impl == 2) && (t.nf == 1 || t.nf == 2 || t.nf == 4) &&
+  return (t.impl == 1 || t.impl == 2) && (t.rows == 1 || t.rows == 2) &&
+         (t.nf == 1 || t.nf == 2 || t.nf == 4) &&
          t.nwaves >= 1 && t.nwaves <= 4 && kok && 64 * t.nwaves * t.k >= npair;
 }
 

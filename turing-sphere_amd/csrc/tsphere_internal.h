@@ -61,9 +61,10 @@ struct LegTables64 {
 //  nf fields per workgroup (1, 2, 4), nwaves waves, k latitude pairs per lane
 //  (1, 2, 4, 8 on the VALU, 1, 2, 4 on the matrix cores; nwaves * 64 * k must
 //  cover the pairs).
+//  rows (VALU kernels): m-rows per workgroup, 2 = the pair (m, M - m), 1 = one.
 //  Split-BF16 form: ctc column tiles (16 columns each) per workgroup.
 struct LegTile {
-  int nf, nwaves, k, ctc, impl;
+  int nf, nwaves, k, ctc, impl, rows;
 };
 
 struct Plan {

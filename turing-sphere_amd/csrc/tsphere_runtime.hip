@@ -770,7 +770,7 @@ int tsphere_accumulate(float* dst, const float* src, long long n, tsphere_stream
 
 // ------------------------------------------------------------ tile tuning
 static LegTile to_tile(const tsphere_leg_tile& t) {
-  return LegTile{t.nf, t.nwaves, t.k, t.ctc, t.impl};
+  return LegTile{t.nf, t.nwaves, t.k, t.ctc, t.impl, t.rows};
 }
 static tsphere_leg_tile from_tile(const LegTile& t) {
   tsphere_leg_tile o{};
@@ -779,6 +779,7 @@ static tsphere_leg_tile from_tile(const LegTile& t) {
   o.k = t.k;
   o.ctc = t.ctc;
   o.impl = t.impl;
+  o.rows = t.rows;
   return o;
 }
 
@@ -786,8 +787,9 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
   if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
   const LegTile t = tile ? to_tile(*tile) : LegTile{};
-  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 2)
-    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0, 1 or 2)");
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 2 ||
+      t.rows < 0 || t.rows > 2)
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0, 1 or 2; rows 0, 1 or 2)");
   // validate against a one-field and a max-fields launch
   for (int nf : {1, p->cfg.max_fields}) {
     const LegTile r = resolve_leg_tile(p->cfg.legendre, stage, p->npair, nf, t);
@@ -826,7 +828,7 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
     const int ct = (2 * nfield + 15) / 16;
     for (int ctc : {1, 2, 4})
-      if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0});
+      if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0, 0});
     return c;
   }
   for (int impl = 1; impl <= 2; impl++)
@@ -838,7 +840,8 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
         while (k <= kmax && 64 * nw * k < p->npair) k *= 2;
         if (k > kmax) continue;
         if (nw > 1 && 64 * (nw - 1) * k >= p->npair) continue;  // a wave with no pairs
-        c.push_back(LegTile{nf, nw, k, 0, impl});
+        c.push_back(LegTile{nf, nw, k, 0, impl, 2});
+        if (impl == 1) c.push_back(LegTile{nf, nw, k, 0, impl, 1});
       }
     }
   return c;

@@ -45,7 +45,7 @@ class PlanInfoC(C.Structure):
 
 class LegTileC(C.Structure):
     _fields_ = [("nf", C.c_int), ("nwaves", C.c_int), ("k", C.c_int), ("ctc", C.c_int),
-                ("impl", C.c_int)]
+                ("impl", C.c_int), ("rows", C.c_int)]
 
 
 _vp = C.c_void_p

@@ -112,16 +112,17 @@ class ShtPlan:
     STAGES = {"synth": 0, "analys": 1}
 
     def set_tile(self, stage: str, nf: int = 0, nwaves: int = 0, k: int = 0,
-                 ctc: int = 0, impl: int = 0) -> None:
+                 ctc: int = 0, impl: int = 0, rows: int = 0) -> None:
         """Fix the Legendre workgroup shape of one stage (0 = built-in); for
         the FP32 form impl 1 = VALU contraction, 2 = FP32 matrix cores."""
-        t = _lib.LegTileC(nf, nwaves, k, ctc, impl)
+        t = _lib.LegTileC(nf, nwaves, k, ctc, impl, rows)
         _lib.call("tsphere_plan_set_tile", self._h, self.STAGES[stage], C.byref(t))
 
     def get_tile(self, stage: str, nfield: int) -> dict:
         t = _lib.LegTileC()
         _lib.call("tsphere_plan_get_tile", self._h, self.STAGES[stage], nfield, C.byref(t))
-        return {"nf": t.nf, "nwaves": t.nwaves, "k": t.k, "ctc": t.ctc, "impl": t.impl}
+        return {"nf": t.nf, "nwaves": t.nwaves, "k": t.k, "ctc": t.ctc, "impl": t.impl,
+                "rows": t.rows}
 
     def tune(self, nfield: int, reps: int = 5, cache: bool = True) -> dict:
         """Autotune the Legendre tiles for `nfield` fields (timed on the

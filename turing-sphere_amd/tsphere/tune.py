@@ -25,7 +25,7 @@ def cache_path() -> str:
 
 def _key(plan, nfield: int) -> str:
     c = plan.cfg
-    return f"gfx950/v2/{plan.legendre}/L{c.lmax}/M{c.mmax}/nlat{c.nlat}/nphi{c.nphi}/nf{nfield}"
+    return f"gfx950/v3/{plan.legendre}/L{c.lmax}/M{c.mmax}/nlat{c.nlat}/nphi{c.nphi}/nf{nfield}"
 
 
 def _load() -> dict:
@@ -61,7 +61,7 @@ def tune_plan(plan, nfield: int, reps: int = 5, cache: bool = True) -> dict:
                 for s in _STAGES:
                     t = hit[s]
                     plan.set_tile(s, t.get("nf", 0), t.get("nwaves", 0), t.get("k", 0),
-                                  t.get("ctc", 0), t.get("impl", 0))
+                                  t.get("ctc", 0), t.get("impl", 0), t.get("rows", 0))
                 return {**hit, "cached": True}
             except (_lib.TsphereError, TypeError):
                 pass  # stale entry: re-time
@@ -69,7 +69,7 @@ def tune_plan(plan, nfield: int, reps: int = 5, cache: bool = True) -> dict:
     ms = (C.c_float * 2)()
     _lib.call("tsphere_plan_tune", plan.handle, nfield, reps, plan.stream.handle, best, ms)
     out = {s: {"nf": best[i].nf, "nwaves": best[i].nwaves, "k": best[i].k, "ctc": best[i].ctc,
-               "impl": best[i].impl} for i, s in enumerate(_STAGES)}
+               "impl": best[i].impl, "rows": best[i].rows} for i, s in enumerate(_STAGES)}
     out["ms"] = [float(ms[0]), float(ms[1])]
     if cache:
         d = _load()
