@@ -307,6 +307,24 @@ def mulberry_uniform(seed: int, n: int) -> np.ndarray:
         return (t ^ (t >> np.uint32(14))).astype(np.float64) / 4294967296.0
 
 
+def hbm_calibration() -> int:
+    """Three device-to-device copies of 256 MiB: each moves 262144 KiB in and
+    262144 KiB out of HBM, which calibrates the FETCH_SIZE / WRITE_SIZE
+    counters that the dominant kernel's traffic figure is read from."""
+    import tsphere
+
+    n = 64 * 1024 * 1024  # floats: 256 MiB
+    a, b = tsphere.DeviceBuffer(n), tsphere.DeviceBuffer(n)
+    s = tsphere.Stream()
+    a.fill(1.0, s)
+    s.sync()
+    for _ in range(3):
+        b.copy_from(a, s)
+    s.sync()
+    log("[bench] hbm calibration: 3 copies of 256 MiB done")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,7 +336,12 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip errors and batched extras")
+    ap.add_argument("--hbm-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (the known-size "
+                         "workload that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
     args = ap.parse_args()
+    if args.hbm_calibration:
+        return hbm_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
               f"step on a single MI355X -- see DESIGN.md for how it would shard.",

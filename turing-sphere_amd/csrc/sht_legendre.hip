@@ -67,6 +67,32 @@ __device__ __forceinline__ int wave_min(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Latitude pair of this lane's chain k (of K).  A chain is a chunk of 64
+// consecutive pairs, so the lanes of one chain span a narrow band of
+// latitudes whose significance points are close and the chain can start at
+// the smallest of them (near the poles at high m that is far above m).  Chunks
+// are dealt to the waves boustrophedon -- wave w takes chunks w, 2 nw - 1 - w,
+// 2 nw + w, ... -- so every wave gets polar and equatorial chunks alike, and
+// chunk numbers (and so latitudes, toward the equator) increase with k.
+__device__ __forceinline__ int pair_of(int k) {
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int chunk = k * nw + ((k & 1) ? nw - 1 - w : w);
+  return chunk * 64 + (threadIdx.x & 63);
+}
+
+// First chain (of K) that is live at l: chains with no lane significant
+// before l are still exactly 0, so chains [kf, K) carry every contribution.
+// (Significance moves toward the poles as l grows, so the live chains are
+// the last ones; if they are not, kf only keeps some zero chains running.)
+template <int K>
+__device__ __forceinline__ int first_live(const int (&lk)[K], int l) {
+  int kf = K - 1;
+#pragma unroll
+  for (int k = K - 1; k >= 0; --k)
+    if (lk[k] <= l) kf = k;
+  return kf;
+}
+
 // The m-row this workgroup handles in its pass r (0 or 1), or -1.  Paired
 // (rows = 2): workgroup b takes m = b and M - b, whose l-loops sum to
 // lmax + 2, laid out XCD by XCD.  Single (rows = 1): workgroup b takes m = b
@@ -157,6 +183,54 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
 }
 
 // ------------------------------------------------------------- synthesis
+// kTile l-values of chains [K0, K) from l = m + i.
+template <int K0, int NF, int K, class P>
+__device__ __forceinline__ void synth_tile(const typename P::ab_t* sab, const float2* sq, int i,
+                                           int l, ChainT<P> (&c)[K], float2 (&accE)[K][NF],
+                                           float2 (&accO)[K][NF]) {
+#pragma unroll
+  for (int k = K0; k < K; ++k)
+    if (c[k].ls == l) {
+      c[k].y0 = c[k].s0;
+      c[k].y1 = c[k].s1;
+    }
+#pragma unroll
+  for (int p = 0; p < kTile; p += 2) {
+    float2 q0[NF], q1[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      q0[f] = sq[(i + p) * NF + f];
+      q1[f] = sq[(i + p + 1) * NF + f];
+    }
+    const auto a0 = sab[i + p + 2];
+    const auto a1 = sab[i + p + 3];
+#pragma unroll
+    for (int k = K0; k < K; ++k) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        cfma(accE[k][f], (float)c[k].y0, q0[f]);
+        cfma(accO[k][f], (float)c[k].y1, q1[f]);
+      }
+      P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+    }
+  }
+}
+
+// Tiles from l on, in phases: phase K0 runs chains [K0, K) until a chain
+// below K0 becomes live (lk[k] <= l), then hands over to phase K0 - 1.  Each
+// phase is a branch-free loop; a chain run before it is live only adds zeros.
+template <int K0, int NF, int K, class P>
+__device__ __forceinline__ void synth_phases(int l, int L, int m, const int (&lk)[K],
+                                             const typename P::ab_t* sab, const float2* sq,
+                                             ChainT<P> (&c)[K], float2 (&accE)[K][NF],
+                                             float2 (&accO)[K][NF]) {
+  int lend = L + 1;
+#pragma unroll
+  for (int k = 0; k < K0; ++k) lend = min(lend, lk[k]);
+  for (; l < lend; l += kTile) synth_tile<K0, NF, K, P>(sab, sq, l - m, l, c, accE, accO);
+  if constexpr (K0 > 0) synth_phases<K0 - 1, NF, K, P>(l, L, m, lk, sab, sq, c, accE, accO);
+}
+
 template <int NF, int K, class P>
 __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfield, Fields Fq,
                                                    float2* __restrict__ fm, int rows) {
@@ -165,7 +239,6 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
   float2* sq = reinterpret_cast<float2*>(sab + (L + 1 + kPadQ));
   const int f0 = blockIdx.y * NF;
-  const int nthr = blockDim.x;
 
   for (int r = 0; r < 2; ++r) {
     const int m = row_m(r, rows, M);
@@ -175,13 +248,14 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
     __syncthreads();
 
     ChainT<P> c[K];
+    int lk[K];  // first l where some lane of chain k is significant
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      load_chain_t<P>(T, m, threadIdx.x + k * nthr, c[k]);
-      lmin = min(lmin, c[k].ls);
+      load_chain_t<P>(T, m, pair_of(k), c[k]);
+      lk[k] = wave_min(c[k].ls);
+      lmin = min(lmin, lk[k]);
     }
-    lmin = wave_min(lmin);
 
     float2 accE[K][NF], accO[K][NF];
 #pragma unroll
@@ -189,40 +263,12 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
 #pragma unroll
       for (int f = 0; f < NF; ++f) accE[k][f] = accO[k][f] = make_float2(0.f, 0.f);
 
-    for (int l = lmin; l <= L; l += kTile) {
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (c[k].ls == l) {
-          c[k].y0 = c[k].s0;
-          c[k].y1 = c[k].s1;
-        }
-      const int i = l - m;
-#pragma unroll
-      for (int p = 0; p < kTile; p += 2) {
-        float2 q0[NF], q1[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          q0[f] = sq[(i + p) * NF + f];
-          q1[f] = sq[(i + p + 1) * NF + f];
-        }
-        const auto a0 = sab[i + p + 2];
-        const auto a1 = sab[i + p + 3];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            cfma(accE[k][f], (float)c[k].y0, q0[f]);
-            cfma(accO[k][f], (float)c[k].y1, q1[f]);
-          }
-          P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
-        }
-      }
-    }
+    if (lmin <= L) synth_phases<K - 1, NF, K, P>(lmin, L, m, lk, sab, sq, c, accE, accO);
 
     const int nlat = T.nlat;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int j = threadIdx.x + k * nthr;
+      const int j = pair_of(k);
       if (j >= T.npair) continue;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -277,6 +323,62 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 }
 
 // -------------------------------------------------------------- analysis
+// One span of SPAN l-values from ls for chains [K0, K): partial sums acc
+// (zeroed here), laid out (l, field, re/im).
+template <int K0, int NF, int K, class P>
+__device__ __forceinline__ void analys_span(const typename P::ab_t* sab, int ls, int m,
+                                            ChainT<P> (&c)[K], const float2 (&wp)[K][NF],
+                                            const float2 (&wm)[K][NF], float (&acc)[64]) {
+  constexpr int SPAN = 32 / NF;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) acc[i] = 0.f;
+#pragma unroll
+  for (int p = 0; p < SPAN; p += 2) {
+    const int l = ls + p;
+    if (p % kTile == 0) {
+#pragma unroll
+      for (int k = K0; k < K; ++k)
+        if (c[k].ls == l) {
+          c[k].y0 = c[k].s0;
+          c[k].y1 = c[k].s1;
+        }
+    }
+#pragma unroll
+    for (int k = K0; k < K; ++k) {
+      const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        float* e = &acc[(p * NF + f) * 2];
+        float* o = &acc[((p + 1) * NF + f) * 2];
+        e[0] = fmaf(y0, wp[k][f].x, e[0]);
+        e[1] = fmaf(y0, wp[k][f].y, e[1]);
+        o[0] = fmaf(y1, wm[k][f].x, o[0]);
+        o[1] = fmaf(y1, wm[k][f].y, o[1]);
+      }
+    }
+    const int i = l - m;
+    const auto a0 = sab[i + 2];
+    const auto a1 = sab[i + 3];
+#pragma unroll
+    for (int k = K0; k < K; ++k) P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+  }
+}
+
+template <int K0, int NF, int K, class P>
+__device__ __forceinline__ void analys_span_from(int kf, const typename P::ab_t* sab, int ls,
+                                                 int m, ChainT<P> (&c)[K],
+                                                 const float2 (&wp)[K][NF],
+                                                 const float2 (&wm)[K][NF], float (&acc)[64]) {
+  if constexpr (K0 + 1 >= K) {
+    analys_span<K0, NF, K, P>(sab, ls, m, c, wp, wm, acc);
+  } else {
+    if (kf <= K0)
+      analys_span<K0, NF, K, P>(sab, ls, m, c, wp, wm, acc);
+    else
+      analys_span_from<K0 + 1, NF, K, P>(kf, sab, ls, m, c, wp, wm, acc);
+  }
+}
+
 template <int NF, int K, class P>
 __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nfield,
                                                     const float2* __restrict__ fm, Fields Fq,
@@ -301,12 +403,14 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
 
     ChainT<P> c[K];
     float2 wp[K][NF], wm[K][NF];
+    int lk[K];  // first l where some lane of chain k is significant
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int j = tid + k * nthr;
+      const int j = pair_of(k);
       load_chain_t<P>(T, m, j, c[k]);
-      lmin = min(lmin, c[k].ls);
+      lk[k] = wave_min(c[k].ls);
+      lmin = min(lmin, lk[k]);
       const int jc = min(j, T.npair - 1);
       const int js = nlat - 1 - jc;
       const float w = j < T.npair ? T.w[jc] : 0.f;
@@ -325,7 +429,6 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
         }
       }
     }
-    lmin = wave_min(lmin);
     if (lane == 0) smin[wv] = lmin;
     __syncthreads();  // also publishes the staged row
     int lblk = smin[0];
@@ -342,41 +445,17 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
 
     int buf = 0;
     for (int ls = lblk; ls <= L; ls += SPAN) {
-      float acc[64];
-#pragma unroll
-      for (int i = 0; i < 64; ++i) acc[i] = 0.f;
-#pragma unroll
-      for (int p = 0; p < SPAN; p += 2) {
-        const int l = ls + p;
-        if (p % kTile == 0) {
-#pragma unroll
-          for (int k = 0; k < K; ++k)
-            if (c[k].ls == l) {
-              c[k].y0 = c[k].s0;
-              c[k].y1 = c[k].s1;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
-#pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            float* e = &acc[(p * NF + f) * 2];
-            float* o = &acc[((p + 1) * NF + f) * 2];
-            e[0] = fmaf(y0, wp[k][f].x, e[0]);
-            e[1] = fmaf(y0, wp[k][f].y, e[1]);
-            o[0] = fmaf(y1, wm[k][f].x, o[0]);
-            o[1] = fmaf(y1, wm[k][f].y, o[1]);
-          }
-        }
-        const int i = l - m;
-        const auto a0 = sab[i + 2];
-        const auto a1 = sab[i + 3];
-#pragma unroll
-        for (int k = 0; k < K; ++k) P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+      if (lmin >= ls + SPAN) {
+        // No lane of this wave is significant before the next span: its
+        // chains are all still 0 and no seed falls in this span, so its
+        // partial sums are exactly 0.
+        red[buf][wv][lane] = 0.f;
+      } else {
+        float acc[64];
+        analys_span_from<0, NF, K, P>(first_live<K>(lk, ls + SPAN - 1), sab, ls, m, c, wp, wm,
+                                      acc);
+        red[buf][wv][lane] = transpose_reduce64(acc);
       }
-      const float v = transpose_reduce64(acc);
-      red[buf][wv][lane] = v;
       __syncthreads();
       if (wv == 0) {
         float s = red[buf][0][lane];
