@@ -24,10 +24,12 @@
 //
 // MI355X mapping.  One workgroup per m-pair (m, mmax - m), whose l-loops
 // together have a constant length lmax + 2, so the triangle is balanced
-// without a tail.  The row's Q_lm (all batched fields) and (a, b) are staged
-// once into LDS with coalesced loads; the inner loop reads them as LDS
-// broadcasts and has no global or scalar loads.  Each lane carries K latitude
-// pairs (K independent dependency chains).
+// without a tail, or -- for launches of one or two fields -- one workgroup per
+// m-row, for twice the resident waves.  The row's Q_lm (all batched fields)
+// and (a, b) are staged once into LDS with coalesced loads; the inner loop
+// reads them as LDS broadcasts and has no global or scalar loads.  Each lane
+// carries K chains (K independent dependency chains), each a chunk of 64
+// consecutive latitude pairs that starts at its own significance point.
 //  * synthesis: a lane does 4 VALU ops of recurrence (a * cos(theta) with the
 //    f32 hi+lo node) + 2 FMAs per field per l;
 //  * analysis: each lane accumulates a span of l-values (64 partial sums:
@@ -95,12 +97,22 @@ __device__ __forceinline__ int first_live(const int (&lk)[K], int l) {
 
 // The m-row this workgroup handles in its pass r (0 or 1), or -1.  Paired
 // (rows = 2): workgroup b takes m = b and M - b, whose l-loops sum to
-// lmax + 2, laid out XCD by XCD.  Single (rows = 1): workgroup b takes m = b
-// alone -- twice the workgroups, each with half the work, so more waves are
-// resident when the field count is small; dispatch order puts the long rows
-// (small m) first.
+// lmax + 2, laid out XCD by XCD.  Single (rows = 1): each workgroup takes one
+// row -- twice the workgroups, each with half the work, so more waves are
+// resident when the field count is small.  Rows go to the XCDs in groups of
+// 16 consecutive m (one 128-byte line of the Fourier scratch holds 16 m of a
+// latitude, so a group shares its lines in one L2), the groups dealt
+// boustrophedon over the 8 XCDs so that each gets long and short rows alike;
+// long rows are dispatched first.  (Workgroups go to XCD b % 8 in turn.)
 __device__ __forceinline__ int row_m(int r, int rows, int M) {
-  if (rows == 1) return r == 0 ? (int)blockIdx.x : -1;
+  if (rows == 1) {
+    if (r != 0) return -1;
+    const int b = blockIdx.x, n = gridDim.x;
+    if (n % 256 != 0) return b;  // fewer than two groups per XCD: no balance
+    const int x = b & 7, t = b >> 3;             // XCD, position on it
+    const int gi = t >> 4, g = gi * 8 + ((gi & 1) ? 7 - x : x);
+    return g * 16 + (t & 15);
+  }
   const int mp = xcd_chunk(blockIdx.x, gridDim.x);
   const int m = r == 0 ? mp : M - mp;
   return (r == 1 && m <= mp) ? -1 : m;
