@@ -163,6 +163,10 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
         "leg_analys": time_stage(lambda: _lib.check(L.tsphere_leg_analys(
             h, 2, fm.ptr, qo.ptr, 2 * nlm, s.handle)), s, 20),
     }
+    grid_ops = [op for op in gm.step_plan.ops if op.kind == "grid"]
+    if grid_ops:
+        # the step runs the Fourier stages inside its generated grid kernel
+        kms["grid_kernel"] = time_stage(lambda: gm.step_plan.launch_grid(grid_ops[0], s), s, 20)
     log(f"[bench] {legendre} kernel ms: {kms}")
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "kernel_ms": kms, "finite": finite,

@@ -170,6 +170,22 @@ int tsphere_fourier_analys(tsphere_plan p, int nfield, const float *spat,
 int tsphere_leg_analys(tsphere_plan p, int nfield, const float *fm, float *qlm,
                        long long qlm_stride, tsphere_stream s);
 
+/* The two Legendre stages over a pointer list of spectral fields (field f at
+ * qlm[f]) and a caller-owned Fourier scratch fm laid out as above, nfield
+ * fields.  With tsphere_plan_fourier_table and tsphere_kernel_launch_grid
+ * they let a generated grid kernel take the place of the Fourier stages and
+ * the element-wise statements between them (the planner's grid fusion of the
+ * reference's encodeSynthInto -> element-wise dispatches -> encodeAnalysInto
+ * sequence, plan.ts:436-478); no reference counterpart of their own. */
+int tsphere_leg_synth_multi(tsphere_plan p, int nfield, const float *const *qlm, float *fm,
+                            tsphere_stream s);
+int tsphere_leg_analys_multi(tsphere_plan p, int nfield, const float *fm, float *const *qlm,
+                             tsphere_stream s);
+/* The plan's device (cos, sin)(2 pi k / nphi) table, 2*nphi floats, which
+ * the row FFT of a generated grid kernel must read to round as the plan's
+ * Fourier stages do. */
+int tsphere_plan_fourier_table(tsphere_plan p, const void **trig);
+
 /* ------------------------------------------- generated element-wise kernels
  * Replaces the WGSL pipeline creation of src/mgpu/plan.ts:145-170 for the
  * kernels buildKernel (src/mgpu/wgsl.ts:321-372) emits: HIP C++ source is
@@ -193,6 +209,13 @@ int tsphere_rtc_check(const char *source, const char *options);
  * params buffer, plan.ts:410-420). */
 int tsphere_kernel_launch(tsphere_kernel k, long long count, int nptr, void *const *ptrs,
                           int nscalar, const float *scalars, tsphere_stream s);
+
+/* Launch a generated kernel on an explicit grid (gx x gy workgroups of
+ * `block` threads) with arguments (pointers..., floats...): the grid kernels,
+ * one workgroup per latitude row (or rows). */
+int tsphere_kernel_launch_grid(tsphere_kernel k, unsigned gx, unsigned gy, unsigned block,
+                               int nptr, void *const *ptrs, int nscalar, const float *scalars,
+                               tsphere_stream s);
 
 /* Ensembles: nptr * per fields in one batched launch per stage, field f at
  * ptr[f / per] + (f % per) * stride floats -- e.g. two species of 64 runs

@@ -275,3 +275,44 @@ def test_adjoint_kernels_compile(key):
            "\n".join(srcs))
     assert "bar0[i] += ob *" in src and "partials[" in src
     assert _lib.load().tsphere_rtc_check(src.encode(), None) == _lib.OK, _lib.last_error()
+
+
+@pytest.mark.parametrize("key", ["brusselator", "schnakenberg", "allencahn"])
+def test_grid_fused_plan(key):
+    """Grid fusion: with an FFT-sized grid, the step's synth -> element-wise
+    -> analys run becomes one grid op (Legendre synthesis, one generated
+    kernel running the row FFTs around the element-wise statements, Legendre
+    analysis) and the generated source -- the shared row-FFT header included
+    -- compiles for gfx950.  Grid values that the step returns are stored;
+    values only analysed are not."""
+    import types
+
+    import tsphere
+    from tsphere.plan import GridOp, HostBuffers, ModelPlan, PlanSpec
+
+    init_fn, step_fn, state, view = compile_fns(key)
+    cfg = tsphere.ShtConfig(63, 63, 64, 128)
+    stub = types.SimpleNamespace(cfg=cfg, fourier_mode="fft", max_fields=8)
+    host = HostBuffers(allocate=False)
+    host.ensure_group(state, 2 * 2080)
+    fb = state + [None] * len(view)
+    step = ModelPlan.create(stub, PlanSpec(step_fn, fb), host, True, "step")
+    kinds = [op.kind for op in step.ops]
+    assert kinds[0] == "grid" and "synth" not in kinds and "analys" not in kinds, kinds
+    g = step.ops[0]
+    assert isinstance(g, GridOp) and len(g.srcs) == len(state) and len(g.dsts) >= 1
+    names = {c.split("_")[0] for c in g.stores}
+    assert g.nstmts >= 1 and step.elementwise_statements() >= g.nstmts
+    src = step.kernel_source()
+    assert "tsfft::synth_row<6>" in src and "tsfft::analys_row<6>" in src
+    # the returned grid views (u, v, ...) are stored by the grid kernel
+    assert len(g.stores) == len(view), (g.stores, view)
+    del names
+    # a DFT-sized grid (nphi not a power of two) keeps the separate stages
+    stub2 = types.SimpleNamespace(cfg=tsphere.ShtConfig(63, 63, 64, 130), fourier_mode="dft",
+                                  max_fields=8)
+    step2 = ModelPlan.create(stub2, PlanSpec(step_fn, fb), host, True, "step")
+    assert "grid" not in [op.kind for op in step2.ops]
+    # and so does grid=False
+    step3 = ModelPlan.create(stub, PlanSpec(step_fn, fb), host, True, "step", grid=False)
+    assert [op.kind for op in step3.ops][0] == "synth"

@@ -29,7 +29,7 @@ DEFAULTS = {
 }
 
 
-def make(key, lmax, nlat, nphi, params, batch=True):
+def make(key, lmax, nlat, nphi, params, batch=True, grid=True):
     import tsphere
     from tsphere.model import GpuModel
 
@@ -37,7 +37,7 @@ def make(key, lmax, nlat, nphi, params, batch=True):
     names, state, view = MODELS[key]
     sht = tsphere.ShtPlan.create(cfg, max_fields=len(state))
     src = open(os.path.join(MFILES, key + ".m")).read()
-    gm = GpuModel.create(sht, cfg, src, names, state, view, batch=batch)
+    gm = GpuModel.create(sht, cfg, src, names, state, view, batch=batch, grid=grid)
     gm.set_params(params)
     return sht, gm
 
@@ -136,11 +136,11 @@ def test_model_checks(gpu, key):
     nlat, nphi = O.grid_for_lmax(lmax, 3)
     sht, gm = make(key, lmax, nlat, nphi, DEFAULTS[key])
     # the fusion guard counts generated element-wise statements (one per .m
-    # line); the batched plan runs them as fewer, fused launches
-    kernels = [l for l in gm.describe()["step"] if l.startswith("kernel")]
-    statements = sum(1 + l.count(" ; ") for l in kernels)
+    # line); the batched plan runs them as fewer, fused launches (a grid op
+    # runs those between the transforms inside its grid kernel)
+    statements = gm.step_plan.elementwise_statements()
     assert statements == {"schnakenberg": 5, "brusselator": 5, "allencahn": 2}[key]
-    assert len(kernels) <= statements
+    assert gm.step_plan.generated_kernels() <= statements
     noise = (0.01 * O.randn(1, nlat * nphi)).astype(np.float32)
     gm.init(noise)
     gm.step(40)
@@ -246,3 +246,36 @@ def test_fused_plan_matches_statementwise(gpu, key):
         sht.destroy()
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("key,lmax,nlat,nphi", [
+    ("brusselator", 31, 32, 64), ("schnakenberg", 15, 17, 32), ("allencahn", 63, 64, 128),
+    ("brusselator", 127, 128, 256), ("schnakenberg", 255, 256, 512)])
+def test_grid_fused_step_matches_separate_stages(gpu, key, lmax, nlat, nphi):
+    """Grid fusion (Legendre synthesis, one generated kernel running the row
+    FFT synthesis, the element-wise statements and the row FFT analysis,
+    Legendre analysis) gives the state and the returned grid values of the
+    plan with separate Fourier stages and element-wise kernel, bit for bit:
+    the row FFT is the library's own code (shared header), the statements
+    the same f32 operations.  Covers several rows per workgroup (small nphi)
+    and an odd latitude count."""
+    emit(f"grid fusion {key} lmax {lmax} {nlat}x{nphi}")
+    outs = []
+    for grid in (False, True):
+        sht, gm = make(key, lmax, nlat, nphi, DEFAULTS[key], grid=grid)
+        kinds = [l.split()[0] for l in gm.describe()["step"]]
+        assert ("grid" in kinds) == grid, kinds
+        rng = np.random.default_rng(7)
+        for s in gm.state:
+            q = (rng.normal(size=2 * sht.nlm) * 0.02).astype(np.float32)
+            q[0] += 1.0
+            gm.upload(s, q)
+        gm.step(20)
+        outs.append([gm.read(s) for s in gm.state] + [gm.read(v) for v in gm.view])
+        gm.destroy()
+        sht.destroy()
+    for a, b in zip(*outs):
+        assert np.all(np.isfinite(b))
+        d = float(np.max(np.abs(a - b)) / max(1e-30, float(np.max(np.abs(a)))))
+        emit(f"  max rel diff {d:.2e}")
+        assert np.array_equal(a, b), d

@@ -31,7 +31,8 @@ def test_addon_loads_and_exports():
     assert out["version"] == _lib.load().tsphere_version()
     for name in ("planCreate", "planQuery", "planDestroy", "synth", "analys", "malloc", "free",
                  "upload", "download", "moduleCompile", "moduleKernel", "kernelLaunch",
-                 "graphBegin", "graphEnd", "graphLaunch", "streamCreate", "streamSync"):
+                 "graphBegin", "graphEnd", "graphLaunch", "streamCreate", "streamSync",
+                 "legSynth", "legAnalys", "fourierTable", "kernelLaunchGrid"):
         assert name in out["names"]
 
 

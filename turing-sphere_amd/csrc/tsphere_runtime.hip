@@ -669,6 +669,32 @@ int tsphere_analys_multi(tsphere_plan p, int nfield, const float* const* spat, f
                     listed((const void* const*)qlm, nfield), S(s));
 }
 
+int tsphere_leg_synth_multi(tsphere_plan p, int nfield, const float* const* qlm, float* fm,
+                            tsphere_stream s) {
+  std::vector<const void*> fmv(nfield > 0 && nfield <= kMaxList ? nfield : 0, fm);
+  if (int r = check_list(p, nfield, (const void* const*)qlm, fmv.data())) return r;
+  TS_HIP(leg_synth(p, nfield, listed((const void* const*)qlm, nfield),
+                   reinterpret_cast<float2*>(fm), S(s)),
+         "leg_synth");
+  return ok();
+}
+
+int tsphere_leg_analys_multi(tsphere_plan p, int nfield, const float* fm, float* const* qlm,
+                             tsphere_stream s) {
+  std::vector<const void*> fmv(nfield > 0 && nfield <= kMaxList ? nfield : 0, fm);
+  if (int r = check_list(p, nfield, fmv.data(), (const void* const*)qlm)) return r;
+  TS_HIP(leg_analys(p, nfield, reinterpret_cast<const float2*>(fm),
+                    listed((const void* const*)qlm, nfield), S(s)),
+         "leg_analys");
+  return ok();
+}
+
+int tsphere_plan_fourier_table(tsphere_plan p, const void** trig) {
+  if (!p || !trig) return fail(TSPHERE_ERR_INVALID, "NULL argument");
+  *trig = p->d_trig;
+  return ok();
+}
+
 static int check_fields(tsphere_plan p, int nptr, int per, const void* const* a,
                         const void* const* b) {
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
@@ -1041,6 +1067,34 @@ int tsphere_kernel_launch(tsphere_kernel k, long long count, int nptr, void* con
   const long long threads = (count + 3) / 4;
   const unsigned blocks = (unsigned)((threads + 255) / 256);
   TS_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, 256, 1, 1, 0, S(s), nullptr, config),
+         "hipModuleLaunchKernel");
+  return ok();
+}
+
+int tsphere_kernel_launch_grid(tsphere_kernel k, unsigned gx, unsigned gy, unsigned block,
+                               int nptr, void* const* ptrs, int nscalar, const float* scalars,
+                               tsphere_stream s) {
+  if (!k) return fail(TSPHERE_ERR_INVALID, "kernel is NULL");
+  if (nptr < 0 || nptr > 64 || nscalar < 0 || nscalar > 64 || block < 1 || block > 1024 ||
+      gx < 1 || gy < 1)
+    return fail(TSPHERE_ERR_INVALID, "kernel_launch_grid: bad grid/block/nptr/nscalar");
+  if ((nptr && !ptrs) || (nscalar && !scalars))
+    return fail(TSPHERE_ERR_INVALID, "kernel_launch_grid: NULL array");
+  // Pack (pointers..., floats...) with natural alignment.
+  alignas(8) unsigned char args[8 * 64 + 4 * 64];
+  size_t off = 0;
+  for (int i = 0; i < nptr; i++) {
+    std::memcpy(args + off, &ptrs[i], 8);
+    off += 8;
+  }
+  for (int i = 0; i < nscalar; i++) {
+    std::memcpy(args + off, &scalars[i], 4);
+    off += 4;
+  }
+  size_t size = off;
+  void* config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                    HIP_LAUNCH_PARAM_END};
+  TS_HIP(hipModuleLaunchKernel(k->fn, gx, gy, 1, block, 1, 1, 0, S(s), nullptr, config),
          "hipModuleLaunchKernel");
   return ok();
 }

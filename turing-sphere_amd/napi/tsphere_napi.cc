@@ -305,6 +305,84 @@ napi_value KernelLaunch(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// Arrays of BigInt pointers / numbers from a JS array.
+bool get_ptrs(napi_env env, napi_value arr, std::vector<void*>& out) {
+  uint32_t n = 0;
+  if (napi_get_array_length(env, arr, &n) != napi_ok) return false;
+  out.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    if (!get_ptr(env, e, &out[i])) return false;
+  }
+  return true;
+}
+
+bool get_floats(napi_env env, napi_value arr, std::vector<float>& out) {
+  uint32_t n = 0;
+  if (napi_get_array_length(env, arr, &n) != napi_ok) return false;
+  out.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    out[i] = static_cast<float>(get_num(env, e));
+  }
+  return true;
+}
+
+// kernelLaunchGrid(kernel, gx, gy, block, [ptr...], [scalar...], stream?)
+napi_value KernelLaunchGrid(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void *k = nullptr, *s = nullptr;
+  std::vector<void*> ptrs;
+  std::vector<float> sc;
+  if (a.argc < 6 || !get_ptr(env, a.v[0], &k) || !get_ptrs(env, a.v[4], ptrs) ||
+      !get_floats(env, a.v[5], sc))
+    return type_error(env, "kernelLaunchGrid(k, gx, gy, block, ptrs, scalars)");
+  if (a.argc >= 7) get_ptr(env, a.v[6], &s);
+  if (int rc = tsphere_kernel_launch_grid(
+          static_cast<tsphere_kernel>(k), static_cast<unsigned>(get_num(env, a.v[1], 0)),
+          static_cast<unsigned>(get_num(env, a.v[2], 0)),
+          static_cast<unsigned>(get_num(env, a.v[3], 0)), static_cast<int>(ptrs.size()),
+          ptrs.data(), static_cast<int>(sc.size()), sc.data(), s))
+    return throw_error(env, rc, "kernelLaunchGrid");
+  return nullptr;
+}
+
+// legSynth(plan, [qlm...], fm, stream?) / legAnalys(plan, fm, [qlm...], stream?)
+template <bool SYNTH>
+napi_value LegStage(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void *p = nullptr, *fm = nullptr, *s = nullptr;
+  std::vector<void*> q;
+  const int qi = SYNTH ? 1 : 2, fi = SYNTH ? 2 : 1;
+  if (a.argc < 3 || !get_ptr(env, a.v[0], &p) || !get_ptrs(env, a.v[qi], q) ||
+      !get_ptr(env, a.v[fi], &fm))
+    return type_error(env, SYNTH ? "legSynth(plan, qlmPtrs, fm)" : "legAnalys(plan, fm, qlmPtrs)");
+  if (a.argc >= 4) get_ptr(env, a.v[3], &s);
+  const int n = static_cast<int>(q.size());
+  const int rc =
+      SYNTH ? tsphere_leg_synth_multi(static_cast<tsphere_plan>(p), n,
+                                      reinterpret_cast<const float* const*>(q.data()),
+                                      static_cast<float*>(fm), s)
+            : tsphere_leg_analys_multi(static_cast<tsphere_plan>(p), n,
+                                       static_cast<const float*>(fm),
+                                       reinterpret_cast<float* const*>(q.data()), s);
+  if (rc) return throw_error(env, rc, SYNTH ? "legSynth" : "legAnalys");
+  return nullptr;
+}
+
+// fourierTable(plan) -> device pointer of the plan's twiddle table
+napi_value FourierTable(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void* p = nullptr;
+  if (a.argc < 1 || !get_ptr(env, a.v[0], &p)) return type_error(env, "fourierTable(plan)");
+  const void* t = nullptr;
+  if (int rc = tsphere_plan_fourier_table(static_cast<tsphere_plan>(p), &t))
+    return throw_error(env, rc, "fourierTable");
+  return make_ptr(env, t);
+}
+
 napi_value GraphBegin(napi_env env, napi_callback_info info) {
   Args a(env, info);
   void* s = nullptr;
@@ -352,6 +430,11 @@ NAPI_MODULE_INIT() {
       {"moduleCompile", nullptr, ModuleCompile, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"moduleKernel", nullptr, ModuleKernel, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"kernelLaunch", nullptr, KernelLaunch, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"kernelLaunchGrid", nullptr, KernelLaunchGrid, nullptr, nullptr, nullptr, napi_default,
+       nullptr},
+      {"legSynth", nullptr, LegStage<true>, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"legAnalys", nullptr, LegStage<false>, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fourierTable", nullptr, FourierTable, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"graphBegin", nullptr, GraphBegin, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"graphEnd", nullptr, GraphEnd, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"graphLaunch", nullptr, GraphLaunch, nullptr, nullptr, nullptr, napi_default, nullptr},

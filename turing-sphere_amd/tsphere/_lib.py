@@ -97,12 +97,17 @@ SIGNATURES = {
     "tsphere_fourier_synth": [_vp, _i, _vp, _vp, _ll, _vp],
     "tsphere_fourier_analys": [_vp, _i, _vp, _ll, _vp, _vp],
     "tsphere_leg_analys": [_vp, _i, _vp, _vp, _ll, _vp],
+    "tsphere_leg_synth_multi": [_vp, _i, C.POINTER(_vp), _vp, _vp],
+    "tsphere_leg_analys_multi": [_vp, _i, _vp, C.POINTER(_vp), _vp],
+    "tsphere_plan_fourier_table": [_vp, C.POINTER(_vp)],
     "tsphere_module_compile": [C.c_char_p, C.c_char_p, C.POINTER(_vp)],
     "tsphere_module_destroy": [_vp],
     "tsphere_module_kernel": [_vp, C.c_char_p, C.POINTER(_vp)],
     "tsphere_kernel_destroy": [_vp],
     "tsphere_rtc_check": [C.c_char_p, C.c_char_p],
     "tsphere_kernel_launch": [_vp, _ll, _i, C.POINTER(_vp), _i, _fp, _vp],
+    "tsphere_kernel_launch_grid": [_vp, C.c_uint, C.c_uint, C.c_uint, _i, C.POINTER(_vp), _i,
+                                   _fp, _vp],
     "tsphere_graph_begin": [_vp],
     "tsphere_graph_end": [_vp, C.POINTER(_vp)],
     "tsphere_graph_launch": [_vp, _vp],

@@ -25,6 +25,7 @@ emitter, with these MI355X-specific choices:
 from __future__ import annotations
 
 import math
+import os
 import re
 from dataclasses import dataclass, field
 
@@ -264,6 +265,7 @@ def build_kernel(stmt: Assign, io: KernelInputs, count: int, label: str, name: s
     elem = f"""{helpers}
 // {label}
 __device__ __forceinline__ float {name}_elem({elem_params}) {{
+#pragma clang fp contract(off)
 {pro}
   return {body};
 }}
@@ -344,18 +346,12 @@ extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long 
                   nparams=npar, body=body)
 
 
-def build_fused_kernel(stmts, ios, count: int, stores, label: str, name: str,
-                       batch: "Batch | None" = None) -> Kernel:
-    """Several consecutive element-wise statements over the same element
-    count as ONE kernel (the planner's fusion of a step's element-wise run):
-    each thread evaluates the statements in program order for its elements,
-    values produced inside the group are registers, and only the c_names in
-    `stores` (values read after the group or leaving the step) are written.
-    External inputs are read before anything is stored, so a statement that
-    reads a buffer a later statement overwrites sees the old value, as the
-    sequential kernels would.  Signature: (long long n, float* out0..,
-    const float* in0.., float p0.. | const float* prm).  Returns the kernel
-    and, via Kernel.body, the input c_names in slot order."""
+def _emit_group(stmts, ios, outs, name: str, label: str):
+    """The per-element device function of a group of element-wise statements:
+    ``NAME_elem(float a<ext>.., float p<k>.., float& o<j>..)`` evaluates the
+    statements in program order (values produced inside the group are
+    registers) and sets o<j> to the value of outs[j].  Returns (code, ext,
+    npar, emitted): ext lists the external tensor inputs in slot order."""
     produced = [st.c_name for st in stmts]
     if len(set(produced)) != len(produced):
         raise UnsupportedOnGpu("a fused group may assign each value once")
@@ -385,12 +381,39 @@ def build_fused_kernel(stmts, ios, count: int, stores, label: str, name: str,
         nslot += 1
     emitted = "\n".join(ctx.prologue + body)
     used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", emitted)}
-    nin, npar, nout = len(ext), len(params), len(stores)
+    nin, npar, nout = len(ext), len(params), len(outs)
     elem_params = ", ".join([f"float a{i}" for i in range(nin)] +
                             [f"float p{i}" for i in range(npar)] +
                             [f"float& o{j}" for j in range(nout)])
     pro = "\n".join("  " + line.strip() for line in ctx.prologue)
-    outs = "\n".join(f"  o{j} = a{slots[c]};" for j, c in enumerate(stores))
+    sets = "\n".join(f"  o{j} = a{slots[c]};" for j, c in enumerate(outs))
+    code = f"""{_pow_helpers(used)}
+// {label}
+__device__ __forceinline__ void {name}_elem({elem_params}) {{
+#pragma clang fp contract(off)
+{pro}
+{chr(10).join(body)}
+{sets}
+}}
+"""
+    return code, ext, npar, emitted
+
+
+def build_fused_kernel(stmts, ios, count: int, stores, label: str, name: str,
+                       batch: "Batch | None" = None) -> Kernel:
+    """Several consecutive element-wise statements over the same element
+    count as ONE kernel (the planner's fusion of a step's element-wise run):
+    each thread evaluates the statements in program order for its elements,
+    values produced inside the group are registers, and only the c_names in
+    `stores` (values read after the group or leaving the step) are written.
+    External inputs are read before anything is stored, so a statement that
+    reads a buffer a later statement overwrites sees the old value, as the
+    sequential kernels would.  Signature: (long long n, float* out0..,
+    const float* in0.., float p0.. | const float* prm).  Returns the kernel
+    and, via Kernel.body, the input c_names in slot order."""
+    code, ext, npar, emitted = _emit_group(stmts, ios, stores, name, label)
+    slots = {c: i for i, c in enumerate(ext)}
+    nin, nout = len(ext), len(stores)
     kout = ", ".join(f"float* out{j}" for j in range(nout))
     kin = "".join(f", const float* in{i}" for i in range(nin))
     aligned = " && ".join([f"ts_al(out{j})" for j in range(nout)] +
@@ -418,15 +441,6 @@ def build_fused_kernel(stmts, ios, count: int, stores, label: str, name: str,
     sargs = ", ".join([f"in{i}[{idx(i, 'i')}]" for i in range(nin)] +
                       [f"p{i}" for i in range(npar)] + [f"s{j}" for j in range(nout)])
     sstores = " ".join(f"out{j}[i] = s{j};" for j in range(nout))
-    helpers = _pow_helpers(used)
-    code = f"""{helpers}
-// {label}
-__device__ __forceinline__ void {name}_elem({elem_params}) {{
-{pro}
-{chr(10).join(body)}
-{outs}
-}}
-"""
     if batch is None:
         kpar = "".join(f", float p{i}" for i in range(npar))
         code += f"""extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long n, {kout}{kin}{kpar}) {{
@@ -485,6 +499,142 @@ extern "C" __global__ __launch_bounds__({WORKGROUP_SIZE}) void {name}(long long 
     return k
 
 
+FFT_ROW_HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                              "csrc", "fft_row.h")
+
+
+def fft_row_source() -> str:
+    """The row-FFT device code the library's Fourier kernels are built from,
+    embedded in generated grid kernels (hipRTC gets it as text)."""
+    with open(FFT_ROW_HEADER) as f:
+        return f.read()
+
+
+def grid_lds_bytes(log2h: int, n_rows: int) -> int:
+    """LDS of a grid kernel: two ping-pong rows plus `n_rows` analysis rows,
+    each RB rows of H complex values per workgroup."""
+    h = 1 << log2h
+    tpr = min(h // 4, 256)
+    rb = 256 // tpr
+    return (2 + n_rows) * rb * h * 8
+
+
+def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, synth, stmts,
+                      ios, stores, analys) -> Kernel:
+    """One latitude row's FFT synthesis -> element-wise statements -> FFT
+    analysis as ONE kernel (the planner's grid fusion).
+
+    ``synth``: c_names of the synthesised grid values, in Fourier-scratch
+    field order (field s of ``fm`` holds value synth[s] after the Legendre
+    synthesis); ``stmts``/``ios``: the element-wise statements; ``stores``:
+    grid values written to their buffers (synthesised or produced); ``analys``:
+    c_names whose rows are analysed into Fourier-scratch field a, in order.
+    The row transforms are the library's (fft_row.h), so the grid values and
+    Fourier coefficients are those of the unfused sequence.  Each workgroup
+    owns whole rows and reads every input row of ``fm`` before it writes any
+    output row, so the analysis overwrites the scratch in place.
+
+    Signature: (float2* fm, const float2* trig, float* st0.., const float*
+    gin0.., float p0..): gin are external grid inputs (neither synthesised nor
+    produced), read from global memory."""
+    produced = [st.c_name for st in stmts]
+    # element-wise outputs the kernel needs: stored values and analysis rows
+    eouts = []
+    for c in list(stores) + list(analys):
+        if c in produced and c not in eouts:
+            eouts.append(c)
+    code, ext, npar, emitted = _emit_group(stmts, ios, eouts, name, label)
+    gins = [c for c in ext if c not in synth]
+    gins += [c for c in analys if c not in produced and c not in synth and c not in gins]
+    for c in stores:
+        if c not in produced and c not in synth:
+            raise UnsupportedOnGpu(f"grid kernel cannot store '{c}', which it does not compute")
+    arow = {c: q for q, c in enumerate(analys)}  # LDS row of each analysed value
+    h = 1 << log2h
+    ns = len(synth)
+    lines = []
+    ap = lines.append
+    ap(f"// {label}")
+    sig = (["float2* __restrict__ fm", "const float2* __restrict__ trig"] +
+           [f"float* st{j}" for j in range(len(stores))] +
+           [f"const float* gin{k}" for k in range(len(gins))] +
+           [f"float p{i}" for i in range(npar)])
+    ap(f'extern "C" __global__ __launch_bounds__(256) void {name}({", ".join(sig)}) {{')
+    ap(f"  using S = tsfft::FftShape<{log2h}>;")
+    ap(f"  constexpr int NLAT = {nlat}, MMAX = {mmax}, NP = 4 * S::BPT;")
+    ap("  __shared__ float2 lds[2][S::RB * S::H];")
+    ap(f"  __shared__ float2 arow[{max(1, len(analys))}][S::RB * S::H];")
+    ap("  const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;")
+    ap("  const int ilat = blockIdx.x * S::RB + r;")
+    ap("  const bool row = ilat < NLAT;")
+    ap("  const int il = row ? ilat : NLAT - 1;")
+    ap("  float2* a = lds[0] + r * S::H;")
+    ap("  float2* b = lds[1] + r * S::H;")
+    ap("  const long long g0 = (long long)il * S::H;  // this row, in complex (float2) units")
+    for si in range(ns):
+        ap(f"  float2 sv{si}[NP];")
+        ap("  {")
+        ap(f"    const float2* res = tsfft::synth_row<{log2h}>(fm + ((long long){si} * NLAT + il) "
+           f"* (MMAX + 1), MMAX, 1.0f, trig, a, b, t);")
+        ap("#pragma unroll")
+        ap(f"    for (int i = 0; i < NP; ++i) sv{si}[i] = res[t + S::TPR * i];")
+        ap("    __syncthreads();  // the next row's first pass overwrites the ping-pong rows")
+        ap("  }")
+    # element-wise over this thread's points (complex index c: reals 2c, 2c+1)
+    ap("#pragma unroll")
+    ap("  for (int i = 0; i < NP; ++i) {")
+    ap("    const int c = t + S::TPR * i;")
+    for k in range(len(gins)):
+        ap(f"    const float2 gv{k} = reinterpret_cast<const float2*>(gin{k})[g0 + c];")
+
+    def operand(cn, comp):
+        if cn in synth:
+            return f"sv{synth.index(cn)}[i].{comp}"
+        return f"gv{gins.index(cn)}.{comp}"
+
+    for j in range(len(eouts)):
+        ap(f"    float2 e{j};")
+    for comp in ("x", "y"):
+        args = ([operand(cn, comp) for cn in ext] + [f"p{i}" for i in range(npar)] +
+                [f"e{j}.{comp}" for j in range(len(eouts))])
+        ap(f"    {name}_elem({', '.join(args)});")
+
+    def value(cn):
+        if cn in eouts:
+            return f"e{eouts.index(cn)}"
+        if cn in synth:
+            return f"sv{synth.index(cn)}[i]"
+        return f"gv{gins.index(cn)}"
+
+    ap("    if (row) {")
+    for j, cn in enumerate(stores):
+        ap(f"      reinterpret_cast<float2*>(st{j})[g0 + c] = {value(cn)};")
+    ap("    }")
+    for cn, q in arow.items():
+        ap(f"    arow[{q}][r * S::H + c] = {value(cn)};")
+    ap("  }")
+    ap("  __syncthreads();")
+    for q, cn in enumerate(analys):
+        ap("  {")
+        ap(f"    const float2* src = arow[{q}] + r * S::H;")
+        ap(f"    const float2* z = tsfft::analys_row<{log2h}>([&](int j) {{ return src[j]; }}, "
+           f"trig, a, b, t);")
+        ap("    if (row) {")
+        ap(f"      float2* G = fm + ((long long){q} * NLAT + ilat) * (MMAX + 1);")
+        ap("      for (int k = t; k <= MMAX; k += S::TPR)")
+        ap(f"        G[k] = tsfft::analys_coef<{log2h}>(z, k, trig);")
+        ap("    }")
+        ap("    __syncthreads();")
+        ap("  }")
+    ap("}")
+    k = Kernel(code=code + "\n".join(lines) + "\n", name=name, count=nlat * 2 * h, label=label,
+               ninputs=len(gins), nparams=npar, body=emitted)
+    k.inputs = gins
+    k.stores = list(stores)
+    k.grid = (log2h, nlat)
+    return k
+
+
 @dataclass
 class Batch:
     """Ensemble shape of a generated kernel: `runs` runs; inputs whose
@@ -505,6 +655,8 @@ def module_source(kernels) -> str:
     model compiles with one hipRTC invocation."""
     seen = set()
     parts = [MODULE_PRELUDE]
+    if any(getattr(k, "grid", None) for k in kernels):
+        parts.append(fft_row_source())
     for k in kernels:
         code = k.code
         # pow helpers are shared: emit each once per module

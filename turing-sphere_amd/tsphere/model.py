@@ -44,11 +44,13 @@ class GpuModel:
     @staticmethod
     def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
                stream: Stream | None = None, batch: bool = True,
-               file_name: str = "model.m", runs: int = 1) -> "GpuModel":
+               file_name: str = "model.m", runs: int = 1, grid: bool = True) -> "GpuModel":
         """GpuModel.create (model.ts:103-163).  ``runs`` > 1: an ensemble of
         that many independent runs of the model (same grid), each value
         holding the runs back to back; the plan needs
-        max_fields >= runs * len(state)."""
+        max_fields >= runs * len(state).  ``grid`` lets the planner fuse the
+        grid stage (FFT synthesis, element-wise statements, FFT analysis)
+        into one kernel where it can."""
         self = GpuModel()
         self.runs = int(runs)
         npts = cfg.nlat * cfg.nphi
@@ -76,12 +78,16 @@ class GpuModel:
         self._host = host
         shared = ("lam",)
         self._init = ModelPlan.create(sht, PlanSpec(init_fn, feedback), host, batch, "init",
-                                      runs=R, shared=shared)
+                                      runs=R, shared=shared, grid=grid)
         self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step",
-                                      runs=R, shared=shared)
+                                      runs=R, shared=shared, grid=grid)
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm * R, "mgpu-state-stash")
         return self
+
+    @property
+    def step_plan(self) -> ModelPlan:
+        return self._step
 
     def set_params(self, params: dict) -> None:
         """Scalars; an ensemble also takes one value per run."""

@@ -31,8 +31,8 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceBuffer, Stream
-from .hipgen import (Batch, KernelInputs, UnsupportedOnGpu, build_fused_kernel, build_kernel,
-                     module_source)
+from .hipgen import (Batch, KernelInputs, UnsupportedOnGpu, build_fused_kernel,
+                     build_grid_kernel, build_kernel, grid_lds_bytes, module_source)
 from .ir import Assign, CompiledFunction, is_numeric, is_tensor, numel, walk
 
 EXTERNAL_OPS = ("synth", "analys")  # externals.ts:95
@@ -143,6 +143,26 @@ class FusedOp:
 
 
 @dataclass
+class GridOp:
+    """Legendre synthesis of `srcs` into the plan's Fourier scratch, ONE
+    generated grid kernel (per latitude row: FFT synthesis, the element-wise
+    statements, FFT analysis), Legendre analysis into `dsts`: the grid fusion
+    of a synth -> element-wise -> analys run.  The kernel writes `stores`
+    and reads `ins` (external grid values)."""
+    kind: str
+    label: str
+    srcs: list
+    dsts: list
+    kernel: object
+    stores: list
+    ins: list
+    members: list           # the ops it replaces
+    nstmts: int = 0
+    handle: object = None
+    blocks: int = 0
+
+
+@dataclass
 class XformOp:
     kind: str               # 'synth' | 'analys'
     label: str
@@ -164,6 +184,8 @@ def _reads(op) -> set:
         return set(op.ins)
     if isinstance(op, FusedOp):
         return set(op.ins)
+    if isinstance(op, GridOp):
+        return set(op.srcs) | set(op.ins)
     if isinstance(op, XformOp):
         return set(op.srcs)
     return {op.src}
@@ -174,6 +196,8 @@ def _writes(op) -> set:
         return {op.out}
     if isinstance(op, FusedOp):
         return set(op.outs)
+    if isinstance(op, GridOp):
+        return set(op.dsts) | set(op.stores)
     if isinstance(op, XformOp):
         return set(op.dsts)
     return {"@host:" + op.dst_host, *op.dst_keys}
@@ -197,11 +221,14 @@ class ModelPlan:
         self._ptr_cache = {}
         self.runs = 1
         self._prm = None       # ensembles: device copy of the per-run scalars
+        self._fm = None        # grid ops: Fourier scratch between the stages
+        self._trig = None      # grid ops: the SHT plan's device twiddle table
 
     # --------------------------------------------------------- creation
     @staticmethod
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
-               label: str = "step", runs: int = 1, shared=(), fuse: bool = True) -> "ModelPlan":
+               label: str = "step", runs: int = 1, shared=(), fuse: bool = True,
+               grid: bool = True) -> "ModelPlan":
         """ModelPlan.create (plan.ts:206-407).
 
         ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
@@ -209,7 +236,12 @@ class ModelPlan:
         copy, e.g. the eigenvalues), each transform moves all runs of all its
         species in one batched launch per stage, and element-wise kernels
         read their scalars per run from a device array -- so a parameter
-        change is a small upload, not a re-capture."""
+        change is a small upload, not a re-capture.
+
+        ``grid`` (with ``fuse``, one run, FFT Fourier stages): a synth ->
+        element-wise -> analys run of ops becomes a GridOp, whose generated
+        kernel keeps the grid values between the row FFTs in LDS and
+        registers."""
         self = ModelPlan()
         self._sht = sht
         self.runs = runs = int(runs)
@@ -318,6 +350,8 @@ class ModelPlan:
         ops = self._fold_feedback(ops, host, host_of)
         if batch and fuse:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
+            if grid and runs == 1:
+                ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, label)
 
         # Buffers for every value the .m computes (reused across steps).
         for c, n in counts.items():
@@ -329,14 +363,21 @@ class ModelPlan:
             self._by_name[names[c]] = self._bufs[c]
         self.ops = ops
 
-        pending_kernels = [op for op in ops if isinstance(op, (KernelOp, FusedOp))]
-        if pending_kernels and not host.allocate:
+        pending_kernels = [op for op in ops if isinstance(op, (KernelOp, FusedOp, GridOp))]
+        if any(isinstance(op, GridOp) for op in ops) and host.allocate:
+            nfm = max(max(len(op.srcs), len(op.dsts)) for op in ops if isinstance(op, GridOp))
+            cfg = sht.cfg
+            self._fm = DeviceBuffer(nfm * cfg.nlat * (cfg.mmax + 1) * 2, f"mgpu-{label}-fm")
+            trig = C.c_void_p()
+            _lib.call("tsphere_plan_fourier_table", sht.handle, C.byref(trig))
+            self._trig = trig.value
+        if pending_kernels:
             src = module_source([op.kernel for op in pending_kernels])
+        if pending_kernels and not host.allocate:
             if _lib.load().tsphere_rtc_check(src.encode(), None) != _lib.OK:
                 raise UnsupportedOnGpu(f"generated HIP failed to compile for '{label}':\n"
                                        f"{_lib.last_error()}\n--- source ---\n{src}")
         elif pending_kernels:
-            src = module_source([op.kernel for op in pending_kernels])
             self._module = _compile(src, label)
             for op in pending_kernels:
                 h = C.c_void_p()
@@ -385,6 +426,54 @@ class ModelPlan:
                                count=group[0].count))
             del produced
             i = j
+        return out
+
+    def _fuse_grid(self, ops, outputs: set, host: HostBuffers, label: str) -> list:
+        """Turn each synth -> element-wise -> analys run into a GridOp: the
+        element-wise statements run between the row FFTs of one generated
+        kernel, so the synthesised grid values are written only if something
+        else reads them (a function output, a later op, a host buffer) and the
+        analysed values never leave the workgroup."""
+        sht = self._sht
+        if sht is None or getattr(sht, "fourier_mode", None) != "fft":
+            return ops
+        cfg = sht.cfg
+        h = cfg.nphi // 2
+        log2h = h.bit_length() - 1
+        if (1 << log2h) != h or not 2 <= log2h <= 11:
+            return ops
+        host_bufs = {id(b) for b in host._slots.values()}
+        out = []
+        i = 0
+        while i < len(ops):
+            S = ops[i]
+            E = ops[i + 1] if i + 1 < len(ops) else None
+            A = ops[i + 2] if i + 2 < len(ops) else None
+            if not (isinstance(S, XformOp) and S.kind == "synth" and
+                    isinstance(E, (KernelOp, FusedOp)) and E.count == cfg.npts and
+                    isinstance(A, XformOp) and A.kind == "analys" and
+                    grid_lds_bytes(log2h, len(A.srcs)) <= 65536):
+                out.append(S)
+                i += 1
+                continue
+            members = E.members if isinstance(E, FusedOp) else [E]
+            stmts = [m.stmt for m in members]
+            ios = [m.io for m in members]
+            produced = [st.c_name for st in stmts]
+            later = set()
+            for o in ops[:i] + ops[i + 3:]:
+                later |= _reads(o)
+            stores = [c for c in list(S.dsts) + produced
+                      if c in outputs or c in later or id(self._bufs.get(c)) in host_bufs]
+            name = f"g{len(out)}_" + "_".join(_ident(st.name) for st in stmts)[:40]
+            lbl = f"{S.label} => {E.label} => {A.label}"
+            kern = build_grid_kernel(name, lbl, log2h, cfg.nlat, cfg.mmax, list(S.dsts), stmts,
+                                     ios, stores, list(A.srcs))
+            rb = 256 // min(h // 4, 256)
+            out.append(GridOp("grid", lbl, list(S.srcs), list(A.dsts), kern, stores,
+                              list(kern.inputs), [S, E, A], nstmts=len(stmts),
+                              blocks=(cfg.nlat + rb - 1) // rb))
+            i += 3
         return out
 
     def _fold_feedback(self, ops, host: HostBuffers, host_of: dict) -> list:
@@ -451,8 +540,10 @@ class ModelPlan:
         key = tuple(cnames)
         arr = self._ptr_cache.get(key)
         if arr is None:
+            special = {"@prm": lambda: self._prm.ptr, "@fm": lambda: self._fm.ptr,
+                       "@trig": lambda: self._trig}
             arr = (C.c_void_p * len(cnames))(*[
-                self._prm.ptr if c == "@prm" else self._bufs[c].ptr for c in cnames])
+                special[c]() if c in special else self._bufs[c].ptr for c in cnames])
             self._ptr_cache[key] = arr
         return arr
 
@@ -473,6 +564,13 @@ class ModelPlan:
                     ptrs = self._ptrs(outs + op.ins + (["@prm"] if ens else []))
                     _lib.check(L.tsphere_kernel_launch(op.handle, op.count, len(ptrs), ptrs, np_,
                                                        pvals, sh), op.label)
+                elif isinstance(op, GridOp):
+                    h = self._sht.handle
+                    _lib.check(L.tsphere_leg_synth_multi(h, len(op.srcs), self._ptrs(op.srcs),
+                                                         self._fm.ptr, sh), op.label)
+                    self.launch_grid(op, stream)
+                    _lib.check(L.tsphere_leg_analys_multi(h, len(op.dsts), self._fm.ptr,
+                                                          self._ptrs(op.dsts), sh), op.label)
                 elif isinstance(op, XformOp):
                     srcs = self._ptrs(op.srcs)
                     dsts = self._ptrs(op.dsts)
@@ -491,6 +589,15 @@ class ModelPlan:
                     src = self._bufs[op.src]
                     dst = self._host.get(op.dst_host)
                     _lib.check(L.tsphere_memcpy_dtod(dst.ptr, src.ptr, src.nbytes, sh), op.label)
+
+    def launch_grid(self, op: "GridOp", stream: Stream) -> None:
+        """Enqueue a grid op's generated kernel alone (between its Legendre
+        stages; also what a timing of that kernel runs)."""
+        ptrs = self._ptrs(["@fm", "@trig"] + op.stores + op.ins)
+        pvals = self._params.ctypes.data_as(C.POINTER(C.c_float))
+        _lib.check(_lib.load().tsphere_kernel_launch_grid(
+            op.handle, op.blocks, 1, 256, len(ptrs), ptrs, len(self.param_names), pvals,
+            stream.handle), op.label)
 
     def run(self, stream: Stream, steps: int = 1, graph_steps: int = 1) -> None:
         """Replay `steps` timesteps through a HIP graph holding `graph_steps`
@@ -521,7 +628,16 @@ class ModelPlan:
 
     def kernel_source(self) -> str:
         return module_source([op.kernel for op in self.ops
-                              if isinstance(op, (KernelOp, FusedOp))])
+                              if isinstance(op, (KernelOp, FusedOp, GridOp))])
+
+    def generated_kernels(self) -> int:
+        """Generated kernels launched per call."""
+        return sum(isinstance(op, (KernelOp, FusedOp, GridOp)) for op in self.ops)
+
+    def elementwise_statements(self) -> int:
+        """Element-wise .m statements the generated kernels evaluate."""
+        return sum(1 if isinstance(op, KernelOp) else len(op.members) if isinstance(op, FusedOp)
+                   else op.nstmts if isinstance(op, GridOp) else 0 for op in self.ops)
 
     def destroy(self) -> None:
         self.invalidate()
@@ -537,6 +653,9 @@ class ModelPlan:
         if self._prm is not None:
             self._prm.free()
             self._prm = None
+        if self._fm is not None:
+            self._fm.free()
+            self._fm = None
 
 
 def _ident(name: str) -> str:

@@ -473,9 +473,8 @@ def run(spec: RunSpec, batch: int = 16, latency_steps: int = 200, digest: bool =
     try:
         s.seed(spec.seed)
         plan = s.describe()
-        klines = [l for l in plan["step"] if l.startswith("kernel")]
-        kernels = len(klines)                                   # generated kernels launched
-        statements = sum(1 + l.count(" ; ") for l in klines)    # element-wise .m lines
+        kernels = s.gpu.step_plan.generated_kernels()           # generated kernels launched
+        statements = s.gpu.step_plan.elementwise_statements()   # element-wise .m lines
         s.step(spec.warmup)
         s.sync()
         batches = max(1, math.ceil(spec.steps / batch))
