@@ -181,6 +181,13 @@ int tsphere_leg_synth_multi(tsphere_plan p, int nfield, const float *const *qlm,
                             tsphere_stream s);
 int tsphere_leg_analys_multi(tsphere_plan p, int nfield, const float *fm, float *const *qlm,
                              tsphere_stream s);
+/* The same for nptr * per fields addressed as tsphere_synth_fields (field f
+ * at qlm[f / per] + (f % per) * qlm_stride floats; Fourier-scratch field f):
+ * an ensemble's grid ops. */
+int tsphere_leg_synth_fields(tsphere_plan p, int nptr, int per, const float *const *qlm,
+                             long long qlm_stride, float *fm, tsphere_stream s);
+int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float *fm,
+                              float *const *qlm, long long qlm_stride, tsphere_stream s);
 /* The plan's device (cos, sin)(2 pi k / nphi) table, 2*nphi floats, which
  * the row FFT of a generated grid kernel must read to round as the plan's
  * Fourier stages do. */

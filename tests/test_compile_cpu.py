@@ -316,3 +316,28 @@ def test_grid_fused_plan(key):
     # and so does grid=False
     step3 = ModelPlan.create(stub, PlanSpec(step_fn, fb), host, True, "step", grid=False)
     assert [op.kind for op in step3.ops][0] == "synth"
+
+
+def test_grid_fused_ensemble_plan():
+    """Ensemble grid fusion: the grid kernel runs every run's rows (grid y =
+    run), reads its scalars per run from the device array and compiles."""
+    import types
+
+    import tsphere
+    from tsphere.plan import GridOp, HostBuffers, ModelPlan, PlanSpec
+
+    R = 4
+    init_fn, step_fn, state, view = compile_fns("schnakenberg")
+    stub = types.SimpleNamespace(cfg=tsphere.ShtConfig(63, 63, 64, 128), fourier_mode="fft",
+                                 max_fields=8)
+    host = HostBuffers(allocate=False)
+    host.ensure_group(state, 2 * 2080 * R)
+    host.ensure("lam", 2 * 2080)
+    fb = state + [None] * len(view)
+    step = ModelPlan.create(stub, PlanSpec(step_fn, fb), host, True, "step", runs=R,
+                            shared=("lam",))
+    g = step.ops[0]
+    assert isinstance(g, GridOp)
+    src = step.kernel_source()
+    assert "const int run = blockIdx.y;" in src and "prm[run * 5 + " in src
+    assert "(0 * R + run)" in src and "R = 4" in src

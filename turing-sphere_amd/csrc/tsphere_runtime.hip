@@ -742,6 +742,28 @@ int tsphere_analys_fields(tsphere_plan p, int nptr, int per, const float* const*
                     grouped((const void* const*)qlm, nptr, per, qlm_stride), S(s));
 }
 
+int tsphere_leg_synth_fields(tsphere_plan p, int nptr, int per, const float* const* qlm,
+                             long long qlm_stride, float* fm, tsphere_stream s) {
+  std::vector<const void*> fmv(nptr > 0 && nptr <= kMaxList ? nptr : 0, fm);
+  if (int r = check_fields(p, nptr, per, (const void* const*)qlm, fmv.data())) return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  TS_HIP(leg_synth(p, nptr * per, grouped((const void* const*)qlm, nptr, per, qlm_stride),
+                   reinterpret_cast<float2*>(fm), S(s)),
+         "leg_synth");
+  return ok();
+}
+
+int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float* fm,
+                              float* const* qlm, long long qlm_stride, tsphere_stream s) {
+  std::vector<const void*> fmv(nptr > 0 && nptr <= kMaxList ? nptr : 0, fm);
+  if (int r = check_fields(p, nptr, per, fmv.data(), (const void* const*)qlm)) return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  TS_HIP(leg_analys(p, nptr * per, reinterpret_cast<const float2*>(fm),
+                    grouped((const void* const*)qlm, nptr, per, qlm_stride), S(s)),
+         "leg_analys");
+  return ok();
+}
+
 // ------------------------------------------------------- transposes
 // synth^T: G = 2^{[m>=1]} * FourierAnalysis(g), q = sum_i P_lm(theta_i) G_m(theta_i)
 // analys^T: F = LegendreSynth(q), g_ij = w_i * FourierSynth(F with m>=1 halved)

@@ -520,7 +520,7 @@ def grid_lds_bytes(log2h: int, n_rows: int) -> int:
 
 
 def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, synth, stmts,
-                      ios, stores, analys) -> Kernel:
+                      ios, stores, analys, batch: "Batch | None" = None) -> Kernel:
     """One latitude row's FFT synthesis -> element-wise statements -> FFT
     analysis as ONE kernel (the planner's grid fusion).
 
@@ -535,8 +535,14 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     output row, so the analysis overwrites the scratch in place.
 
     Signature: (float2* fm, const float2* trig, float* st0.., const float*
-    gin0.., float p0..): gin are external grid inputs (neither synthesised nor
-    produced), read from global memory."""
+    gin0.., float p0.. | const float* prm): gin are external grid inputs
+    (neither synthesised nor produced), read from global memory.
+
+    With ``batch`` (an ensemble of R runs) the grid is (row tiles, R):
+    workgroup (x, run) takes the rows of one run, Fourier-scratch field
+    s * R + run for value s (the batched Legendre stages' field order), grid
+    values at run * nlat * nphi (one copy for inputs in ``batch.shared``),
+    scalars from prm[run * npar + k]."""
     produced = [st.c_name for st in stmts]
     # element-wise outputs the kernel needs: stored values and analysis rows
     eouts = []
@@ -552,16 +558,24 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     arow = {c: q for q, c in enumerate(analys)}  # LDS row of each analysed value
     h = 1 << log2h
     ns = len(synth)
+    R = batch.runs if batch is not None else 1
+    shared = set(batch.shared) if batch is not None else set()
     lines = []
     ap = lines.append
     ap(f"// {label}")
     sig = (["float2* __restrict__ fm", "const float2* __restrict__ trig"] +
            [f"float* st{j}" for j in range(len(stores))] +
            [f"const float* gin{k}" for k in range(len(gins))] +
-           [f"float p{i}" for i in range(npar)])
+           ([f"float p{i}" for i in range(npar)] if batch is None else
+            ["const float* __restrict__ prm"]))
     ap(f'extern "C" __global__ __launch_bounds__(256) void {name}({", ".join(sig)}) {{')
     ap(f"  using S = tsfft::FftShape<{log2h}>;")
-    ap(f"  constexpr int NLAT = {nlat}, MMAX = {mmax}, NP = 4 * S::BPT;")
+    ap(f"  constexpr int NLAT = {nlat}, MMAX = {mmax}, NP = 4 * S::BPT, R = {R};")
+    ap("  const int run = blockIdx.y;")
+    ap("  const long long grun = (long long)run * NLAT * S::H;  // a run's grid, float2 units")
+    if batch is not None:
+        for i in range(npar):
+            ap(f"  const float p{i} = prm[run * {npar} + {i}];")
     ap("  __shared__ float2 lds[2][S::RB * S::H];")
     ap(f"  __shared__ float2 arow[{max(1, len(analys))}][S::RB * S::H];")
     ap("  const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;")
@@ -574,8 +588,8 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     for si in range(ns):
         ap(f"  float2 sv{si}[NP];")
         ap("  {")
-        ap(f"    const float2* res = tsfft::synth_row<{log2h}>(fm + ((long long){si} * NLAT + il) "
-           f"* (MMAX + 1), MMAX, 1.0f, trig, a, b, t);")
+        ap(f"    const float2* res = tsfft::synth_row<{log2h}>(fm + ((long long)({si} * R + run) "
+           f"* NLAT + il) * (MMAX + 1), MMAX, 1.0f, trig, a, b, t);")
         ap("#pragma unroll")
         ap(f"    for (int i = 0; i < NP; ++i) sv{si}[i] = res[t + S::TPR * i];")
         ap("    __syncthreads();  // the next row's first pass overwrites the ping-pong rows")
@@ -584,8 +598,11 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     ap("#pragma unroll")
     ap("  for (int i = 0; i < NP; ++i) {")
     ap("    const int c = t + S::TPR * i;")
-    for k in range(len(gins)):
-        ap(f"    const float2 gv{k} = reinterpret_cast<const float2*>(gin{k})[g0 + c];")
+    def goff(cn):
+        return "g0 + c" if cn in shared else "grun + g0 + c"
+
+    for k, cn in enumerate(gins):
+        ap(f"    const float2 gv{k} = reinterpret_cast<const float2*>(gin{k})[{goff(cn)}];")
 
     def operand(cn, comp):
         if cn in synth:
@@ -608,7 +625,7 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
 
     ap("    if (row) {")
     for j, cn in enumerate(stores):
-        ap(f"      reinterpret_cast<float2*>(st{j})[g0 + c] = {value(cn)};")
+        ap(f"      reinterpret_cast<float2*>(st{j})[grun + g0 + c] = {value(cn)};")
     ap("    }")
     for cn, q in arow.items():
         ap(f"    arow[{q}][r * S::H + c] = {value(cn)};")
@@ -620,15 +637,15 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
         ap(f"    const float2* z = tsfft::analys_row<{log2h}>([&](int j) {{ return src[j]; }}, "
            f"trig, a, b, t);")
         ap("    if (row) {")
-        ap(f"      float2* G = fm + ((long long){q} * NLAT + ilat) * (MMAX + 1);")
+        ap(f"      float2* G = fm + ((long long)({q} * R + run) * NLAT + ilat) * (MMAX + 1);")
         ap("      for (int k = t; k <= MMAX; k += S::TPR)")
         ap(f"        G[k] = tsfft::analys_coef<{log2h}>(z, k, trig);")
         ap("    }")
         ap("    __syncthreads();")
         ap("  }")
     ap("}")
-    k = Kernel(code=code + "\n".join(lines) + "\n", name=name, count=nlat * 2 * h, label=label,
-               ninputs=len(gins), nparams=npar, body=emitted)
+    k = Kernel(code=code + "\n".join(lines) + "\n", name=name, count=nlat * 2 * h * R,
+               label=label, ninputs=len(gins), nparams=npar, body=emitted)
     k.inputs = gins
     k.stores = list(stores)
     k.grid = (log2h, nlat)

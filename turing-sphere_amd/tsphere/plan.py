@@ -238,7 +238,7 @@ class ModelPlan:
         read their scalars per run from a device array -- so a parameter
         change is a small upload, not a re-capture.
 
-        ``grid`` (with ``fuse``, one run, FFT Fourier stages): a synth ->
+        ``grid`` (with ``fuse``, FFT Fourier stages): a synth ->
         element-wise -> analys run of ops becomes a GridOp, whose generated
         kernel keeps the grid values between the row FFTs in LDS and
         registers."""
@@ -350,8 +350,8 @@ class ModelPlan:
         ops = self._fold_feedback(ops, host, host_of)
         if batch and fuse:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
-            if grid and runs == 1:
-                ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, label)
+            if grid:
+                ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
 
         # Buffers for every value the .m computes (reused across steps).
         for c, n in counts.items():
@@ -365,7 +365,8 @@ class ModelPlan:
 
         pending_kernels = [op for op in ops if isinstance(op, (KernelOp, FusedOp, GridOp))]
         if any(isinstance(op, GridOp) for op in ops) and host.allocate:
-            nfm = max(max(len(op.srcs), len(op.dsts)) for op in ops if isinstance(op, GridOp))
+            nfm = runs * max(max(len(op.srcs), len(op.dsts)) for op in ops
+                             if isinstance(op, GridOp))
             cfg = sht.cfg
             self._fm = DeviceBuffer(nfm * cfg.nlat * (cfg.mmax + 1) * 2, f"mgpu-{label}-fm")
             trig = C.c_void_p()
@@ -428,7 +429,7 @@ class ModelPlan:
             i = j
         return out
 
-    def _fuse_grid(self, ops, outputs: set, host: HostBuffers, label: str) -> list:
+    def _fuse_grid(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c) -> list:
         """Turn each synth -> element-wise -> analys run into a GridOp: the
         element-wise statements run between the row FFTs of one generated
         kernel, so the synthesised grid values are written only if something
@@ -450,7 +451,7 @@ class ModelPlan:
             E = ops[i + 1] if i + 1 < len(ops) else None
             A = ops[i + 2] if i + 2 < len(ops) else None
             if not (isinstance(S, XformOp) and S.kind == "synth" and
-                    isinstance(E, (KernelOp, FusedOp)) and E.count == cfg.npts and
+                    isinstance(E, (KernelOp, FusedOp)) and E.count == cfg.npts * runs and
                     isinstance(A, XformOp) and A.kind == "analys" and
                     grid_lds_bytes(log2h, len(A.srcs)) <= 65536):
                 out.append(S)
@@ -468,7 +469,8 @@ class ModelPlan:
             name = f"g{len(out)}_" + "_".join(_ident(st.name) for st in stmts)[:40]
             lbl = f"{S.label} => {E.label} => {A.label}"
             kern = build_grid_kernel(name, lbl, log2h, cfg.nlat, cfg.mmax, list(S.dsts), stmts,
-                                     ios, stores, list(A.srcs))
+                                     ios, stores, list(A.srcs),
+                                     Batch(runs, shared_c) if runs > 1 else None)
             rb = 256 // min(h // 4, 256)
             out.append(GridOp("grid", lbl, list(S.srcs), list(A.dsts), kern, stores,
                               list(kern.inputs), [S, E, A], nstmts=len(stmts),
@@ -566,11 +568,21 @@ class ModelPlan:
                                                        pvals, sh), op.label)
                 elif isinstance(op, GridOp):
                     h = self._sht.handle
-                    _lib.check(L.tsphere_leg_synth_multi(h, len(op.srcs), self._ptrs(op.srcs),
-                                                         self._fm.ptr, sh), op.label)
+                    if ens:
+                        rc = L.tsphere_leg_synth_fields(h, len(op.srcs), self.runs,
+                                                        self._ptrs(op.srcs), nq, self._fm.ptr, sh)
+                    else:
+                        rc = L.tsphere_leg_synth_multi(h, len(op.srcs), self._ptrs(op.srcs),
+                                                       self._fm.ptr, sh)
+                    _lib.check(rc, op.label)
                     self.launch_grid(op, stream)
-                    _lib.check(L.tsphere_leg_analys_multi(h, len(op.dsts), self._fm.ptr,
-                                                          self._ptrs(op.dsts), sh), op.label)
+                    if ens:
+                        rc = L.tsphere_leg_analys_fields(h, len(op.dsts), self.runs, self._fm.ptr,
+                                                         self._ptrs(op.dsts), nq, sh)
+                    else:
+                        rc = L.tsphere_leg_analys_multi(h, len(op.dsts), self._fm.ptr,
+                                                        self._ptrs(op.dsts), sh)
+                    _lib.check(rc, op.label)
                 elif isinstance(op, XformOp):
                     srcs = self._ptrs(op.srcs)
                     dsts = self._ptrs(op.dsts)
@@ -593,11 +605,12 @@ class ModelPlan:
     def launch_grid(self, op: "GridOp", stream: Stream) -> None:
         """Enqueue a grid op's generated kernel alone (between its Legendre
         stages; also what a timing of that kernel runs)."""
-        ptrs = self._ptrs(["@fm", "@trig"] + op.stores + op.ins)
+        ens = self.runs > 1
+        ptrs = self._ptrs(["@fm", "@trig"] + op.stores + op.ins + (["@prm"] if ens else []))
         pvals = self._params.ctypes.data_as(C.POINTER(C.c_float))
         _lib.check(_lib.load().tsphere_kernel_launch_grid(
-            op.handle, op.blocks, 1, 256, len(ptrs), ptrs, len(self.param_names), pvals,
-            stream.handle), op.label)
+            op.handle, op.blocks, self.runs, 256, len(ptrs), ptrs,
+            0 if ens else len(self.param_names), pvals, stream.handle), op.label)
 
     def run(self, stream: Stream, steps: int = 1, graph_steps: int = 1) -> None:
         """Replay `steps` timesteps through a HIP graph holding `graph_steps`
