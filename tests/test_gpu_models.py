@@ -248,18 +248,24 @@ def test_fused_plan_matches_statementwise(gpu, key):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("groups", [True, False])
 @pytest.mark.parametrize("key,lmax,nlat,nphi", [
     ("brusselator", 31, 32, 64), ("schnakenberg", 15, 17, 32), ("allencahn", 63, 64, 128),
     ("brusselator", 127, 128, 256), ("schnakenberg", 255, 256, 512)])
-def test_grid_fused_step_matches_separate_stages(gpu, key, lmax, nlat, nphi):
+def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nlat, nphi,
+                                                 groups):
     """Grid fusion (Legendre synthesis, one generated kernel running the row
     FFT synthesis, the element-wise statements and the row FFT analysis,
     Legendre analysis) gives the state and the returned grid values of the
     plan with separate Fourier stages and element-wise kernel, bit for bit:
     the row FFT is the library's own code (shared header), the statements
-    the same f32 operations.  Covers several rows per workgroup (small nphi)
-    and an odd latitude count."""
-    emit(f"grid fusion {key} lmax {lmax} {nlat}x{nphi}")
+    the same f32 operations.  Both kernel forms (one thread group per field's
+    row FFT, or one group transforming the rows in turn); several rows per
+    workgroup (small nphi) and an odd latitude count."""
+    import tsphere.hipgen
+
+    monkeypatch.setattr(tsphere.hipgen, "GRID_GROUPS", groups)
+    emit(f"grid fusion {key} lmax {lmax} {nlat}x{nphi} groups {groups}")
     outs = []
     for grid in (False, True):
         sht, gm = make(key, lmax, nlat, nphi, DEFAULTS[key], grid=grid)

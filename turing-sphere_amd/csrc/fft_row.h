@@ -138,6 +138,26 @@ __device__ __forceinline__ float2* rest_passes(float2* a, float2* b,
   return a;
 }
 
+// The passes after the first for workgroups in which several groups of 256
+// threads each transform their own row: `active` (uniform per group) masks a
+// group's arithmetic but never its barriers, so every thread of the
+// workgroup meets the same barriers.  Same result buffer as rest_passes.
+template <int SIGN, int LOG2H>
+__device__ __forceinline__ float2* rest_passes_masked(bool active, float2* a, float2* b,
+                                                      const float2* __restrict__ trig, int t) {
+  using S = FftShape<LOG2H>;
+#pragma unroll
+  for (int Ns = S::ODD ? 2 : 4; Ns < S::H; Ns <<= 2) {
+    __syncthreads();
+    if (active) r4_pass<SIGN, LOG2H>(a, b, trig, t, Ns);
+    float2* tmp = a;
+    a = b;
+    b = tmp;
+  }
+  __syncthreads();
+  return a;
+}
+
 // Synthesis pre-processing, fourier.ts:309-350: Z[k] from the row's
 // Hermitian spectrum F[0..mmax] (mmax <= H - 1 since nphi >= 2 mmax + 1),
 // m >= 1 entries scaled by mscale.

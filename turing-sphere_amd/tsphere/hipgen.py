@@ -560,6 +560,14 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     ns = len(synth)
     R = batch.runs if batch is not None else 1
     shared = set(batch.shared) if batch is not None else set()
+    groups = grid_groups(log2h, ns, len(analys))
+    if groups:
+        body = _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
+                                   npar, stores, analys, R, shared, batch is not None, groups)
+        k = Kernel(code=code + body, name=name, count=nlat * 2 * h * R, label=label,
+                   ninputs=len(gins), nparams=npar, body=emitted)
+        k.inputs, k.stores, k.grid, k.block = gins, list(stores), (log2h, nlat), 256 * groups
+        return k
     lines = []
     ap = lines.append
     ap(f"// {label}")
@@ -649,7 +657,139 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
     k.inputs = gins
     k.stores = list(stores)
     k.grid = (log2h, nlat)
+    k.block = 256
     return k
+
+
+GRID_GROUPS = True  # False: always the one-group form (tests compare the two)
+
+
+def grid_groups(log2h: int, ns: int, na: int) -> int:
+    """Thread groups of a group-parallel grid kernel (one 256-thread group per
+    transformed row, each with its own two LDS rows: the fields' row FFTs run
+    side by side), or 0 when they do not fit a workgroup (then the rows are
+    transformed one after another by one group)."""
+    if not GRID_GROUPS:
+        return 0
+    g = max(ns, na, 1)
+    g = 1 if g == 1 else 2 if g == 2 else 4 if g <= 4 else 0
+    if not g:
+        return 0
+    h = 1 << log2h
+    rb = 256 // min(h // 4, 256)
+    return g if g * 2 * rb * h * 8 <= 65536 else 0
+
+
+def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts, npar, stores,
+                        analys, R, shared, ensemble, G) -> str:
+    """Group-parallel grid kernel: group g (threads 256 g .. 256 g + 255)
+    synthesises value synth[g] and later analyses analys[g], each in its own
+    ping-pong rows; between the two, every thread of the workgroup evaluates
+    the element-wise statements over a share of the rows' points, reading the
+    synthesised rows and writing the analysed ones in LDS (in place where a
+    synthesised row is no longer needed: each point is read and written by
+    one thread)."""
+    h = 1 << log2h
+    q0 = 1 if log2h & 1 else 2
+    npass = (log2h - q0) // 2
+    res = npass % 2          # ping-pong row holding a transform's result
+    ns, na = len(synth), len(analys)
+    lines = []
+    ap = lines.append
+    ap(f"// {label}")
+    sig = (["float2* __restrict__ fm", "const float2* __restrict__ trig"] +
+           [f"float* st{j}" for j in range(len(stores))] +
+           [f"const float* gin{k}" for k in range(len(gins))] +
+           ([f"float p{i}" for i in range(npar)] if not ensemble else
+            ["const float* __restrict__ prm"]))
+    ap(f'extern "C" __global__ __launch_bounds__({256 * G}) void {name}({", ".join(sig)}) {{')
+    ap(f"  using S = tsfft::FftShape<{log2h}>;")
+    ap(f"  constexpr int NLAT = {nlat}, MMAX = {mmax}, R = {R}, G = {G};")
+    ap("  constexpr int NPT = S::RB * S::H / (256 * G);  // complex points per thread")
+    ap("  __shared__ float2 lds[G][2][S::RB * S::H];")
+    ap("  const int run = blockIdx.y;")
+    ap("  const long long grun = (long long)run * NLAT * S::H;  // a run's grid, float2 units")
+    if ensemble:
+        for i in range(npar):
+            ap(f"  const float p{i} = prm[run * {npar} + {i}];")
+    ap("  const int g = threadIdx.x >> 8, lt = threadIdx.x & 255;")
+    ap("  const int r = lt / S::TPR, t = lt % S::TPR;")
+    ap("  const int ilat = blockIdx.x * S::RB + r;")
+    ap("  const int il = ilat < NLAT ? ilat : NLAT - 1;")
+    ap("  float2* a = lds[g][0] + r * S::H;")
+    ap("  float2* b = lds[g][1] + r * S::H;")
+    # synthesis: group g < ns
+    ap(f"  const bool sact = g < {ns};")
+    ap("  if (sact) {")
+    ap(f"    const tsfft::SynthInput<{log2h}> in{{fm + ((long long)(g * R + run) * NLAT + il) * "
+       f"(MMAX + 1), trig, MMAX, 1.0f}};")
+    ap(f"    tsfft::first_pass<+1, {log2h}>(in, a, t);")
+    ap("  }")
+    ap(f"  tsfft::rest_passes_masked<+1, {log2h}>(sact, a, b, trig, t);")
+    # element-wise over the block's points
+
+    def src_of(cn):  # LDS row of a synthesised value
+        return f"lds[{synth.index(cn)}][{res}]"
+
+    def dst_of(q):   # LDS row an analysed value is written to / read from
+        return f"lds[{q}][{res}]" if q < ns else f"lds[{q}][0]"
+
+    def goff(cn):
+        return "g0 + c" if cn in shared else "grun + g0 + c"
+
+    ap("#pragma unroll")
+    ap("  for (int i = 0; i < NPT; ++i) {")
+    ap("    const int ca = threadIdx.x + 256 * G * i;   // point index in the block's rows")
+    ap("    const int rr = ca / S::H, c = ca % S::H;")
+    ap("    const int lat = blockIdx.x * S::RB + rr;")
+    ap("    const long long g0 = (long long)(lat < NLAT ? lat : NLAT - 1) * S::H;")
+    for k, cn in enumerate(gins):
+        ap(f"    const float2 gv{k} = reinterpret_cast<const float2*>(gin{k})[{goff(cn)}];")
+    for cn in synth:
+        ap(f"    const float2 sv{synth.index(cn)} = {src_of(cn)}[ca];")
+
+    def operand(cn, comp):
+        if cn in synth:
+            return f"sv{synth.index(cn)}.{comp}"
+        return f"gv{gins.index(cn)}.{comp}"
+
+    for j in range(len(eouts)):
+        ap(f"    float2 e{j};")
+    for comp in ("x", "y"):
+        args = ([operand(cn, comp) for cn in ext] + [f"p{i}" for i in range(npar)] +
+                [f"e{j}.{comp}" for j in range(len(eouts))])
+        ap(f"    {name}_elem({', '.join(args)});")
+
+    def value(cn):
+        if cn in eouts:
+            return f"e{eouts.index(cn)}"
+        if cn in synth:
+            return f"sv{synth.index(cn)}"
+        return f"gv{gins.index(cn)}"
+
+    ap("    if (lat < NLAT) {")
+    for j, cn in enumerate(stores):
+        ap(f"      reinterpret_cast<float2*>(st{j})[grun + g0 + c] = {value(cn)};")
+    ap("    }")
+    for q, cn in enumerate(analys):
+        ap(f"    {dst_of(q)}[ca] = {value(cn)};")
+    ap("  }")
+    ap("  __syncthreads();")
+    # analysis: group g < na reads its row, transforms into the other buffer
+    ap(f"  const bool aact = g < {na};")
+    ap(f"  float2* ain = {'lds[g][' + str(res) + ']' if ns >= na else '(g < ' + str(ns) + ' ? lds[g][' + str(res) + '] : lds[g][0])'} + r * S::H;")
+    ap(f"  float2* aout = (ain == lds[g][0] + r * S::H ? lds[g][1] : lds[g][0]) + r * S::H;")
+    ap("  if (aact) {")
+    ap(f"    tsfft::first_pass<-1, {log2h}>([&](int j) {{ return ain[j]; }}, aout, t);")
+    ap("  }")
+    ap(f"  const float2* z = tsfft::rest_passes_masked<-1, {log2h}>(aact, aout, ain, trig, t);")
+    ap("  if (aact && ilat < NLAT) {")
+    ap("    float2* G2 = fm + ((long long)(g * R + run) * NLAT + ilat) * (MMAX + 1);")
+    ap("    for (int k = t; k <= MMAX; k += S::TPR)")
+    ap(f"      G2[k] = tsfft::analys_coef<{log2h}>(z, k, trig);")
+    ap("  }")
+    ap("}")
+    return "\n".join(lines) + "\n"
 
 
 @dataclass

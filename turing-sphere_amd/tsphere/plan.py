@@ -160,6 +160,7 @@ class GridOp:
     nstmts: int = 0
     handle: object = None
     blocks: int = 0
+    block: int = 256
 
 
 @dataclass
@@ -474,7 +475,7 @@ class ModelPlan:
             rb = 256 // min(h // 4, 256)
             out.append(GridOp("grid", lbl, list(S.srcs), list(A.dsts), kern, stores,
                               list(kern.inputs), [S, E, A], nstmts=len(stmts),
-                              blocks=(cfg.nlat + rb - 1) // rb))
+                              blocks=(cfg.nlat + rb - 1) // rb, block=kern.block))
             i += 3
         return out
 
@@ -609,7 +610,7 @@ class ModelPlan:
         ptrs = self._ptrs(["@fm", "@trig"] + op.stores + op.ins + (["@prm"] if ens else []))
         pvals = self._params.ctypes.data_as(C.POINTER(C.c_float))
         _lib.check(_lib.load().tsphere_kernel_launch_grid(
-            op.handle, op.blocks, self.runs, 256, len(ptrs), ptrs,
+            op.handle, op.blocks, self.runs, op.block, len(ptrs), ptrs,
             0 if ens else len(self.param_names), pvals, stream.handle), op.label)
 
     def run(self, stream: Stream, steps: int = 1, graph_steps: int = 1) -> None:
