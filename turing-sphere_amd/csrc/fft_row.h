@@ -18,7 +18,8 @@
 //   analysis   z[m] = x[2m] + i x[2m+1], Z = FFT_H^{-}(z),
 //              X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2pi i k/N} (-i)(Z[k] - conj Z[H-k])/2
 // with X[0] = Re F_0, X[m] = F_m (m <= mmax), 0 above.  Twiddles come from the
-// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N).
+// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N); those of the
+// radix-4 passes are staged per workgroup in LDS, in per-pass tables.
 #pragma once
 
 namespace tsfft {
@@ -36,10 +37,16 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) {
   return make_float2(a.x - b.x, a.y - b.y);
 }
 
-// Twiddle e^{SIGN 2 pi i idx / N} from the (cos, sin) table.
+// Twiddle e^{SIGN 2 pi i idx / N} from the plan's (cos, sin) table.
 template <int SIGN>
 __device__ __forceinline__ float2 tw(const float2* __restrict__ trig, int idx) {
   const float2 t = trig[idx];
+  return SIGN > 0 ? t : make_float2(t.x, -t.y);
+}
+
+// e^{SIGN i phi} from a staged e^{+i phi}.
+template <int SIGN>
+__device__ __forceinline__ float2 twc(float2 t) {
   return SIGN > 0 ? t : make_float2(t.x, -t.y);
 }
 
@@ -56,26 +63,32 @@ struct FftShape {
   static constexpr int BPT = Q / TPR;
   static constexpr int RB = 256 / TPR;
   static constexpr bool ODD = LOG2H & 1;
+  static constexpr int NS0 = ODD ? 2 : 4;  // Ns of the first pass after the first
+  static constexpr int NPASS = (LOG2H - (ODD ? 1 : 2)) / 2;
+  // Pass twiddle tables, k-major per pass: pass Ns holds W[k - 1][jm] =
+  // e^{+2 pi i k jm / (4 Ns)}, k = 1..3, jm < Ns, at offset Ns - NS0.
+  static constexpr int NTW = NPASS > 0 ? NS0 * ((1 << (2 * NPASS)) - 1) : 1;
 };
 
 // One radix-4 Stockham pass, natural order, from src to dst (both the
 // thread's row in LDS).  Butterfly j reads src[j + k Q], k = 0..3, and writes
 // dst[4(j - jm) + jm + k Ns], jm = j mod Ns, with twiddles
-// e^{SIGN 2 pi i jm k / (4 Ns)} = trig[jm k N / (4 Ns)].
+// e^{SIGN 2 pi i jm k / (4 Ns)} = trig[jm k N / (4 Ns)], read from the pass's
+// staged table W (lanes of consecutive jm read consecutive entries, so the
+// reads do not conflict in LDS banks, as strided reads of one table would).
 template <int SIGN, int LOG2H>
-__device__ __forceinline__ void r4_pass(const float2* src, float2* dst,
-                                        const float2* __restrict__ trig, int t, int Ns) {
+__device__ __forceinline__ void r4_pass(const float2* src, float2* dst, const float2* W, int t,
+                                        int Ns) {
 #pragma clang fp contract(off)
   using S = FftShape<LOG2H>;
-  const int step = S::N / (4 * Ns);
 #pragma unroll
   for (int b = 0; b < S::BPT; ++b) {
     const int j = t + S::TPR * b;
     const int jm = j & (Ns - 1);
     const float2 x0 = src[j];
-    const float2 x1 = cmul(src[j + S::Q], tw<SIGN>(trig, jm * step));
-    const float2 x2 = cmul(src[j + 2 * S::Q], tw<SIGN>(trig, 2 * jm * step));
-    const float2 x3 = cmul(src[j + 3 * S::Q], tw<SIGN>(trig, 3 * jm * step));
+    const float2 x1 = cmul(src[j + S::Q], twc<SIGN>(W[jm]));
+    const float2 x2 = cmul(src[j + 2 * S::Q], twc<SIGN>(W[Ns + jm]));
+    const float2 x3 = cmul(src[j + 3 * S::Q], twc<SIGN>(W[2 * Ns + jm]));
     const float2 a = cadd(x0, x2), bb = csub(x0, x2);
     const float2 c = cadd(x1, x3), e = csub(x1, x3);
     const float2 d = SIGN > 0 ? make_float2(-e.y, e.x) : make_float2(e.y, -e.x);
@@ -123,13 +136,12 @@ __device__ __forceinline__ void first_pass(In in, float2* dst, int t) {
 // pass: a pass only writes the buffer the previous pass read).  Returns the
 // buffer that holds the result, after a final barrier.
 template <int SIGN, int LOG2H>
-__device__ __forceinline__ float2* rest_passes(float2* a, float2* b,
-                                               const float2* __restrict__ trig, int t) {
+__device__ __forceinline__ float2* rest_passes(float2* a, float2* b, const float2* W, int t) {
   using S = FftShape<LOG2H>;
 #pragma unroll
-  for (int Ns = S::ODD ? 2 : 4; Ns < S::H; Ns <<= 2) {
+  for (int Ns = S::NS0; Ns < S::H; Ns <<= 2) {
     __syncthreads();
-    r4_pass<SIGN, LOG2H>(a, b, trig, t, Ns);
+    r4_pass<SIGN, LOG2H>(a, b, W + (Ns - S::NS0), t, Ns);
     float2* tmp = a;
     a = b;
     b = tmp;
@@ -144,12 +156,12 @@ __device__ __forceinline__ float2* rest_passes(float2* a, float2* b,
 // workgroup meets the same barriers.  Same result buffer as rest_passes.
 template <int SIGN, int LOG2H>
 __device__ __forceinline__ float2* rest_passes_masked(bool active, float2* a, float2* b,
-                                                      const float2* __restrict__ trig, int t) {
+                                                      const float2* W, int t) {
   using S = FftShape<LOG2H>;
 #pragma unroll
-  for (int Ns = S::ODD ? 2 : 4; Ns < S::H; Ns <<= 2) {
+  for (int Ns = S::NS0; Ns < S::H; Ns <<= 2) {
     __syncthreads();
-    if (active) r4_pass<SIGN, LOG2H>(a, b, trig, t, Ns);
+    if (active) r4_pass<SIGN, LOG2H>(a, b, W + (Ns - S::NS0), t, Ns);
     float2* tmp = a;
     a = b;
     b = tmp;
@@ -190,19 +202,19 @@ struct SynthInput {
 // t + TPR b, b < 4 BPT, after the final barrier).
 template <int LOG2H>
 __device__ __forceinline__ float2* synth_row(const float2* __restrict__ F, int mmax, float mscale,
-                                            const float2* __restrict__ trig, float2* a, float2* b,
-                                            int t) {
+                                            const float2* __restrict__ trig, const float2* W,
+                                            float2* a, float2* b, int t) {
   const SynthInput<LOG2H> in{F, trig, mmax, mscale};
   first_pass<+1, LOG2H>(in, a, t);
-  return rest_passes<+1, LOG2H>(a, b, trig, t);
+  return rest_passes<+1, LOG2H>(a, b, W, t);
 }
 
 // One row's analysis FFT from in(j) = x[2j] + i x[2j+1]; returns z.
 template <int LOG2H, class In>
-__device__ __forceinline__ const float2* analys_row(In in, const float2* __restrict__ trig,
-                                                   float2* a, float2* b, int t) {
+__device__ __forceinline__ const float2* analys_row(In in, const float2* W, float2* a, float2* b,
+                                                   int t) {
   first_pass<-1, LOG2H>(in, a, t);
-  return rest_passes<-1, LOG2H>(a, b, trig, t);
+  return rest_passes<-1, LOG2H>(a, b, W, t);
 }
 
 // Analysis post-processing, fourier.ts:352-386: X[k] from z, 0 <= k < H.
@@ -219,6 +231,23 @@ __device__ __forceinline__ float2 analys_coef(const float2* z, int k,
   const float2 xo = make_float2(d.y, -d.x);
   const float2 w = make_float2(trig[k].x, -trig[k].y);
   return cadd(xe, cmul(xo, w));
+}
+
+// Stage the pass twiddle tables (FftShape::NTW entries) from the plan's
+// (cos, sin) table into the workgroup's LDS table W, then a barrier.  The
+// passes read their twiddles from W: a table read from global memory inside
+// each pass put one L2 round trip between every pair of barriers.
+template <int LOG2H>
+__device__ __forceinline__ void stage_twiddles(const float2* __restrict__ trig, float2* W) {
+  using S = FftShape<LOG2H>;
+  for (int i = threadIdx.x; i < S::NTW; i += blockDim.x) {
+    int Ns = S::NS0;
+    while (i >= 4 * Ns - S::NS0) Ns <<= 2;  // pass of entry i: offsets Ns - NS0
+    const int loc = i - (Ns - S::NS0);
+    const int k = loc / Ns + 1, jm = loc - (k - 1) * Ns;
+    W[i] = trig[k * jm * (S::N / (4 * Ns))];
+  }
+  __syncthreads();
 }
 
 }  // namespace tsfft

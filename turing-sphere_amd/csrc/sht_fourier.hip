@@ -49,6 +49,8 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat, int tiles
                                                    Fields Fs, FourierScale sc) {
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
+  __shared__ float2 W[S::NTW];
+  tsfft::stage_twiddles<LOG2H>(trig, W);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat, int tiles
   const float2* F = fm + ((long long)f * nlat + (row ? ilat : nlat - 1)) * (mmax + 1);
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
-  const float2* res = tsfft::synth_row<LOG2H>(F, mmax, sc.mscale, trig, a, b, t);
+  const float2* res = tsfft::synth_row<LOG2H>(F, mmax, sc.mscale, trig, W, a, b, t);
   if (row) {
     float2* out = reinterpret_cast<float2*>(const_cast<char*>(Fs.at(f))) + (long long)ilat * S::H;
     const float w = sc.rowscale ? sc.rowscale[ilat] : 1.f;
@@ -77,6 +79,8 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
                                                     FourierScale sc) {
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
+  __shared__ float2 W[S::NTW];
+  tsfft::stage_twiddles<LOG2H>(trig, W);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
       reinterpret_cast<const float2*>(Fs.at(f)) + (long long)(row ? ilat : nlat - 1) * S::H;
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
-  const float2* z = tsfft::analys_row<LOG2H>([&](int j) { return in[j]; }, trig, a, b, t);
+  const float2* z = tsfft::analys_row<LOG2H>([&](int j) { return in[j]; }, W, a, b, t);
   if (row) {
     float2* G = fm + ((long long)f * nlat + ilat) * (mmax + 1);
     for (int k = t; k <= mmax; k += S::TPR) {
