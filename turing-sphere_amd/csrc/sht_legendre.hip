@@ -69,6 +69,17 @@ __device__ __forceinline__ int wave_min(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Last seed position among the lanes of a chain (-1 if no lane is ever
+// significant): past it no lane of the chain takes a seed, and the loops run
+// without the per-lane seed test.
+__device__ __forceinline__ int last_seed(int ls, int L) { return wave_max(ls <= L ? ls : -1); }
+
 // Latitude pair of this lane's chain k (of K).  A chain is a chunk of 64
 // consecutive pairs, so the lanes of one chain span a narrow band of
 // latitudes whose significance points are close and the chain can start at
@@ -195,17 +206,20 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
 }
 
 // ------------------------------------------------------------- synthesis
-// kTile l-values of chains [K0, K) from l = m + i.
-template <int K0, int NF, int K, class P>
+// kTile l-values of chains [K0, K) from l = m + i (INJ: lanes may take
+// their seed at l).
+template <int K0, bool INJ, int NF, int K, class P>
 __device__ __forceinline__ void synth_tile(const typename P::ab_t* sab, const float2* sq, int i,
                                            int l, ChainT<P> (&c)[K], float2 (&accE)[K][NF],
                                            float2 (&accO)[K][NF]) {
+  if constexpr (INJ) {
 #pragma unroll
-  for (int k = K0; k < K; ++k)
-    if (c[k].ls == l) {
-      c[k].y0 = c[k].s0;
-      c[k].y1 = c[k].s1;
-    }
+    for (int k = K0; k < K; ++k)
+      if (c[k].ls == l) {
+        c[k].y0 = c[k].s0;
+        c[k].y1 = c[k].s1;
+      }
+  }
 #pragma unroll
   for (int p = 0; p < kTile; p += 2) {
     float2 q0[NF], q1[NF];
@@ -231,16 +245,23 @@ __device__ __forceinline__ void synth_tile(const typename P::ab_t* sab, const fl
 // Tiles from l on, in phases: phase K0 runs chains [K0, K) until a chain
 // below K0 becomes live (lk[k] <= l), then hands over to phase K0 - 1.  Each
 // phase is a branch-free loop; a chain run before it is live only adds zeros.
+// Within a phase, tiles past the last seed of its chains (lz) skip the seed
+// test.
 template <int K0, int NF, int K, class P>
 __device__ __forceinline__ void synth_phases(int l, int L, int m, const int (&lk)[K],
-                                             const typename P::ab_t* sab, const float2* sq,
-                                             ChainT<P> (&c)[K], float2 (&accE)[K][NF],
-                                             float2 (&accO)[K][NF]) {
-  int lend = L + 1;
+                                             const int (&lz)[K], const typename P::ab_t* sab,
+                                             const float2* sq, ChainT<P> (&c)[K],
+                                             float2 (&accE)[K][NF], float2 (&accO)[K][NF]) {
+  int lend = L + 1, injend = 0;
 #pragma unroll
   for (int k = 0; k < K0; ++k) lend = min(lend, lk[k]);
-  for (; l < lend; l += kTile) synth_tile<K0, NF, K, P>(sab, sq, l - m, l, c, accE, accO);
-  if constexpr (K0 > 0) synth_phases<K0 - 1, NF, K, P>(l, L, m, lk, sab, sq, c, accE, accO);
+#pragma unroll
+  for (int k = K0; k < K; ++k) injend = max(injend, lz[k] + 1);
+  const int l1 = min(lend, injend);
+  for (; l < l1; l += kTile) synth_tile<K0, true, NF, K, P>(sab, sq, l - m, l, c, accE, accO);
+  for (; l < lend; l += kTile) synth_tile<K0, false, NF, K, P>(sab, sq, l - m, l, c, accE, accO);
+  if constexpr (K0 > 0)
+    synth_phases<K0 - 1, NF, K, P>(l, L, m, lk, lz, sab, sq, c, accE, accO);
 }
 
 template <int NF, int K, class P>
@@ -260,12 +281,13 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
     __syncthreads();
 
     ChainT<P> c[K];
-    int lk[K];  // first l where some lane of chain k is significant
+    int lk[K], lz[K];  // first and last l where a lane of chain k takes its seed
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       load_chain_t<P>(T, m, pair_of(k), c[k]);
       lk[k] = wave_min(c[k].ls);
+      lz[k] = last_seed(c[k].ls, L);
       lmin = min(lmin, lk[k]);
     }
 
@@ -275,7 +297,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(typename P::Tables T, int nfi
 #pragma unroll
       for (int f = 0; f < NF; ++f) accE[k][f] = accO[k][f] = make_float2(0.f, 0.f);
 
-    if (lmin <= L) synth_phases<K - 1, NF, K, P>(lmin, L, m, lk, sab, sq, c, accE, accO);
+    if (lmin <= L) synth_phases<K - 1, NF, K, P>(lmin, L, m, lk, lz, sab, sq, c, accE, accO);
 
     const int nlat = T.nlat;
 #pragma unroll
@@ -337,7 +359,7 @@ __device__ __forceinline__ float transpose_reduce64(float (&v)[64]) {
 // -------------------------------------------------------------- analysis
 // One span of SPAN l-values from ls for chains [K0, K): partial sums acc
 // (zeroed here), laid out (l, field, re/im).
-template <int K0, int NF, int K, class P>
+template <int K0, bool INJ, int NF, int K, class P>
 __device__ __forceinline__ void analys_span(const typename P::ab_t* sab, int ls, int m,
                                             ChainT<P> (&c)[K], const float2 (&wp)[K][NF],
                                             const float2 (&wm)[K][NF], float (&acc)[64]) {
@@ -347,7 +369,7 @@ __device__ __forceinline__ void analys_span(const typename P::ab_t* sab, int ls,
 #pragma unroll
   for (int p = 0; p < SPAN; p += 2) {
     const int l = ls + p;
-    if (p % kTile == 0) {
+    if (INJ && p % kTile == 0) {
 #pragma unroll
       for (int k = K0; k < K; ++k)
         if (c[k].ls == l) {
@@ -376,18 +398,18 @@ __device__ __forceinline__ void analys_span(const typename P::ab_t* sab, int ls,
   }
 }
 
-template <int K0, int NF, int K, class P>
+template <int K0, bool INJ, int NF, int K, class P>
 __device__ __forceinline__ void analys_span_from(int kf, const typename P::ab_t* sab, int ls,
                                                  int m, ChainT<P> (&c)[K],
                                                  const float2 (&wp)[K][NF],
                                                  const float2 (&wm)[K][NF], float (&acc)[64]) {
   if constexpr (K0 + 1 >= K) {
-    analys_span<K0, NF, K, P>(sab, ls, m, c, wp, wm, acc);
+    analys_span<K0, INJ, NF, K, P>(sab, ls, m, c, wp, wm, acc);
   } else {
     if (kf <= K0)
-      analys_span<K0, NF, K, P>(sab, ls, m, c, wp, wm, acc);
+      analys_span<K0, INJ, NF, K, P>(sab, ls, m, c, wp, wm, acc);
     else
-      analys_span_from<K0 + 1, NF, K, P>(kf, sab, ls, m, c, wp, wm, acc);
+      analys_span_from<K0 + 1, INJ, NF, K, P>(kf, sab, ls, m, c, wp, wm, acc);
   }
 }
 
@@ -416,12 +438,14 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
     ChainT<P> c[K];
     float2 wp[K][NF], wm[K][NF];
     int lk[K];  // first l where some lane of chain k is significant
+    int lzmax = -1;  // last l where a lane of this wave takes its seed
     int lmin = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = pair_of(k);
       load_chain_t<P>(T, m, j, c[k]);
       lk[k] = wave_min(c[k].ls);
+      lzmax = max(lzmax, last_seed(c[k].ls, L));
       lmin = min(lmin, lk[k]);
       const int jc = min(j, T.npair - 1);
       const int js = nlat - 1 - jc;
@@ -464,8 +488,11 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
         red[buf][wv][lane] = 0.f;
       } else {
         float acc[64];
-        analys_span_from<0, NF, K, P>(first_live<K>(lk, ls + SPAN - 1), sab, ls, m, c, wp, wm,
-                                      acc);
+        const int kf = first_live<K>(lk, ls + SPAN - 1);
+        if (ls <= lzmax)  // a lane may still take its seed in this span
+          analys_span_from<0, true, NF, K, P>(kf, sab, ls, m, c, wp, wm, acc);
+        else
+          analys_span_from<0, false, NF, K, P>(kf, sab, ls, m, c, wp, wm, acc);
         red[buf][wv][lane] = transpose_reduce64(acc);
       }
       __syncthreads();
