@@ -18,6 +18,11 @@ Trajectories are checkpointed: the forward pass keeps the state every
 ``checkpoint_every`` steps; the backward pass re-runs each segment from its
 checkpoint, keeping the segment's step inputs, and differentiates its steps
 in reverse, re-running each step once more to regenerate its intermediates.
+Steps that only advance the state run through the forward plan (fused, and
+grid-fused where it applies; bit-identical to the tape's step), the initial
+pass as HIP-graph replays; each position j of a segment has one captured
+graph for "keep input j, advance" and one for "regenerate step j,
+differentiate it", so a gradient is a short sequence of graph launches.
 """
 from __future__ import annotations
 
@@ -263,6 +268,12 @@ class AdjointModel:
         tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape",
                                 fuse=False)
         self.tape = tape
+        # state <- step(state) only: the fused plan, writing the state in place
+        self.fwd = ModelPlan.create(sht, PlanSpec(fn, list(self.state) +
+                                                  [None] * (len(fn.outputs) - len(self.state))),
+                                    host, True, "forward")
+        self._graphs = {}
+        self._seg = self._carry = None
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         # state outputs: the value each state variable takes after the step
         self.out_of = {}
@@ -318,7 +329,30 @@ class AdjointModel:
     # -------------------------------------------------------------- utils
     def set_params(self, values: dict) -> None:
         self.tape.set_params(values)
+        self.fwd.set_params(values)
         self._params[:len(self.param_slots)] = self.tape._params[:len(self.param_slots)]
+        self._drop_graphs()  # their launches hold the old scalars
+
+    def _drop_graphs(self) -> None:
+        for g in self._graphs.values():
+            _lib.call("tsphere_graph_destroy", g)
+        self._graphs = {}
+
+    def _replay(self, key, record) -> None:
+        """Launch the graph captured under `key`, capturing `record()` on
+        first use (every pointer it touches is fixed until the graphs are
+        dropped)."""
+        g = self._graphs.get(key)
+        if g is None:
+            _lib.call("tsphere_graph_begin", self.stream.handle)
+            try:
+                record()
+            finally:
+                h = C.c_void_p()
+                rc = _lib.load().tsphere_graph_end(self.stream.handle, C.byref(h))
+            _lib.check(rc, "graph capture")
+            g = self._graphs[key] = h.value
+        _lib.call("tsphere_graph_launch", g, self.stream.handle)
 
     def _state_buf(self, s):
         return self._host.get(s)
@@ -343,23 +377,12 @@ class AdjointModel:
     def _step_once(self) -> None:
         self.tape.encode_steps(self.stream, 1)
 
-    def _advance(self) -> None:
-        """One step: state <- step(state)."""
-        self._step_once()
-        L = _lib.load()
-        n = 2 * self.nlm
-        for s in self.state:
-            _lib.check(L.tsphere_memcpy_dtod(self._state_buf(s).ptr,
-                                             self.tape._bufs[self.out_of[s]].ptr, 4 * n,
-                                             self.stream.handle), "advance")
-
     # ------------------------------------------------------------ forward
     def forward(self, state0: dict, steps: int) -> dict:
         """Run `steps` steps from state0 (host arrays); returns the final state."""
         for s in self.state:
             self._state_buf(s).upload(np.asarray(state0[s], np.float32), self.stream)
-        for _ in range(steps):
-            self._advance()
+        self.fwd.run(self.stream, steps, max(1, min(steps, self.checkpoint_every)))
         return {s: self._state_buf(s).download(self.stream) for s in self.state}
 
     # ------------------------------------------------------------ backward
@@ -406,15 +429,22 @@ class AdjointModel:
         C_ = self.checkpoint_every
         nseg = max(1, math.ceil(steps / C_))
         chk = DeviceBuffer(nseg * n1, "checkpoints")
-        seg = DeviceBuffer(C_ * n1, "segment-inputs")
-        carry = DeviceBuffer(n1, "state-cotangent")
+        if self._seg is None or self._seg.count < C_ * n1:
+            # the per-position graphs hold the segment and carry buffers: keep
+            # them for the model's life, re-capturing only when they grow
+            self._drop_graphs()
+            if self._seg is not None:
+                self._seg.free()
+            self._seg = DeviceBuffer(C_ * n1, "segment-inputs")
+        if self._carry is None:
+            self._carry = DeviceBuffer(n1, "state-cotangent")
+        seg, carry = self._seg, self._carry
         try:
             for s in self.state:
                 self._state_buf(s).upload(np.asarray(state0[s], np.float32), self.stream)
-            for k in range(steps):
-                if k % C_ == 0:
-                    self._store_state(chk, (k // C_) * n1, from_outputs=False)
-                self._advance()
+            for g in range(nseg):
+                self._store_state(chk, g * n1, from_outputs=False)
+                self.fwd.run(self.stream, min(C_, steps - g * C_), C_)
             carry.upload(np.concatenate([np.asarray(cot[s], np.float32).reshape(-1)
                                          for s in self.state]), self.stream)
             for a in self._adj:
@@ -422,27 +452,32 @@ class AdjointModel:
                     a[4].fill(0.0, self.stream)
             L = _lib.load()
             nq = 2 * self.nlm
+            def keep_and_advance(j):
+                self._store_state(seg, j * n1, from_outputs=False)
+                self.fwd.encode_steps(self.stream, 1)
+
+            def differentiate(j):
+                self._load_state(seg, j * n1)
+                self._step_once()          # regenerate this step's intermediates
+                self._zero_bars()
+                for i, s in enumerate(self.state):
+                    _lib.check(L.tsphere_memcpy_dtod(
+                        self._bar[self.out_of[s]].ptr, _off(carry, i * nq), 4 * nq,
+                        self.stream.handle), "cotangent in")
+                self._backward_step()
+                for i, s in enumerate(self.state):
+                    _lib.check(L.tsphere_memcpy_dtod(
+                        _off(carry, i * nq), self._bar[self.in_of[s]].ptr, 4 * nq,
+                        self.stream.handle), "cotangent out")
+
             for g in reversed(range(nseg)):
-                k0 = g * C_
-                k1 = min(steps, k0 + C_)
+                n = min(steps, g * C_ + C_) - g * C_
                 self._load_state(chk, g * n1)
-                for j in range(k1 - k0):
-                    self._store_state(seg, j * n1, from_outputs=False)
-                    if j < k1 - k0 - 1:
-                        self._advance()
-                for j in reversed(range(k1 - k0)):
-                    self._load_state(seg, j * n1)
-                    self._step_once()          # regenerate this step's intermediates
-                    self._zero_bars()
-                    for i, s in enumerate(self.state):
-                        _lib.check(L.tsphere_memcpy_dtod(
-                            self._bar[self.out_of[s]].ptr, _off(carry, i * nq), 4 * nq,
-                            self.stream.handle), "cotangent in")
-                    self._backward_step()
-                    for i, s in enumerate(self.state):
-                        _lib.check(L.tsphere_memcpy_dtod(
-                            _off(carry, i * nq), self._bar[self.in_of[s]].ptr, 4 * nq,
-                            self.stream.handle), "cotangent out")
+                for j in range(n - 1):
+                    self._replay(("keep", C_, j), lambda j=j: keep_and_advance(j))
+                self._store_state(seg, (n - 1) * n1, from_outputs=False)
+                for j in reversed(range(n)):
+                    self._replay(("diff", C_, j), lambda j=j: differentiate(j))
             out = carry.download(self.stream)
             grads = {s: out[i * nq:(i + 1) * nq].copy() for i, s in enumerate(self.state)}
             pg = np.zeros(len(self.param_slots))
@@ -454,8 +489,6 @@ class AdjointModel:
         finally:
             self.stream.sync()
             chk.free()
-            seg.free()
-            carry.free()
 
     # ------------------------------------------------------------ helpers
     def synth_adjoint(self, gbar) -> np.ndarray:
@@ -473,6 +506,12 @@ class AdjointModel:
 
     def destroy(self) -> None:
         self.stream.sync()
+        self._drop_graphs()
+        self.fwd.destroy()
+        for b in (self._seg, self._carry):
+            if b is not None:
+                b.free()
+        self._seg = self._carry = None
         for h in self._kernels:
             _lib.call("tsphere_kernel_destroy", h)
         self._kernels = []
