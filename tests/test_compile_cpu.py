@@ -341,3 +341,30 @@ def test_grid_fused_ensemble_plan():
     src = step.kernel_source()
     assert "const int run = blockIdx.y;" in src and "prm[run * 5 + " in src
     assert "(g * R + run)" in src and "R = 4" in src
+
+
+def test_grid_fused_plan_external_input():
+    """A grid op that reads a host grid buffer (noise) inside its statements
+    and analyses a synthesised value directly compiles, with the buffer as
+    a kernel input."""
+    import types
+
+    import tsphere
+    from tsphere.mfront import CompiledModel, model_bindings
+    from tsphere.plan import GridOp, HostBuffers, ModelPlan, PlanSpec
+
+    forced = open(os.path.join(ROOT, "tests", "models", "forced.m")).read()
+    nlm, npts = 528, 33 * 64
+    cm = CompiledModel(forced, model_bindings(["U"], ["c", "D", "dt"], npts, nlm),
+                       {"npts": npts, "nlm": nlm})
+    step_fn = cm.specialize("step", 2)
+    cm.finish()
+    stub = types.SimpleNamespace(cfg=tsphere.ShtConfig(31, 31, 33, 64), fourier_mode="fft",
+                                 max_fields=2)
+    host = HostBuffers(allocate=False)
+    host.ensure_group(["U"], 2 * nlm)
+    host.ensure("noise", npts)
+    step = ModelPlan.create(stub, PlanSpec(step_fn, ["U", None]), host, True, "step")
+    g = step.ops[0]
+    assert isinstance(g, GridOp) and len(g.ins) == 1 and len(g.dsts) == 2
+    assert "const float* gin0" in step.kernel_source()

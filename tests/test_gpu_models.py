@@ -285,3 +285,32 @@ def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nl
         d = float(np.max(np.abs(a - b)) / max(1e-30, float(np.max(np.abs(a)))))
         emit(f"  max rel diff {d:.2e}")
         assert np.array_equal(a, b), d
+
+
+FORCED_SRC = open(os.path.join(ROOT, "tests", "models", "forced.m")).read()
+
+
+@pytest.mark.parametrize("groups", [True, False])
+def test_grid_fusion_external_grid_input(gpu, monkeypatch, groups):
+    """A grid op whose statements read a grid value from a host buffer (the
+    noise field) and which analyses a synthesised value directly, next to a
+    produced one, matches the separate stages bit for bit."""
+    import tsphere
+    import tsphere.hipgen
+    from tsphere.model import GpuModel
+
+    monkeypatch.setattr(tsphere.hipgen, "GRID_GROUPS", groups)
+    cfg = tsphere.ShtConfig(31, 31, 33, 64)
+    outs = []
+    for grid in (False, True):
+        sht = tsphere.ShtPlan.create(cfg, max_fields=2)
+        gm = GpuModel.create(sht, cfg, FORCED_SRC, ["c", "D", "dt"], ["U"], ["u"], grid=grid)
+        assert ("grid" in [l.split()[0] for l in gm.describe()["step"]]) == grid
+        gm.set_params({"c": 0.3, "D": 1e-3, "dt": 0.05})
+        gm.init((0.5 + 0.1 * O.randn(9, cfg.npts)).astype(np.float32))
+        gm.step(15)
+        outs.append([gm.read("U"), gm.read("u")])
+        gm.destroy()
+        sht.destroy()
+    for a, b in zip(*outs):
+        assert np.all(np.isfinite(b)) and np.array_equal(a, b)
