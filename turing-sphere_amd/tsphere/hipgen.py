@@ -604,13 +604,13 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
         ap(f"    for (int i = 0; i < NP; ++i) sv{si}[i] = res[t + S::TPR * i];")
         ap("    __syncthreads();  // the next row's first pass overwrites the ping-pong rows")
         ap("  }")
+    def goff(cn):
+        return "g0 + c" if cn in shared else "grun + g0 + c"
+
     # element-wise over this thread's points (complex index c: reals 2c, 2c+1)
     ap("#pragma unroll")
     ap("  for (int i = 0; i < NP; ++i) {")
     ap("    const int c = t + S::TPR * i;")
-    def goff(cn):
-        return "g0 + c" if cn in shared else "grun + g0 + c"
-
     for k, cn in enumerate(gins):
         ap(f"    const float2 gv{k} = reinterpret_cast<const float2*>(gin{k})[{goff(cn)}];")
 
