@@ -253,13 +253,17 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  * The Legendre stage's workgroup shape (no reference counterpart: WebGPU's
  * workgroup sizes are fixed in sht.ts:70-89).  Stage 0 = synthesis,
  * 1 = analysis; fields left 0 use the built-in choice.
- *   FP32 form:  impl 1 = contraction on the VALU, 2 = on the FP32 matrix cores
- *               (v_mfma_f32_16x16x4_f32; the recurrence stays on the VALU);
- *               nf fields per workgroup (1, 2, 4), nwaves waves (1..4),
+ *   FP32 and FP64-recurrence forms:  impl 1 = contraction on the VALU,
+ *               2 = on the FP32 matrix cores as 16x16x4 tiles
+ *               (v_mfma_f32_16x16x4_f32), 3 = on the FP32 matrix cores as 16
+ *               4x4 blocks per wave (v_mfma_f32_4x4x1_16b_f32: two fields'
+ *               re/im fill the 4 columns); the recurrence stays on the VALU;
+ *               nf fields per workgroup (1, 2, 4), nwaves waves (1..4;
+ *               1..8 for impl 3),
  *               k latitude pairs per lane (1, 2, 4, 8; matrix cores 1, 2, 4),
- *               64*nwaves*k >= pairs; rows = m-rows per workgroup on the VALU
- *               (2: the pair (m, mmax - m), balanced; 1: one row, twice the
- *               workgroups).
+ *               64*nwaves*k >= pairs; rows = m-rows per workgroup for impl 1
+ *               and 3 (2: the pair (m, mmax - m), balanced; 1: one row, twice
+ *               the workgroups).
  *   MFMA form:  ctc 16-column tiles per workgroup (1, 2; 4 for synthesis). */
 typedef struct {
   int nf, nwaves, k, ctc, impl, rows;

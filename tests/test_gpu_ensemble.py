@@ -188,7 +188,7 @@ def test_tile_shapes_and_tuning(gpu, tmp_path, monkeypatch):
     with pytest.raises(tsphere.TsphereError):
         plan.set_tile("analys", k=3)
     with pytest.raises(tsphere.TsphereError):
-        plan.set_tile("synth", nwaves=5)
+        plan.set_tile("synth", nwaves=9)
     with pytest.raises(tsphere.TsphereError):
         plan.set_tile("synth", rows=3)
     res = plan.tune(nf, reps=3)

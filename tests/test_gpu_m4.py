@@ -1,0 +1,88 @@
+"""FP32 Legendre contraction on 4x4 matrix blocks (impl 3:
+v_mfma_f32_4x4x1_16b_f32, the recurrence on the VALU) in the FP32 and the
+FP64-recurrence forms, against the VALU contraction (impl 1) of the same form
+and the f64 oracle.  Same arithmetic class, different summation order: the
+two agree to 2e-6 relative, and impl 3 meets each form's tolerance against
+f64 (FP32: the stage tolerance of tests/test_gpu_sht.py; FP64 recurrence:
+1e-5 per transform, 2e-5 per round trip)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import emit
+from test_gpu_sht import _random_field_spectrum
+
+pytestmark = pytest.mark.gpu
+
+
+def leg_tol(lmax, legendre):
+    return 1e-5 if legendre == "f64rec" else max(3e-6, 2e-10 * (lmax + 1) ** 2)
+
+
+_SMALL = [(15, 16, 32), (15, 17, 32), (12, 13, 26), (63, 64, 128), (127, 128, 256),
+          (255, 256, 512)]
+_CASES = ([g + (nf, "fp32") for g in _SMALL for nf in (1, 2, 3, 5)]
+          + [g + (nf, "f64rec") for g in _SMALL[:4] for nf in (1, 2, 5)]
+          + [(511, 512, 1024, 2, "fp32"), (1023, 1024, 2048, 2, "fp32"),
+             (1023, 1024, 2048, 2, "f64rec")])
+
+
+def _plan_and_fields(lmax, nlat, nphi, nf, legendre, seed):
+    import tsphere
+
+    cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
+    plan = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=nf)
+    qs = np.concatenate([_random_field_spectrum(O.Cfg(lmax, lmax, nlat, nphi), seed + f)
+                         for f in range(nf)])
+    q = tsphere.DeviceBuffer(nf * 2 * plan.nlm)
+    sp = tsphere.DeviceBuffer(nf * cfg.npts)
+    back = tsphere.DeviceBuffer(nf * 2 * plan.nlm)
+    q.upload(qs, plan.stream)
+
+    def run(**tile):
+        for st in ("synth", "analys"):
+            plan.set_tile(st, **tile)
+        plan.encode_synth_into(plan.stream, plan.create_synth_binding(q, sp, nf, 2 * plan.nlm,
+                                                                      cfg.npts))
+        plan.encode_analys_into(plan.stream, plan.create_analys_binding(sp, back, nf, cfg.npts,
+                                                                        2 * plan.nlm))
+        return (sp.download(plan.stream).reshape(nf, -1),
+                back.download(plan.stream).reshape(nf, -1))
+
+    return plan, qs, run
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi,nf,legendre", _CASES)
+def test_block_matrix_stages(gpu, lmax, nlat, nphi, nf, legendre):
+    emit(f"m4 {legendre} lmax{lmax} {nlat}x{nphi} nf {nf}")
+    plan, qs, run = _plan_and_fields(lmax, nlat, nphi, nf, legendre, 40)
+    ref = O.ShtReference(O.Cfg(lmax, lmax, nlat, nphi))
+    s1, b1 = run(impl=1)
+    s3, b3 = run(impl=3)
+    for f in range(nf):
+        es = O.rel_l2(s3[f], s1[f])
+        eb = O.rel_l2(b3[f], b1[f])
+        qf = qs[f * 2 * plan.nlm:(f + 1) * 2 * plan.nlm]
+        e64 = O.rel_l2(s3[f], ref.synth_fast(qf.astype(np.float64)))
+        a64 = O.rel_l2(b3[f], ref.analys_fast(s3[f].astype(np.float64)))
+        if f in (0, nf - 1):
+            emit(f"  field {f}: vs VALU synth {es:.1e} analys {eb:.1e}; vs f64 synth "
+                 f"{e64:.1e} analys {a64:.1e}")
+        assert es < 2e-6 and eb < 2e-6
+        assert e64 < leg_tol(lmax, legendre) and a64 < leg_tol(lmax, legendre)
+    plan.destroy()
+
+
+@pytest.mark.parametrize("nwaves,k,rows", [(1, 4, 2), (2, 2, 1), (4, 1, 2), (2, 4, 1),
+                                           (4, 4, 2), (4, 2, 1), (8, 1, 1), (8, 2, 2)])
+@pytest.mark.parametrize("nfb", [1, 2, 4])
+def test_block_matrix_tiles(gpu, nwaves, k, rows, nfb):
+    """Every block-matrix workgroup shape gives the VALU result up to
+    summation order (three fields, so a four-field workgroup has an idle
+    field and a one-field workgroup idle columns)."""
+    emit(f"m4 tile nwaves {nwaves} k {k} rows {rows} nf {nfb}")
+    plan, qs, run = _plan_and_fields(255, 256, 512, 3, "fp32", 7)
+    s1, b1 = run(impl=1)
+    s3, b3 = run(impl=3, nwaves=nwaves, k=k, rows=rows, nf=nfb)
+    assert O.rel_l2(s3, s1) < 2e-6 and O.rel_l2(b3, b1) < 2e-6
+    plan.destroy()

@@ -20,17 +20,21 @@ SOURCES = ["tsphere_runtime.hip", "sht_legendre.hip", "sht_legendre_mfma.hip",
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs,
-           "-o", tmp, "-lhiprtc"]
+# Per-source additions.  The Legendre kernels keep their matrix-core
+# accumulators in VGPRs: with the default AGPR form the compiler rotated the
+# 4x4-block accumulators through accvgpr moves inside the l-loop (24 moves per
+# 32 VALU ops of the recurrence).
+EXTRA = {"sht_legendre.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+OBJDIR = os.path.join(HERE, "_build")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -41,6 +45,24 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
     if verbose and err.strip():
         print(err, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *FLAGS, "-c", *EXTRA.get(src, []), "-I", os.path.join(ROOT, "include"),
+                     "-I", CSRC, os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    tmp = OUT + ".tmp"
+    _run([HIPCC, *FLAGS, "-shared", *objs, "-o", tmp, "-lhiprtc"], verbose)
     os.replace(tmp, OUT)
     build_napi(verbose)
     return OUT

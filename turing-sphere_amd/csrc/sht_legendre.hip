@@ -887,8 +887,415 @@ hipError_t analys_mf_nf(const typename P::Tables& T, int nfield, const LegTile& 
   }
 }
 
-// Built-in choice of the FP32 form's contraction unit (1 VALU, 2 matrix cores).
-constexpr int kDefaultImpl = 1;
+// ---------------------------------- contraction on 4x4 FP32 matrix blocks
+// v_mfma_f32_4x4x1_16b_f32 runs 16 independent 4 x 4 x 1 products per wave:
+// block b is lanes 4b..4b+3, A_b[r] comes from lane 4b + r, B_b[j] from lane
+// 4b + j, and D_b[r][j] lands in register r of lane 4b + j (checked on
+// gfx950).  Its four columns are exactly (field pair) x (re, im) of two
+// fields, so with "lane = latitude pair" no column is idle and no operand
+// needs a transpose for the synthesis: the matrix cores take every sum
+// (2 x 64 FP32 FMAs per wave per l and field pair, at twice the VALU's FMA
+// rate and on their own pipe) and the VALU runs the recurrence alone.
+__device__ __forceinline__ f4v mf441(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+// Q of l-offset i, field pair fp, for this lane's column j (0 for the idle
+// half of a one-field launch).  sqf: the staged row as floats, (i, f, re/im).
+template <int NF>
+__device__ __forceinline__ float qcol(const float* sqf, int i, int fp, int j) {
+  if constexpr (NF == 1) {
+    const float v = sqf[i * 2 + (j & 1)];
+    return j < 2 ? v : 0.f;
+  } else {
+    return sqf[i * 2 * NF + 4 * fp + j];
+  }
+}
+
+template <int K0, bool INJ, int NF, int K, class P>
+__device__ __forceinline__ void synth_tile_m4(const typename P::ab_t* sab, const float* sqf, int i,
+                                              int l, int j, ChainT<P> (&c)[K],
+                                              f4v (&accE)[K][(NF + 1) / 2],
+                                              f4v (&accO)[K][(NF + 1) / 2]) {
+  constexpr int FP = (NF + 1) / 2;
+  if constexpr (INJ) {
+#pragma unroll
+    for (int k = K0; k < K; ++k)
+      if (c[k].ls == l) {
+        c[k].y0 = c[k].s0;
+        c[k].y1 = c[k].s1;
+      }
+  }
+#pragma unroll
+  for (int p = 0; p < kTile; p += 2) {
+    float bE[FP], bO[FP];
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) {
+      bE[fp] = qcol<NF>(sqf, i + p, fp, j);
+      bO[fp] = qcol<NF>(sqf, i + p + 1, fp, j);
+    }
+    const auto a0 = sab[i + p + 2];
+    const auto a1 = sab[i + p + 3];
+#pragma unroll
+    for (int k = K0; k < K; ++k) {
+      const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) {
+        accE[k][fp] = mf441(y0, bE[fp], accE[k][fp]);
+        accO[k][fp] = mf441(y1, bO[fp], accO[k][fp]);
+      }
+      P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+    }
+  }
+}
+
+// The phase structure of synth_phases (chains [K0, K) until a lower chain
+// becomes live) around synth_tile_m4.
+template <int K0, int NF, int K, class P>
+__device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, const int (&lk)[K],
+                                                const int (&lz)[K], const typename P::ab_t* sab,
+                                                const float* sqf, ChainT<P> (&c)[K],
+                                                f4v (&accE)[K][(NF + 1) / 2],
+                                                f4v (&accO)[K][(NF + 1) / 2]) {
+  int lend = L + 1, injend = 0;
+#pragma unroll
+  for (int k = 0; k < K0; ++k) lend = min(lend, lk[k]);
+#pragma unroll
+  for (int k = K0; k < K; ++k) injend = max(injend, lz[k] + 1);
+  const int l1 = min(lend, injend);
+  for (; l < l1; l += kTile) synth_tile_m4<K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+  for (; l < lend; l += kTile)
+    synth_tile_m4<K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+  if constexpr (K0 > 0)
+    synth_phases_m4<K0 - 1, NF, K, P>(l, L, m, j, lk, lz, sab, sqf, c, accE, accO);
+}
+
+// Synthesis: per chain and field pair, D_E / D_O (E: even l - m, O: odd)
+// accumulate P[pair][l] * Q[l][col] over l with the row's 4 pairs of a block
+// as rows; the lane of column j ends with the sums of the 4 pairs of its block.
+template <int NF, int K, class P>
+__global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int nfield, Fields Fq,
+                                                      float2* __restrict__ fm, int rows) {
+  constexpr int FP = (NF + 1) / 2;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  float2* sq = reinterpret_cast<float2*>(sab + (L + 1 + kPadQ));
+  const float* sqf = reinterpret_cast<const float*>(sq);
+  const int f0 = blockIdx.y * NF;
+  const int lane = threadIdx.x & 63, j = lane & 3, blk = lane >> 2;
+
+  for (int r = 0; r < 2; ++r) {
+    const int m = row_m(r, rows, M);
+    if (m < 0) break;
+    __syncthreads();  // the previous row's readers are done
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
+    __syncthreads();
+
+    ChainT<P> c[K];
+    int lk[K], lz[K];
+    int lmin = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      load_chain_t<P>(T, m, pair_of(k), c[k]);
+      lk[k] = wave_min(c[k].ls);
+      lz[k] = last_seed(c[k].ls, L);
+      lmin = min(lmin, lk[k]);
+    }
+
+    f4v accE[K][FP], accO[K][FP];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) accE[k][fp] = accO[k][fp] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    if (lmin <= L) synth_phases_m4<K - 1, NF, K, P>(lmin, L, m, j, lk, lz, sab, sqf, c, accE, accO);
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int pb = pair_of(k) - lane + 4 * blk;  // pair of block row 0
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) {
+        const int f = 2 * fp + (j >> 1);
+        if (f >= NF || f0 + f >= nfield) continue;
+        // fm row-major per latitude: (f, ilat, m) at (f nlat + ilat)(M+1) + m
+        float* out = reinterpret_cast<float*>(fm + (long long)(f0 + f) * nlat * (M + 1) + m) +
+                     (j & 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int jp = pb + q;
+          if (jp >= T.npair) continue;
+          const float e = accE[k][fp][q], o = accO[k][fp][q];
+          out[2LL * jp * (M + 1)] = e + o;
+          out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+        }
+      }
+    }
+  }
+}
+
+// Analysis: Q[l][col] = sum_pairs P[pair][l] W[pair][col].  The sum runs
+// over latitude pairs, which are the lanes, so each chain's 16 values of a
+// span go through a wave-private LDS tile (row = pair, 16-byte stores) and
+// come back as one 16-byte load of 4 consecutive l of one pair per lane.
+// Block b takes the pairs 4b..4b+3 of the chain in turn (t): its lane r
+// supplies P[4b + t][ls + 4r + s] as A for accumulator s (s = 0..3), its lane
+// j the weight W[4b + t][col j] as B, so register r of accumulator s of
+// lane 4b + j sums P W over the block's pairs for l = ls + 4r + s.  Seeds sit
+// at l - m = 0 mod 4, so accumulator s holds one parity of l - m: even s
+// takes (G_N + G_S) w, odd s (G_N - G_S) w.  A span ends with a transposing
+// reduction over the 16 blocks (lane bits 2..5) and the waves' sums meeting
+// in LDS, as in k_leg_analys.
+constexpr int M4_SPAN = 16;
+constexpr int M4_PROW = M4_SPAN + 4;  // floats per pair row of the tile (conflict-free)
+
+template <int N>
+__device__ __forceinline__ void reduce_blocks(float (&v)[N]) {
+  // lane bit 5: lanes < 32 keep v[i], lanes >= 32 keep v[i + N/2]
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
+                                                    __float_as_uint(v[i + N / 2]), false, false);
+    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+  // lane bit 4
+#pragma unroll
+  for (int i = 0; i < N / 4; ++i) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]),
+                                                    __float_as_uint(v[i + N / 4]), false, false);
+    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+  const int lane = threadIdx.x & 63;
+  // lane bit 3: row_ror 8 pairs lane i with i ^ 8 within each row of 16
+  {
+    const bool hi = lane & 8;
+#pragma unroll
+    for (int i = 0; i < N / 8; ++i) {
+      const float keep = hi ? v[i + N / 8] : v[i];
+      const float send = hi ? v[i] : v[i + N / 8];
+      v[i] = keep + dpp<0x128>(send);
+    }
+  }
+  // lane bit 2
+  {
+    const bool hi = lane & 4;
+#pragma unroll
+    for (int i = 0; i < N / 16; ++i) {
+      const float keep = hi ? v[i + N / 16] : v[i];
+      const float send = hi ? v[i] : v[i + N / 16];
+      v[i] = keep + __shfl_xor(send, 4);
+    }
+  }
+}
+
+template <int NF, int K, class P>
+__global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int nfield,
+                                                       const float2* __restrict__ fm, Fields Fq,
+                                                       int rows) {
+  constexpr int FP = (NF + 1) / 2;
+  constexpr int NV = 16 * FP;  // accumulator values per lane
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ float red[2][8][64 * FP];
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
+  const int j = lane & 3, blk = lane >> 2;
+  float* tile = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1)) +
+                wv * (K * 64 * M4_PROW);  // one tile per chain
+  const int f0 = blockIdx.y * NF;
+
+  for (int r = 0; r < 2; ++r) {
+    const int m = row_m(r, rows, M);
+    if (m < 0) break;
+    const long long base = lm_base(L, m);
+    __syncthreads();
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+
+    ChainT<P> c[K];
+    int lk[K];
+    int lzmax = -1, lmin = 0x7fffffff;
+    // B operands for the whole row: W of pair 4 blk + t of chain k, this lane's
+    // column, even (WE) and odd (WO) parity
+    float WE[K][4][FP], WO[K][4][FP];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      load_chain_t<P>(T, m, pair_of(k), c[k]);
+      lk[k] = wave_min(c[k].ls);
+      lzmax = max(lzmax, last_seed(c[k].ls, L));
+      lmin = min(lmin, lk[k]);
+      const int pb = pair_of(k) - lane + 4 * blk;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int jp = pb + t;
+        const int jc = min(jp, T.npair - 1);
+        const int js = nlat - 1 - jc;
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) {
+          const int f = 2 * fp + (j >> 1);
+          const bool live = jp < T.npair && f < NF && f0 + f < nfield;
+          const int fld = min(f0 + min(f, NF - 1), nfield - 1);
+          const float* g =
+              reinterpret_cast<const float*>(fm + (long long)fld * nlat * (M + 1) + m) + (j & 1);
+          const float w = live ? T.w[jc] : 0.f;
+          const float gN = g[2LL * jc * (M + 1)], gS = g[2LL * js * (M + 1)];
+          if (js == jc) {  // equator of an odd grid pairs with itself
+            WE[k][t][fp] = gN * w;
+            WO[k][t][fp] = 0.f;
+          } else {
+            WE[k][t][fp] = (gN + gS) * w;
+            WO[k][t][fp] = (gN - gS) * w;
+          }
+        }
+      }
+    }
+    if (lane == 0) smin[wv] = lmin;
+    __syncthreads();  // also publishes the staged row
+    int lblk = smin[0];
+    for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+    lblk = min(lblk, L + 1);
+
+    // coefficients below every lane's significance point are exactly 0
+    for (int i = m + tid; i < lblk; i += nthr)
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        if (f0 + f < nfield)
+          reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
+              make_float2(0.f, 0.f);
+
+    int buf = 0;
+    for (int ls = lblk; ls <= L; ls += M4_SPAN) {
+      float v[NV];
+      if (lmin >= ls + M4_SPAN) {
+        // every chain of this wave is still exactly 0 in this span
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) red[buf][wv][lane * FP + fp] = 0.f;
+      } else {
+        f4v acc[4][FP];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) acc[s][fp] = f4v{0.f, 0.f, 0.f, 0.f};
+        // 1) each live chain's 16 values of the span -> its pair row of its tile
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (lk[k] >= ls + M4_SPAN) continue;  // chain still 0 (wave-uniform)
+          float y[M4_SPAN];
+#pragma unroll
+          for (int p = 0; p < M4_SPAN; p += 2) {
+            const int l = ls + p;
+            if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+            y[p] = (float)c[k].y0;
+            y[p + 1] = (float)c[k].y1;
+            const auto a0 = sab[l - m + 2], a1 = sab[l - m + 3];
+            P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+          }
+          float* tk = tile + k * (64 * M4_PROW);
+#pragma unroll
+          for (int q = 0; q < M4_SPAN / 4; ++q)
+            *reinterpret_cast<float4*>(tk + lane * M4_PROW + 4 * q) =
+                make_float4(y[4 * q], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]);
+        }
+        wave_sync_lds();
+        // 2) block products over the block's 4 pairs of each live chain
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (lk[k] >= ls + M4_SPAN) continue;
+          const float* tk = tile + k * (64 * M4_PROW);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float4 a4 =
+                *reinterpret_cast<const float4*>(tk + (4 * blk + t) * M4_PROW + 4 * j);
+            const float a[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+              for (int fp = 0; fp < FP; ++fp)
+                acc[s][fp] = mf441(a[s], (s & 1) ? WO[k][t][fp] : WE[k][t][fp], acc[s][fp]);
+          }
+        }
+        wave_sync_lds();  // the tiles are rewritten in the next span
+        // value (s, r, fp) at (4 s + r) FP + fp: after the reduction lane 4b + j
+        // holds index b FP + fp, i.e. s = b / 4, r = b % 4
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int fp = 0; fp < FP; ++fp) v[(4 * s + q) * FP + fp] = acc[s][fp][q];
+        reduce_blocks<NV>(v);
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) red[buf][wv][lane * FP + fp] = v[fp];
+      }
+      __syncthreads();
+      if (wv == 0) {
+        const int l = ls + 4 * (blk & 3) + (blk >> 2);
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) {
+          float s = red[buf][0][lane * FP + fp];
+          for (int w = 1; w < nwaves; ++w) s += red[buf][w][lane * FP + fp];
+          const int f = 2 * fp + (j >> 1);
+          if (l <= L && f < NF && f0 + f < nfield) {
+            float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
+                         2 * (base + (l - m));
+            dst[j & 1] = s;
+          }
+        }
+      }
+      buf ^= 1;
+    }
+  }
+}
+
+template <int NF, int K, class P>
+hipError_t synth_m4(const typename P::Tables& T, int nfield, int nwaves, int rows, const Fields& q,
+                    float2* fm, hipStream_t s) {
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  hipLaunchKernelGGL((k_leg_synth_m4<NF, K, P>), grid, dim3(64 * nwaves),
+                     synth_lds<P>(T.lmax, NF), s, T, nfield, q, fm, rows);
+  return hipGetLastError();
+}
+
+template <int NF, int K, class P>
+hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int rows,
+                     const float2* fm, const Fields& q, hipStream_t s) {
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+                     sizeof(float) * (size_t)nwaves * K * 64 * M4_PROW;
+  hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
+                     q, rows);
+  return hipGetLastError();
+}
+
+template <int NF, class P>
+hipError_t synth_m4_nf(const typename P::Tables& T, int nfield, const LegTile& t, const Fields& q,
+                       float2* fm, hipStream_t s) {
+  switch (t.k) {
+    case 1: return synth_m4<NF, 1, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 2: return synth_m4<NF, 2, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 4: return synth_m4<NF, 4, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NF, class P>
+hipError_t analys_m4_nf(const typename P::Tables& T, int nfield, const LegTile& t,
+                        const float2* fm, const Fields& q, hipStream_t s) {
+  switch (t.k) {
+    case 1: return analys_m4<NF, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 2: return analys_m4<NF, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 4: return analys_m4<NF, 4, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Built-in choice of the contraction unit (1 VALU, 2 16x16x4 matrix tiles,
+// 3 4x4 matrix blocks).
+constexpr int kDefaultImpl = 3;
 
 int round_k(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 0; }
 
@@ -903,15 +1310,25 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
     return t;
   }
-  if (t.impl == 0) t.impl = kDefaultImpl;
+  if (t.impl == 0) {
+    t.impl = kDefaultImpl;
+    // the FP64 recurrence's extra VALU work leaves a few-field synthesis no
+    // room for the block products' issue slots (measured: VALU sums faster)
+    if (legendre == TSPHERE_LEG_F64REC && stage == 0 && nfield < 4) t.impl = 1;
+  }
   // one row per workgroup when a launch has one or two fields (its workgroups
-  // are few and long); the matrix-core kernels always pair rows
+  // are few and long); the 16x16x4 matrix-core kernels always pair rows
   if (t.rows == 0) t.rows = nfield <= 2 ? 1 : 2;
   if (t.impl == 2) t.rows = 2;
-  if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;  // beyond the matrix-core kernels' reach
+  // beyond the matrix-core kernels' reach
+  if (t.impl == 2 && npair > 64 * 4 * 4) t.impl = 1;
+  if (t.impl == 3 && npair > 64 * 8 * 4) t.impl = 1;
   if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
-  if (t.nwaves == 0 && t.k == 0) t.nwaves = std::min(4, (npair + 63) / 64);
-  if (t.nwaves == 0) t.nwaves = std::min(4, (npair + 64 * t.k - 1) / (64 * t.k));
+  // block-matrix kernels of one or two fields per workgroup: up to 8 waves
+  // of one chain each (measured ahead of 4 waves of two)
+  const int wmax = (t.impl == 3 && t.nf <= 2) ? 8 : 4;
+  if (t.nwaves == 0 && t.k == 0) t.nwaves = std::min(wmax, (npair + 63) / 64);
+  if (t.nwaves == 0) t.nwaves = std::min(wmax, (npair + 64 * t.k - 1) / (64 * t.k));
   if (t.k == 0) t.k = round_k((npair + 64 * t.nwaves - 1) / (64 * t.nwaves));
   return t;
 }
@@ -919,10 +1336,11 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
 namespace {
 
 bool fp32_tile_ok(const LegTile& t, int npair) {
-  const bool kok = t.impl == 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
-  return (t.impl == 1 || t.impl == 2) && (t.rows == 1 || t.rows == 2) &&
+  const bool kok = t.impl >= 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
+  const int wmax = t.impl == 3 ? 8 : 4;
+  return (t.impl >= 1 && t.impl <= 3) && (t.rows == 1 || t.rows == 2) &&
          (t.nf == 1 || t.nf == 2 || t.nf == 4) &&
-         t.nwaves >= 1 && t.nwaves <= 4 && kok && 64 * t.nwaves * t.k >= npair;
+         t.nwaves >= 1 && t.nwaves <= wmax && kok && 64 * t.nwaves * t.k >= npair;
 }
 
 }  // namespace
@@ -936,6 +1354,11 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
       case 1: return synth_mf_nf<1, RecF>(T, nfield, t, q, fm, s);
       case 2: return synth_mf_nf<2, RecF>(T, nfield, t, q, fm, s);
       default: return synth_mf_nf<4, RecF>(T, nfield, t, q, fm, s);
+    }
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return synth_m4_nf<1, RecF>(T, nfield, t, q, fm, s);
+      case 2: return synth_m4_nf<2, RecF>(T, nfield, t, q, fm, s);
+      default: return synth_m4_nf<4, RecF>(T, nfield, t, q, fm, s);
     }
   switch (t.nf) {
     case 1: return synth_nf<1, RecF>(T, nfield, t, q, fm, s);
@@ -954,6 +1377,11 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
       case 2: return analys_mf_nf<2, RecF>(T, nfield, t, fm, q, s);
       default: return analys_mf_nf<4, RecF>(T, nfield, t, fm, q, s);
     }
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return analys_m4_nf<1, RecF>(T, nfield, t, fm, q, s);
+      case 2: return analys_m4_nf<2, RecF>(T, nfield, t, fm, q, s);
+      default: return analys_m4_nf<4, RecF>(T, nfield, t, fm, q, s);
+    }
   switch (t.nf) {
     case 1: return analys_nf<1, RecF>(T, nfield, t, fm, q, s);
     case 2: return analys_nf<2, RecF>(T, nfield, t, fm, q, s);
@@ -971,6 +1399,11 @@ hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields&
       case 2: return synth_mf_nf<2, RecD>(T, nfield, t, q, fm, s);
       default: return synth_mf_nf<4, RecD>(T, nfield, t, q, fm, s);
     }
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return synth_m4_nf<1, RecD>(T, nfield, t, q, fm, s);
+      case 2: return synth_m4_nf<2, RecD>(T, nfield, t, q, fm, s);
+      default: return synth_m4_nf<4, RecD>(T, nfield, t, q, fm, s);
+    }
   switch (t.nf) {
     case 1: return synth_nf<1, RecD>(T, nfield, t, q, fm, s);
     case 2: return synth_nf<2, RecD>(T, nfield, t, q, fm, s);
@@ -987,6 +1420,11 @@ hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2
       case 1: return analys_mf_nf<1, RecD>(T, nfield, t, fm, q, s);
       case 2: return analys_mf_nf<2, RecD>(T, nfield, t, fm, q, s);
       default: return analys_mf_nf<4, RecD>(T, nfield, t, fm, q, s);
+    }
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return analys_m4_nf<1, RecD>(T, nfield, t, fm, q, s);
+      case 2: return analys_m4_nf<2, RecD>(T, nfield, t, fm, q, s);
+      default: return analys_m4_nf<4, RecD>(T, nfield, t, fm, q, s);
     }
   switch (t.nf) {
     case 1: return analys_nf<1, RecD>(T, nfield, t, fm, q, s);

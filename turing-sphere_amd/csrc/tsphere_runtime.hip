@@ -835,9 +835,9 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
   if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
   const LegTile t = tile ? to_tile(*tile) : LegTile{};
-  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 2 ||
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 3 ||
       t.rows < 0 || t.rows > 2)
-    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0, 1 or 2; rows 0, 1 or 2)");
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0..3; rows 0, 1 or 2)");
   // validate against a one-field and a max-fields launch
   for (int nf : {1, p->cfg.max_fields}) {
     const LegTile r = resolve_leg_tile(p->cfg.legendre, stage, p->npair, nf, t);
@@ -845,13 +845,13 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
       if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
         return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
     } else {
-      const bool kok = r.impl == 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
+      const bool kok = r.impl >= 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
                                    : (r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8);
-      if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.nwaves < 1 || r.nwaves > 4 || !kok ||
-          64LL * r.nwaves * r.k < p->npair)
+      if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.nwaves < 1 ||
+          r.nwaves > (r.impl == 3 ? 8 : 4) || !kok || 64LL * r.nwaves * r.k < p->npair)
         return fail(TSPHERE_ERR_INVALID,
                     "FP32 tile impl=%d nf=%d nwaves=%d k=%d: need nf in {1,2,4}, nwaves in "
-                    "1..4, k in {1,2,4,8} (VALU) or {1,2,4} (matrix cores), "
+                    "1..4 (1..8 for impl 3), k in {1,2,4,8} (VALU) or {1,2,4} (matrix cores), "
                     "64*nwaves*k >= %d latitude pairs",
                     r.impl, r.nf, r.nwaves, r.k, p->npair);
     }
@@ -879,17 +879,17 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
       if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0, 0});
     return c;
   }
-  for (int impl = 1; impl <= 2; impl++)
+  for (int impl = 1; impl <= 3; impl++)
     for (int nf : {1, 2, 4}) {
       if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
-      for (int nw = 1; nw <= 4; nw++) {
-        const int kmax = impl == 2 ? 4 : 8;
+      for (int nw = 1; nw <= (impl == 3 ? 8 : 4); nw += (nw >= 4 ? 4 : 1)) {
+        const int kmax = impl >= 2 ? 4 : 8;
         int k = 1;
         while (k <= kmax && 64 * nw * k < p->npair) k *= 2;
         if (k > kmax) continue;
         if (nw > 1 && 64 * (nw - 1) * k >= p->npair) continue;  // a wave with no pairs
         c.push_back(LegTile{nf, nw, k, 0, impl, 2});
-        if (impl == 1) c.push_back(LegTile{nf, nw, k, 0, impl, 1});
+        if (impl != 2) c.push_back(LegTile{nf, nw, k, 0, impl, 1});
       }
     }
   return c;

@@ -114,7 +114,9 @@ class ShtPlan:
     def set_tile(self, stage: str, nf: int = 0, nwaves: int = 0, k: int = 0,
                  ctc: int = 0, impl: int = 0, rows: int = 0) -> None:
         """Fix the Legendre workgroup shape of one stage (0 = built-in); for
-        the FP32 form impl 1 = VALU contraction, 2 = FP32 matrix cores."""
+        the FP32 and FP64-recurrence forms impl 1 = VALU contraction,
+        2 = FP32 matrix cores as 16x16x4 tiles, 3 = FP32 matrix cores as
+        16 4x4 blocks per wave."""
         t = _lib.LegTileC(nf, nwaves, k, ctc, impl, rows)
         _lib.call("tsphere_plan_set_tile", self._h, self.STAGES[stage], C.byref(t))
 
