@@ -50,14 +50,6 @@ __device__ __forceinline__ long long lm_base(int lmax, int m) {
   return (long long)m * (lmax + 1) - ((long long)m * (m - 1)) / 2;
 }
 
-// a * cos(theta) * y + c, with cos(theta) = ct.x + ct.y (f32 hi + lo of the
-// f64 node): rounding the node to f32 would move theta, and every ytilde_l
-// with it, by ~ l 2^-24 / sin(theta).  The low half costs one FMA.
-__device__ __forceinline__ float act_fma(float a, float y, float2 ct, float c) {
-  const float u = a * y;
-  return fmaf(u, ct.x, fmaf(u, ct.y, c));
-}
-
 __device__ __forceinline__ void cfma(float2& acc, float y, float2 q) {
   acc.x = fmaf(y, q.x, acc.x);
   acc.y = fmaf(y, q.y, acc.y);
@@ -129,34 +121,39 @@ __device__ __forceinline__ int row_m(int r, int rows, int M) {
   return (r == 1 && m <= mp) ? -1 : m;
 }
 
-// Recurrence precision of the Legendre kernels.  RecF: f32 state, cos(theta) as an
-// f32 hi + lo pair (4 VALU ops per l).  RecD: f64 state from the f64 tables
-// and seeds (3 f64 ops per l; on gfx950 an f64 FMA issues at ~88% of the f32
-// rate), rounded to f32 once per value before the FP32 sums: the FP32 form's
-// error at high degree comes from rounding the recurrence state, which this
-// removes, at about the FP32 form's cost.
-// (y_l, y_{l+1}) -> (y_{l+2}, y_{l+3}): ytilde_l = a_l cos(theta) ytilde_{l-1} + b_l ytilde_{l-2}
+// Recurrence precision of the Legendre kernels, both on the rescaled
+// functions z_l = ytilde_l / c_l of LegTables:
+//   z_l = alpha_l cos(theta) z_{l-1} + z_{l-2}.
+// RecF: f32 state, cos(theta) as an f32 hi + lo pair (3 VALU ops per l:
+// one product, two FMAs; rounding the node to f32 would move theta, and every
+// value with it, by ~ l 2^-24 / sin(theta)).  RecD: f64 state from the f64 alpha, node and seeds
+// (2 f64 ops per l), rounded to f32 once per value before the FP32 sums: the
+// FP32 form's error at high degree comes from rounding the recurrence state,
+// which this removes, at about the FP32 form's cost.
+// (z_l, z_{l+1}) -> (z_{l+2}, z_{l+3}) with a0 = alpha_{l+2}, a1 = alpha_{l+3}
 struct RecF {
   using Tables = LegTables;
-  using ab_t = float2;
+  using ab_t = float;
   using ct_t = float2;
   using y_t = float;
-  static __device__ __forceinline__ ab_t zero_ab() { return make_float2(0.f, 0.f); }
+  static __device__ __forceinline__ ab_t zero_ab() { return 0.f; }
   static __device__ __forceinline__ void advance(ab_t a0, ab_t a1, ct_t ct, y_t& y0, y_t& y1) {
-    const float t0 = act_fma(a0.x, y1, ct, a0.y * y0);
-    y1 = act_fma(a1.x, t0, ct, a1.y * y1);
+    const float u0 = a0 * y1;
+    const float t0 = fmaf(u0, ct.x, fmaf(u0, ct.y, y0));
+    const float u1 = a1 * t0;
+    y1 = fmaf(u1, ct.x, fmaf(u1, ct.y, y1));
     y0 = t0;
   }
 };
 struct RecD {
-  using Tables = LegTables64;
-  using ab_t = double2;
+  using Tables = LegTablesZ64;
+  using ab_t = double;
   using ct_t = double;
   using y_t = double;
-  static __device__ __forceinline__ ab_t zero_ab() { return make_double2(0.0, 0.0); }
+  static __device__ __forceinline__ ab_t zero_ab() { return 0.0; }
   static __device__ __forceinline__ void advance(ab_t a0, ab_t a1, ct_t ct, y_t& y0, y_t& y1) {
-    const double t0 = fma(a0.x * y1, ct, a0.y * y0);
-    y1 = fma(a1.x * t0, ct, a1.y * y1);
+    const double t0 = fma(a0 * y1, ct, y0);
+    y1 = fma(a1 * t0, ct, y1);
     y0 = t0;
   }
 };
@@ -185,7 +182,8 @@ __device__ __forceinline__ void load_chain_t(const typename P::Tables& T, int m,
   c.y0 = c.y1 = 0;
 }
 
-// Stage a row's (a, b) (in the recurrence's precision) and the fields' Q.
+// Stage a row's alpha (in the recurrence's precision) and the fields' Q
+// scaled by c_l (the synthesis sums z_l (c_l Q_l)).
 template <int NF, class P>
 __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, int f0,
                                             int nfield, const Fields& Fq, bool with_q,
@@ -195,11 +193,13 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
   for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
     sab[i] = i < n ? T.ab[base + i] : P::zero_ab();
     if (with_q) {
+      const float cz = i < n ? T.cz[base + i] : 0.f;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int fld = min(f0 + f, nfield - 1);
-        sq[i * NF + f] =
+        const float2 q =
             i < n ? reinterpret_cast<const float2*>(Fq.at(fld))[base + i] : make_float2(0.f, 0.f);
+        sq[i * NF + f] = make_float2(q.x * cz, q.y * cz);
       }
     }
   }
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
         if (l <= L && f0 + f < nfield) {
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
-          dst[rem & 1] = s;
+          dst[rem & 1] = s * T.cz[base + (l - m)];
         }
       }
       buf ^= 1;
@@ -836,7 +836,8 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(typename P::Tables T, int
           const int f = (col % (2 * NF)) >> 1;
           if (f0 + f < nfield)
             reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f)))[2 * (base + (l - m)) +
-                                                                        (col & 1)] = s;
+                                                                        (col & 1)] =
+                s * T.cz[base + (l - m)];
         }
       }
       __syncthreads();  // red is rewritten next iteration
@@ -912,45 +913,51 @@ __device__ __forceinline__ float qcol(const float* sqf, int i, int fp, int j) {
   }
 }
 
-template <int K0, bool INJ, int NF, int K, class P>
+// NT l-values (4 or 8) of chains [K0, K) from l = m + i; the tile's LDS reads
+// are issued before its first product, so a longer tile hides more of their
+// latency.
+template <int NT, int K0, bool INJ, int NF, int K, class P>
 __device__ __forceinline__ void synth_tile_m4(const typename P::ab_t* sab, const float* sqf, int i,
                                               int l, int j, ChainT<P> (&c)[K],
                                               f4v (&accE)[K][(NF + 1) / 2],
                                               f4v (&accO)[K][(NF + 1) / 2]) {
   constexpr int FP = (NF + 1) / 2;
-  if constexpr (INJ) {
+  float bq[NT][FP];
+  typename P::ab_t al[NT];
 #pragma unroll
-    for (int k = K0; k < K; ++k)
-      if (c[k].ls == l) {
-        c[k].y0 = c[k].s0;
-        c[k].y1 = c[k].s1;
-      }
+  for (int p = 0; p < NT; ++p) {
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) bq[p][fp] = qcol<NF>(sqf, i + p, fp, j);
+    al[p] = sab[i + p + 2];
   }
 #pragma unroll
-  for (int p = 0; p < kTile; p += 2) {
-    float bE[FP], bO[FP];
+  for (int p = 0; p < NT; p += 2) {
+    if constexpr (INJ) {
+      if (p % kTile == 0) {
 #pragma unroll
-    for (int fp = 0; fp < FP; ++fp) {
-      bE[fp] = qcol<NF>(sqf, i + p, fp, j);
-      bO[fp] = qcol<NF>(sqf, i + p + 1, fp, j);
+        for (int k = K0; k < K; ++k)
+          if (c[k].ls == l + p) {
+            c[k].y0 = c[k].s0;
+            c[k].y1 = c[k].s1;
+          }
+      }
     }
-    const auto a0 = sab[i + p + 2];
-    const auto a1 = sab[i + p + 3];
 #pragma unroll
     for (int k = K0; k < K; ++k) {
       const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
 #pragma unroll
       for (int fp = 0; fp < FP; ++fp) {
-        accE[k][fp] = mf441(y0, bE[fp], accE[k][fp]);
-        accO[k][fp] = mf441(y1, bO[fp], accO[k][fp]);
+        accE[k][fp] = mf441(y0, bq[p][fp], accE[k][fp]);
+        accO[k][fp] = mf441(y1, bq[p + 1][fp], accO[k][fp]);
       }
-      P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+      P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
     }
   }
 }
 
 // The phase structure of synth_phases (chains [K0, K) until a lower chain
-// becomes live) around synth_tile_m4.
+// becomes live) around synth_tile_m4: 8-l tiles where they fit before the
+// phase's end, a 4-l tile for the rest (phase ends are 4-aligned in l - m).
 template <int K0, int NF, int K, class P>
 __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, const int (&lk)[K],
                                                 const int (&lz)[K], const typename P::ab_t* sab,
@@ -963,9 +970,19 @@ __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, cons
 #pragma unroll
   for (int k = K0; k < K; ++k) injend = max(injend, lz[k] + 1);
   const int l1 = min(lend, injend);
-  for (; l < l1; l += kTile) synth_tile_m4<K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
-  for (; l < lend; l += kTile)
-    synth_tile_m4<K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+  while (l < l1) {
+    if (l + 8 <= lend) {
+      synth_tile_m4<8, K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+      l += 8;
+    } else {
+      synth_tile_m4<4, K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+      l += 4;
+    }
+  }
+  for (; l + 8 <= lend; l += 8)
+    synth_tile_m4<8, K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+  for (; l < lend; l += 4)
+    synth_tile_m4<4, K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
   if constexpr (K0 > 0)
     synth_phases_m4<K0 - 1, NF, K, P>(l, L, m, j, lk, lz, sab, sqf, c, accE, accO);
 }
@@ -1036,64 +1053,30 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
 
 // Analysis: Q[l][col] = sum_pairs P[pair][l] W[pair][col].  The sum runs
 // over latitude pairs, which are the lanes, so each chain's 16 values of a
-// span go through a wave-private LDS tile (row = pair, 16-byte stores) and
-// come back as one 16-byte load of 4 consecutive l of one pair per lane.
-// Block b takes the pairs 4b..4b+3 of the chain in turn (t): its lane r
-// supplies P[4b + t][ls + 4r + s] as A for accumulator s (s = 0..3), its lane
-// j the weight W[4b + t][col j] as B, so register r of accumulator s of
-// lane 4b + j sums P W over the block's pairs for l = ls + 4r + s.  Seeds sit
-// at l - m = 0 mod 4, so accumulator s holds one parity of l - m: even s
-// takes (G_N + G_S) w, odd s (G_N - G_S) w.  A span ends with a transposing
-// reduction over the 16 blocks (lane bits 2..5) and the waves' sums meeting
-// in LDS, as in k_leg_analys.
+// span go through a wave-private LDS tile (row = l, column = pair) and come
+// back as 16-byte loads of 4 consecutive pairs at one l.  Block b is an
+// l-group lg (4 values of l - m of one parity: offsets {0,2,4,6}, {8,..,14},
+// {1,..,7}, {9,..,15} of the span) times a pair-group pg (16 pairs of the
+// chain): its lane r supplies P[pair][l_r] as A, its lane j the weight
+// W[pair][col j] of the block's parity as B ((G_N + G_S) w even, (G_N - G_S) w
+// odd), the pair running over the 16 of pg in turn.  Register r of lane
+// 4b + j then sums P W over the pair-group for l_r, so a span ends with a sum
+// over the 4 pair-groups (lane bits 4, 5: two swaps) and the waves' sums
+// meeting in LDS.  Seeds sit at l - m = 0 mod 4, so a span's offsets keep the
+// parity of l - m.  Lane bits 2, 3 map to l-groups (0, 2, 1, 3) so that the
+// two blocks of an 8-lane LDS pass read rows 4 apart in bank terms.
 constexpr int M4_SPAN = 16;
-constexpr int M4_PROW = M4_SPAN + 4;  // floats per pair row of the tile (conflict-free)
+constexpr int M4_TROW = 64 + 4;  // floats per l row of the tile
 
-template <int N>
-__device__ __forceinline__ void reduce_blocks(float (&v)[N]) {
-  // lane bit 5: lanes < 32 keep v[i], lanes >= 32 keep v[i + N/2]
-#pragma unroll
-  for (int i = 0; i < N / 2; ++i) {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
-                                                    __float_as_uint(v[i + N / 2]), false, false);
-    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  }
-  // lane bit 4
-#pragma unroll
-  for (int i = 0; i < N / 4; ++i) {
-    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]),
-                                                    __float_as_uint(v[i + N / 4]), false, false);
-    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  }
-  const int lane = threadIdx.x & 63;
-  // lane bit 3: row_ror 8 pairs lane i with i ^ 8 within each row of 16
-  {
-    const bool hi = lane & 8;
-#pragma unroll
-    for (int i = 0; i < N / 8; ++i) {
-      const float keep = hi ? v[i + N / 8] : v[i];
-      const float send = hi ? v[i] : v[i + N / 8];
-      v[i] = keep + dpp<0x128>(send);
-    }
-  }
-  // lane bit 2
-  {
-    const bool hi = lane & 4;
-#pragma unroll
-    for (int i = 0; i < N / 16; ++i) {
-      const float keep = hi ? v[i + N / 16] : v[i];
-      const float send = hi ? v[i] : v[i + N / 16];
-      v[i] = keep + __shfl_xor(send, 4);
-    }
-  }
-}
+__device__ __forceinline__ int m4_lgroup(int blk) { return ((blk & 1) << 1) | ((blk >> 1) & 1); }
+// l offset in the span of row r of l-group g
+__device__ __forceinline__ int m4_off(int g, int r) { return (g >> 1) + 2 * r + 8 * (g & 1); }
 
 template <int NF, int K, class P>
 __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int nfield,
                                                        const float2* __restrict__ fm, Fields Fq,
                                                        int rows) {
   constexpr int FP = (NF + 1) / 2;
-  constexpr int NV = 16 * FP;  // accumulator values per lane
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][8][64 * FP];
   __shared__ int smin[8];
@@ -1102,8 +1085,10 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nthr = blockDim.x, nwaves = nthr >> 6;
   const int j = lane & 3, blk = lane >> 2;
+  const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
+  const bool odd = lg >> 1;  // this block's parity of l - m
   float* tile = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1)) +
-                wv * (K * 64 * M4_PROW);  // one tile per chain
+                wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
   const int f0 = blockIdx.y * NF;
 
   for (int r = 0; r < 2; ++r) {
@@ -1116,40 +1101,52 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
     ChainT<P> c[K];
     int lk[K];
     int lzmax = -1, lmin = 0x7fffffff;
-    // B operands for the whole row: W of pair 4 blk + t of chain k, this lane's
-    // column, even (WE) and odd (WO) parity
-    float WE[K][4][FP], WO[K][4][FP];
+    // B operands for the whole row: each lane computes W of its own pair
+    // (every column, both parities) into the chain's tile, then gathers the
+    // 16 of its block: pair 16 pg + t, column j, the block's parity.
+    float W[K][16][FP];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       load_chain_t<P>(T, m, pair_of(k), c[k]);
       lk[k] = wave_min(c[k].ls);
       lzmax = max(lzmax, last_seed(c[k].ls, L));
       lmin = min(lmin, lk[k]);
-      const int pb = pair_of(k) - lane + 4 * blk;
+      const int jp = pair_of(k);
+      const int jc = min(jp, T.npair - 1);
+      const int js = nlat - 1 - jc;
+      const float w = jp < T.npair ? T.w[jc] : 0.f;
+      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane * (8 * FP);  // [pair][parity][fp][col]
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int jp = pb + t;
-        const int jc = min(jp, T.npair - 1);
-        const int js = nlat - 1 - jc;
-#pragma unroll
-        for (int fp = 0; fp < FP; ++fp) {
-          const int f = 2 * fp + (j >> 1);
-          const bool live = jp < T.npair && f < NF && f0 + f < nfield;
-          const int fld = min(f0 + min(f, NF - 1), nfield - 1);
-          const float* g =
-              reinterpret_cast<const float*>(fm + (long long)fld * nlat * (M + 1) + m) + (j & 1);
-          const float w = live ? T.w[jc] : 0.f;
-          const float gN = g[2LL * jc * (M + 1)], gS = g[2LL * js * (M + 1)];
-          if (js == jc) {  // equator of an odd grid pairs with itself
-            WE[k][t][fp] = gN * w;
-            WO[k][t][fp] = 0.f;
-          } else {
-            WE[k][t][fp] = (gN + gS) * w;
-            WO[k][t][fp] = (gN - gS) * w;
-          }
+      for (int f = 0; f < 2 * FP; ++f) {
+        const bool live = f < NF && f0 + f < nfield;
+        const int fld = min(f0 + min(f, NF - 1), nfield - 1);
+        const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
+        const float2 gN = g[(long long)jc * (M + 1)], gS = g[(long long)js * (M + 1)];
+        float4 we{0.f, 0.f, 0.f, 0.f};
+        if (live) {
+          if (js == jc)  // equator of an odd grid pairs with itself
+            we = make_float4(gN.x * w, gN.y * w, 0.f, 0.f);
+          else
+            we = make_float4((gN.x + gS.x) * w, (gN.y + gS.y) * w, (gN.x - gS.x) * w,
+                             (gN.y - gS.y) * w);
         }
+        const int fp = f >> 1, c0 = 2 * (f & 1);
+        tw[(0 * FP + fp) * 4 + c0] = we.x;
+        tw[(0 * FP + fp) * 4 + c0 + 1] = we.y;
+        tw[(1 * FP + fp) * 4 + c0] = we.z;
+        tw[(1 * FP + fp) * 4 + c0 + 1] = we.w;
       }
     }
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp)
+          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) + (16 * pg + t) * (8 * FP) +
+                             ((odd ? FP : 0) + fp) * 4 + j];
+    wave_sync_lds();  // the tiles are rewritten by the spans
     if (lane == 0) smin[wv] = lmin;
     __syncthreads();  // also publishes the staged row
     int lblk = smin[0];
@@ -1166,22 +1163,16 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
 
     int buf = 0;
     for (int ls = lblk; ls <= L; ls += M4_SPAN) {
-      float v[NV];
       if (lmin >= ls + M4_SPAN) {
         // every chain of this wave is still exactly 0 in this span
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp) red[buf][wv][lane * FP + fp] = 0.f;
       } else {
-        f4v acc[4][FP];
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-          for (int fp = 0; fp < FP; ++fp) acc[s][fp] = f4v{0.f, 0.f, 0.f, 0.f};
-        // 1) each live chain's 16 values of the span -> its pair row of its tile
+        // 1) each live chain's 16 values of the span -> column `lane` of its tile
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (lk[k] >= ls + M4_SPAN) continue;  // chain still 0 (wave-uniform)
-          float y[M4_SPAN];
+          float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
 #pragma unroll
           for (int p = 0; p < M4_SPAN; p += 2) {
             const int l = ls + p;
@@ -1189,51 +1180,52 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
               c[k].y0 = c[k].s0;
               c[k].y1 = c[k].s1;
             }
-            y[p] = (float)c[k].y0;
-            y[p + 1] = (float)c[k].y1;
+            tk[p * M4_TROW] = (float)c[k].y0;
+            tk[(p + 1) * M4_TROW] = (float)c[k].y1;
             const auto a0 = sab[l - m + 2], a1 = sab[l - m + 3];
             P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
           }
-          float* tk = tile + k * (64 * M4_PROW);
-#pragma unroll
-          for (int q = 0; q < M4_SPAN / 4; ++q)
-            *reinterpret_cast<float4*>(tk + lane * M4_PROW + 4 * q) =
-                make_float4(y[4 * q], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]);
         }
         wave_sync_lds();
-        // 2) block products over the block's 4 pairs of each live chain
+        // 2) block products over the pair-group's 16 pairs of each live chain
+        f4v acc[FP];
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+        const int arow = m4_off(lg, j) * M4_TROW + 16 * pg;  // lane j supplies row j
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (lk[k] >= ls + M4_SPAN) continue;
-          const float* tk = tile + k * (64 * M4_PROW);
+          const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const float4 a4 =
-                *reinterpret_cast<const float4*>(tk + (4 * blk + t) * M4_PROW + 4 * j);
+          for (int t = 0; t < 16; t += 4) {
+            const float4 a4 = *reinterpret_cast<const float4*>(tk + t);
             const float a[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-              for (int fp = 0; fp < FP; ++fp)
-                acc[s][fp] = mf441(a[s], (s & 1) ? WO[k][t][fp] : WE[k][t][fp], acc[s][fp]);
+              for (int fp = 0; fp < FP; ++fp) acc[fp] = mf441(a[u], W[k][t + u][fp], acc[fp]);
           }
         }
         wave_sync_lds();  // the tiles are rewritten in the next span
-        // value (s, r, fp) at (4 s + r) FP + fp: after the reduction lane 4b + j
-        // holds index b FP + fp, i.e. s = b / 4, r = b % 4
+        // 3) sum over the 4 pair-groups: lane bit 5 keeps rows {0,1} / {2,3},
+        // lane bit 4 row {0} / {1} of those: lane 4b + j ends with row pg
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int fp = 0; fp < FP; ++fp) {
+          float v[4] = {acc[fp][0], acc[fp][1], acc[fp][2], acc[fp][3]};
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int fp = 0; fp < FP; ++fp) v[(4 * s + q) * FP + fp] = acc[s][fp][q];
-        reduce_blocks<NV>(v);
-#pragma unroll
-        for (int fp = 0; fp < FP; ++fp) red[buf][wv][lane * FP + fp] = v[fp];
+          for (int i = 0; i < 2; ++i) {
+            const auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
+                                                             __float_as_uint(v[i + 2]), false, false);
+            v[i] = __uint_as_float(s2[0]) + __uint_as_float(s2[1]);
+          }
+          const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
+                                                           __float_as_uint(v[1]), false, false);
+          red[buf][wv][lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
+        }
       }
       __syncthreads();
       if (wv == 0) {
-        const int l = ls + 4 * (blk & 3) + (blk >> 2);
+        const int l = ls + m4_off(lg, pg);
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp) {
           float s = red[buf][0][lane * FP + fp];
@@ -1242,7 +1234,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
           if (l <= L && f < NF && f0 + f < nfield) {
             float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                          2 * (base + (l - m));
-            dst[j & 1] = s;
+            dst[j & 1] = s * T.cz[base + (l - m)];
           }
         }
       }
@@ -1265,7 +1257,7 @@ hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int ro
                      const float2* fm, const Fields& q, hipStream_t s) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
-                     sizeof(float) * (size_t)nwaves * K * 64 * M4_PROW;
+                     sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
   hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q, rows);
   return hipGetLastError();
@@ -1310,12 +1302,7 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
     return t;
   }
-  if (t.impl == 0) {
-    t.impl = kDefaultImpl;
-    // the FP64 recurrence's extra VALU work leaves a few-field synthesis no
-    // room for the block products' issue slots (measured: VALU sums faster)
-    if (legendre == TSPHERE_LEG_F64REC && stage == 0 && nfield < 4) t.impl = 1;
-  }
+  if (t.impl == 0) t.impl = kDefaultImpl;
   // one row per workgroup when a launch has one or two fields (its workgroups
   // are few and long); the 16x16x4 matrix-core kernels always pair rows
   if (t.rows == 0) t.rows = nfield <= 2 ? 1 : 2;
@@ -1389,7 +1376,7 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
   }
 }
 
-hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields& q, float2* fm,
+hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
                                  LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 0, T.npair, nfield, tile);
@@ -1411,7 +1398,7 @@ hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields&
   }
 }
 
-hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2* fm,
+hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 1, T.npair, nfield, tile);

@@ -30,13 +30,21 @@ constexpr float kInvScale = 1.0f / 72057594037927936.0f;  // 2^-56, exact
 constexpr float kAccuracy = 1e-15f;
 constexpr float kRescaleThr = kAccuracy * kScale + 1.0f;  // ~73.06
 
-// Device tables of one plan (all f32 except the seed exponent).
+// Device tables of the FP32 form.  The recurrence runs on the rescaled
+// functions z_l = ytilde_l / c_l with c_m = c_{m+1} = 1 and c_l = b_l c_{l-2}
+// (coeffs.ts:28-56's b_l^m), for which it reads
+//   z_l = alpha_l cos(theta) z_{l-1} + z_{l-2},  alpha_l = a_l c_{l-1} / c_l
+// -- one coefficient per step instead of two, so one VALU op less per l.
+// |c_l| stays within [0.24, 1.13] up to lmax 1023 (no range issue in f32);
+// the synthesis takes c_l Q_lm as its coefficients and the analysis
+// multiplies its sums by c_l.
 struct LegTables {
-  const float2* ab;      // [nlm + pad]: (a_l^m, b_l^m), m-major like Q
+  const float* ab;       // [nlm + pad]: alpha_l^m, m-major like Q (0 at l = m)
+  const float* cz;       // [nlm + pad]: c_l^m
   const float2* ct;      // [npair]: cos(theta_j) of the northern latitude of pair j,
                          //   as an unevaluated f32 sum (hi, lo) of the f64 node
   const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
-  const SeedL* seed;     // [(mmax+1) * npair]
+  const SeedL* seed;     // [(mmax+1) * npair]: z at the seed position
   int lmax, mmax, nlat, npair;
 };
 
@@ -56,6 +64,17 @@ struct LegTables64 {
   int lmax, mmax, nlat, npair;
 };
 
+// Tables of the FP64-recurrence form: the rescaled recurrence of LegTables
+// in f64.
+struct LegTablesZ64 {
+  const double* ab;      // [nlm + pad]: alpha_l^m in f64
+  const float* cz;       // [nlm + pad]: c_l^m
+  const double* ct;      // [npair]: cos(theta_j), f64
+  const float* w;        // [nlat]: Gauss weight * 2 pi / nphi
+  const SeedD* seed;     // [(mmax+1) * npair]: z at the seed position, f64
+  int lmax, mmax, nlat, npair;
+};
+
 // Tile shape of a Legendre launch (0 = the built-in choice for that entry).
 //  FP32 form: impl 1 = contraction on the VALU, 2 = on the FP32 matrix cores;
 //  nf fields per workgroup (1, 2, 4), nwaves waves, k latitude pairs per lane
@@ -72,13 +91,16 @@ struct Plan {
   int fourier = TSPHERE_FOURIER_FFT;
   int npair = 0;
   long long nlm = 0;
-  float2* d_ab = nullptr;
+  float* d_alz = nullptr;     // FP32 form: alpha, z seeds
+  float* d_cz = nullptr;      // FP32 and FP64-recurrence forms: c_l
   float2* d_ct = nullptr;
   float* d_w = nullptr;
   float* d_ones = nullptr;  // [nlat] of 1.0: the unit weights of the synthesis transpose
-  SeedL* d_seed = nullptr;
-  double2* d_ab64 = nullptr;   // MFMA path only
-  double* d_ct64 = nullptr;
+  SeedL* d_seedz = nullptr;
+  double* d_alz64 = nullptr;  // FP64-recurrence form: alpha, z seeds in f64
+  SeedD* d_seedz64 = nullptr;
+  double2* d_ab64 = nullptr;   // split-BF16 form: (a, b), seeds in f64
+  double* d_ct64 = nullptr;    // both f64 forms
   SeedD* d_seed64 = nullptr;
   LegTile tile[2] = {};  // [synthesis, analysis]; zeros = built-in heuristic
   float2* d_trig = nullptr;
@@ -86,8 +108,12 @@ struct Plan {
   size_t bytes = 0;
   std::vector<double> theta, cos_theta, weights;
   LegTables tables(bool unit_w = false) const {
-    return LegTables{d_ab, d_ct, unit_w ? d_ones : d_w, d_seed, cfg.lmax, cfg.mmax, cfg.nlat,
-                     npair};
+    return LegTables{d_alz, d_cz, d_ct, unit_w ? d_ones : d_w, d_seedz, cfg.lmax, cfg.mmax,
+                     cfg.nlat, npair};
+  }
+  LegTablesZ64 tablesz64(bool unit_w = false) const {
+    return LegTablesZ64{d_alz64,  d_cz,     d_ct64,   unit_w ? d_ones : d_w, d_seedz64,
+                        cfg.lmax, cfg.mmax, cfg.nlat, npair};
   }
   LegTables64 tables64(bool unit_w = false) const {
     return LegTables64{d_ab64,    d_ct64,   unit_w ? d_ones : d_w, d_seed64, cfg.lmax,
@@ -137,6 +163,8 @@ struct HostTables {
   std::vector<SeedL> seed;     // (mmax+1)*npair
   std::vector<SeedD> seed64;   // (mmax+1)*npair, same seeds in f64
   std::vector<double> ab64;    // 2*(nlm + pad), f64 coefficients
+  std::vector<double> alz64, cz64;  // nlm + pad: alpha_l, c_l of the rescaled recurrence
+  std::vector<SeedD> seedz64;       // (mmax+1)*npair: the seeds as z = ytilde / c
   std::vector<float> trig;     // 2*nphi
   std::vector<double> x, wg;   // f64 Gauss nodes / weights
 };
@@ -159,9 +187,9 @@ hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields&
 hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s);
 // The FP64-recurrence form: the VALU kernels with an f64 recurrence.
-hipError_t launch_leg_synth_f64r(const LegTables64& T, int nfield, const Fields& q, float2* fm,
+hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
                                  LegTile tile, hipStream_t s);
-hipError_t launch_leg_analys_f64r(const LegTables64& T, int nfield, const float2* fm,
+hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s);
 // The shape a launch of `nfield` fields would use (tile fields resolved).
 LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile tile);

@@ -134,6 +134,23 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
   }
   t.ab.assign(2 * (nlm + kAbPad), 0.0f);
   for (size_t i = 0; i < t.ab64.size(); i++) t.ab[i] = (float)t.ab64[i];
+  // Rescaled recurrence: c_m = c_{m+1} = 1, c_l = b_l c_{l-2},
+  // alpha_l = a_l c_{l-1} / c_l (alpha_{m+1} = a_{m+1}).
+  t.alz64.assign(nlm + kAbPad, 0.0);
+  t.cz64.assign(nlm + kAbPad, 0.0);
+  for (int m = 0; m <= mmax; m++) {
+    const long long b0 = lm_index(lmax, m, m);
+    t.cz64[b0] = 1.0;
+    if (m + 1 <= lmax) {
+      t.cz64[b0 + 1] = 1.0;
+      t.alz64[b0 + 1] = t.ab64[2 * (b0 + 1)];
+    }
+    for (int l = m + 2; l <= lmax; l++) {
+      const long long i = b0 + (l - m);
+      t.cz64[i] = t.ab64[2 * i + 1] * t.cz64[i - 2];
+      t.alz64[i] = t.ab64[2 * i] * t.cz64[i - 1] / t.cz64[i];
+    }
+  }
   t.amm.resize(mmax + 1);
   for (int m = 0; m <= mmax; m++) t.amm[m] = (float)amm[m];
 
@@ -157,6 +174,7 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
   // and keep the values at the last l = m + 4k at or before it.
   t.seed.resize((size_t)(mmax + 1) * npair);
   t.seed64.resize((size_t)(mmax + 1) * npair);
+  t.seedz64.resize((size_t)(mmax + 1) * npair);
   const double thr = (double)kAccuracy;
   for (int m = 0; m <= mmax; m++) {
     const double la = std::log2(std::fabs(amm[m]));
@@ -209,6 +227,13 @@ void build_host_tables(const tsphere_sht_config& c, HostTables& t) {
       }
       t.seed[(size_t)m * npair + j] = seed;
       t.seed64[(size_t)m * npair + j] = seed64;
+      SeedD sz = seed64;
+      if (sz.l <= lmax) {
+        const long long i = lm_index(lmax, sz.l, m);
+        sz.y0 = seed64.y0 / t.cz64[i];
+        sz.y1 = sz.l + 1 <= lmax ? seed64.y1 / t.cz64[i + 1] : 0.0;
+      }
+      t.seedz64[(size_t)m * npair + j] = sz;
     }
   }
 
@@ -403,7 +428,8 @@ int tsphere_plan_tables(const tsphere_sht_config* cfg, float* ab, float* amm, fl
 
 static void plan_free(Plan* p) {
   if (!p) return;
-  for (void* ptr : {(void*)p->d_ab, (void*)p->d_ct, (void*)p->d_w, (void*)p->d_ones, (void*)p->d_seed,
+  for (void* ptr : {(void*)p->d_alz, (void*)p->d_cz, (void*)p->d_ct, (void*)p->d_w,
+                    (void*)p->d_ones, (void*)p->d_seedz, (void*)p->d_alz64, (void*)p->d_seedz64,
                     (void*)p->d_ab64, (void*)p->d_ct64, (void*)p->d_seed64, (void*)p->d_trig,
                     (void*)p->d_fm})
     if (ptr) (void)hipFree(ptr);
@@ -451,24 +477,43 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
 
   hipError_t e = hipSuccess;
   const size_t nfm = (size_t)cfg->max_fields * (cfg->mmax + 1) * cfg->nlat;
-  if (e == hipSuccess) e = upload(&p->d_ab, reinterpret_cast<const float2*>(t.ab.data()),
-                                  (size_t)(p->nlm + kAbPad));
+  const size_t ntab = (size_t)(p->nlm + kAbPad);
   std::vector<float2> ct2(p->npair);
   for (int j = 0; j < p->npair; j++) ct2[j] = make_float2(t.ct[j], t.ct_lo[j]);
-  if (e == hipSuccess) e = upload(&p->d_ct, ct2.data(), (size_t)p->npair);
   if (e == hipSuccess) e = upload(&p->d_w, t.w.data(), (size_t)cfg->nlat);
   const std::vector<float> ones(cfg->nlat, 1.0f);
   if (e == hipSuccess) e = upload(&p->d_ones, ones.data(), (size_t)cfg->nlat);
-  if (e == hipSuccess) e = upload(&p->d_seed, t.seed.data(), t.seed.size());
-  size_t bytes64 = 0;
-  if (cfg->legendre == TSPHERE_LEG_BF16X3 || cfg->legendre == TSPHERE_LEG_F64REC) {
-    if (e == hipSuccess)
-      e = upload(&p->d_ab64, reinterpret_cast<const double2*>(t.ab64.data()),
-                 (size_t)(p->nlm + kAbPad));
+  size_t bytes_tab = 0;
+  if (cfg->legendre != TSPHERE_LEG_BF16X3) {  // c_l of the rescaled recurrence
+    std::vector<float> cz(ntab);
+    for (size_t i = 0; i < ntab; i++) cz[i] = (float)t.cz64[i];
+    if (e == hipSuccess) e = upload(&p->d_cz, cz.data(), ntab);
+    bytes_tab += sizeof(float) * ntab;
+  }
+  if (cfg->legendre == TSPHERE_LEG_FP32) {
+    std::vector<float> al(ntab);
+    for (size_t i = 0; i < ntab; i++) al[i] = (float)t.alz64[i];
+    std::vector<SeedL> sz(t.seedz64.size());
+    for (size_t i = 0; i < sz.size(); i++)
+      sz[i] = SeedL{(float)t.seedz64[i].y0, (float)t.seedz64[i].y1, t.seedz64[i].l, 0};
+    if (e == hipSuccess) e = upload(&p->d_alz, al.data(), ntab);
+    if (e == hipSuccess) e = upload(&p->d_ct, ct2.data(), (size_t)p->npair);
+    if (e == hipSuccess) e = upload(&p->d_seedz, sz.data(), sz.size());
+    bytes_tab += sizeof(float) * ntab + sizeof(float2) * p->npair + sizeof(SeedL) * sz.size();
+  } else {
     if (e == hipSuccess) e = upload(&p->d_ct64, t.x.data(), (size_t)p->npair);
+    bytes_tab += sizeof(double) * p->npair;
+  }
+  if (cfg->legendre == TSPHERE_LEG_F64REC) {
+    if (e == hipSuccess) e = upload(&p->d_alz64, t.alz64.data(), ntab);
+    if (e == hipSuccess) e = upload(&p->d_seedz64, t.seedz64.data(), t.seedz64.size());
+    bytes_tab += sizeof(double) * ntab + sizeof(SeedD) * t.seedz64.size();
+  }
+  if (cfg->legendre == TSPHERE_LEG_BF16X3) {
+    if (e == hipSuccess)
+      e = upload(&p->d_ab64, reinterpret_cast<const double2*>(t.ab64.data()), ntab);
     if (e == hipSuccess) e = upload(&p->d_seed64, t.seed64.data(), t.seed64.size());
-    bytes64 = sizeof(double2) * (p->nlm + kAbPad) + sizeof(double) * p->npair +
-              sizeof(SeedD) * t.seed64.size();
+    bytes_tab += sizeof(double2) * ntab + sizeof(SeedD) * t.seed64.size();
   }
   if (e == hipSuccess)
     e = upload(&p->d_trig, reinterpret_cast<const float2*>(t.trig.data()), (size_t)cfg->nphi);
@@ -478,9 +523,8 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     delete p;
     return hip_fail(e, "tsphere_plan_create");
   }
-  p->bytes = sizeof(float2) * (p->nlm + kAbPad + p->npair) + 2 * sizeof(float) * cfg->nlat +
-             sizeof(SeedL) * t.seed.size() + sizeof(float2) * cfg->nphi + sizeof(float2) * nfm +
-             bytes64;
+  p->bytes = bytes_tab + 2 * sizeof(float) * cfg->nlat + sizeof(float2) * cfg->nphi +
+             sizeof(float2) * nfm;
   *out = p;
   return ok();
 }
@@ -556,7 +600,7 @@ static Fields listed(const void* const* v, int n) {
 
 static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hipStream_t s) {
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
-    return launch_leg_synth_f64r(p->tables64(), nfield, q, fm, p->tile[0], s);
+    return launch_leg_synth_f64r(p->tablesz64(), nfield, q, fm, p->tile[0], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->tile[0], s);
   return launch_leg_synth(p->tables(), nfield, q, fm, p->tile[0], s);
@@ -565,7 +609,7 @@ static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hi
 static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, const Fields& q,
                              hipStream_t s, bool unit_w = false) {
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
-    return launch_leg_analys_f64r(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
+    return launch_leg_analys_f64r(p->tablesz64(unit_w), nfield, fm, q, p->tile[1], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_analys_mfma(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
   return launch_leg_analys(p->tables(unit_w), nfield, fm, q, p->tile[1], s);
@@ -923,8 +967,8 @@ int tsphere_plan_tune(tsphere_plan p, int nfield, int reps, tsphere_stream strea
           return stage == 0 ? launch_leg_synth_mfma(p->tables64(), nfield, F, p->d_fm, t, s)
                             : launch_leg_analys_mfma(p->tables64(), nfield, p->d_fm, F, t, s);
         if (p->cfg.legendre == TSPHERE_LEG_F64REC)
-          return stage == 0 ? launch_leg_synth_f64r(p->tables64(), nfield, F, p->d_fm, t, s)
-                            : launch_leg_analys_f64r(p->tables64(), nfield, p->d_fm, F, t, s);
+          return stage == 0 ? launch_leg_synth_f64r(p->tablesz64(), nfield, F, p->d_fm, t, s)
+                            : launch_leg_analys_f64r(p->tablesz64(), nfield, p->d_fm, F, t, s);
         return stage == 0 ? launch_leg_synth(p->tables(), nfield, F, p->d_fm, t, s)
                           : launch_leg_analys(p->tables(), nfield, p->d_fm, F, t, s);
       };
