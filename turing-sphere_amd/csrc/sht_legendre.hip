@@ -43,7 +43,7 @@ namespace tsphere {
 namespace {
 
 constexpr int kTile = 4;   // l-values between seed injections (seeds are 4-aligned)
-constexpr int kPadQ = 8;   // zero entries after a staged row (reads run past lmax)
+constexpr int kPadQ = 24;  // zero entries after a staged row (reads run past lmax)
 constexpr int kPadAnalys = 40;  // analysis spans read up to 32 + 3 entries past lmax
 
 __device__ __forceinline__ long long lm_base(int lmax, int m) {
@@ -913,23 +913,33 @@ __device__ __forceinline__ float qcol(const float* sqf, int i, int fp, int j) {
   }
 }
 
-// NT l-values (4 or 8) of chains [K0, K) from l = m + i; the tile's LDS reads
-// are issued before its first product, so a longer tile hides more of their
-// latency.
+// Operands of NT l-values (4 or 8) from l = m + i: the B columns of Q and
+// the recurrence's alpha.
+template <int NT, int NF, class P>
+struct TileOps {
+  float bq[NT][(NF + 1) / 2];
+  typename P::ab_t al[NT];
+  __device__ __forceinline__ void load(const typename P::ab_t* gal, const float* sqf, int i, int j) {
+#pragma unroll
+    for (int p = 0; p < NT; ++p) {
+#pragma unroll
+      for (int fp = 0; fp < (NF + 1) / 2; ++fp) bq[p][fp] = qcol<NF>(sqf, i + p, fp, j);
+      al[p] = gal[i + p + 2];
+    }
+  }
+};
+
+// NT l-values of chains [K0, K) from l on, with operands already loaded.
+// The recurrence of the whole tile runs first and its products follow as
+// one group: on gfx950 the block products and the VALU do not overlap, and
+// every switch between the two costs extra issue time (measured: 8 products
+// and 24 FMAs take 94 ns interleaved, 71 ns grouped, per SIMD).
 template <int NT, int K0, bool INJ, int NF, int K, class P>
-__device__ __forceinline__ void synth_tile_m4(const typename P::ab_t* sab, const float* sqf, int i,
-                                              int l, int j, ChainT<P> (&c)[K],
+__device__ __forceinline__ void synth_tile_m4(const TileOps<NT, NF, P>& o, int l, ChainT<P> (&c)[K],
                                               f4v (&accE)[K][(NF + 1) / 2],
                                               f4v (&accO)[K][(NF + 1) / 2]) {
   constexpr int FP = (NF + 1) / 2;
-  float bq[NT][FP];
-  typename P::ab_t al[NT];
-#pragma unroll
-  for (int p = 0; p < NT; ++p) {
-#pragma unroll
-    for (int fp = 0; fp < FP; ++fp) bq[p][fp] = qcol<NF>(sqf, i + p, fp, j);
-    al[p] = sab[i + p + 2];
-  }
+  float zv[NT][K];
 #pragma unroll
   for (int p = 0; p < NT; p += 2) {
     if constexpr (INJ) {
@@ -944,15 +954,32 @@ __device__ __forceinline__ void synth_tile_m4(const typename P::ab_t* sab, const
     }
 #pragma unroll
     for (int k = K0; k < K; ++k) {
-      const float y0 = (float)c[k].y0, y1 = (float)c[k].y1;
-#pragma unroll
-      for (int fp = 0; fp < FP; ++fp) {
-        accE[k][fp] = mf441(y0, bq[p][fp], accE[k][fp]);
-        accO[k][fp] = mf441(y1, bq[p + 1][fp], accO[k][fp]);
-      }
-      P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
+      zv[p][k] = (float)c[k].y0;
+      zv[p + 1][k] = (float)c[k].y1;
+      P::advance(o.al[p], o.al[p + 1], c[k].ct, c[k].y0, c[k].y1);
     }
   }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int p = 0; p < NT; p += 2)
+#pragma unroll
+    for (int k = K0; k < K; ++k)
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) {
+        accE[k][fp] = mf441(zv[p][k], o.bq[p][fp], accE[k][fp]);
+        accO[k][fp] = mf441(zv[p + 1][k], o.bq[p + 1][fp], accO[k][fp]);
+      }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int NT, int K0, bool INJ, int NF, int K, class P>
+__device__ __forceinline__ void synth_tile_m4_at(const typename P::ab_t* gal, const float* sqf,
+                                                 int m, int l, int j, ChainT<P> (&c)[K],
+                                                 f4v (&accE)[K][(NF + 1) / 2],
+                                                 f4v (&accO)[K][(NF + 1) / 2]) {
+  TileOps<NT, NF, P> o;
+  o.load(gal, sqf, l - m, j);
+  synth_tile_m4<NT, K0, INJ, NF, K, P>(o, l, c, accE, accO);
 }
 
 // The phase structure of synth_phases (chains [K0, K) until a lower chain
@@ -960,7 +987,7 @@ __device__ __forceinline__ void synth_tile_m4(const typename P::ab_t* sab, const
 // phase's end, a 4-l tile for the rest (phase ends are 4-aligned in l - m).
 template <int K0, int NF, int K, class P>
 __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, const int (&lk)[K],
-                                                const int (&lz)[K], const typename P::ab_t* sab,
+                                                const int (&lz)[K], const typename P::ab_t* gal,
                                                 const float* sqf, ChainT<P> (&c)[K],
                                                 f4v (&accE)[K][(NF + 1) / 2],
                                                 f4v (&accO)[K][(NF + 1) / 2]) {
@@ -972,19 +999,19 @@ __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, cons
   const int l1 = min(lend, injend);
   while (l < l1) {
     if (l + 8 <= lend) {
-      synth_tile_m4<8, K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+      synth_tile_m4_at<8, K0, true, NF, K, P>(gal, sqf, m, l, j, c, accE, accO);
       l += 8;
     } else {
-      synth_tile_m4<4, K0, true, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+      synth_tile_m4_at<4, K0, true, NF, K, P>(gal, sqf, m, l, j, c, accE, accO);
       l += 4;
     }
   }
   for (; l + 8 <= lend; l += 8)
-    synth_tile_m4<8, K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+    synth_tile_m4_at<8, K0, false, NF, K, P>(gal, sqf, m, l, j, c, accE, accO);
   for (; l < lend; l += 4)
-    synth_tile_m4<4, K0, false, NF, K, P>(sab, sqf, l - m, l, j, c, accE, accO);
+    synth_tile_m4_at<4, K0, false, NF, K, P>(gal, sqf, m, l, j, c, accE, accO);
   if constexpr (K0 > 0)
-    synth_phases_m4<K0 - 1, NF, K, P>(l, L, m, j, lk, lz, sab, sqf, c, accE, accO);
+    synth_phases_m4<K0 - 1, NF, K, P>(l, L, m, j, lk, lz, gal, sqf, c, accE, accO);
 }
 
 // Synthesis: per chain and field pair, D_E / D_O (E: even l - m, O: odd)
@@ -1077,8 +1104,9 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
                                                        const float2* __restrict__ fm, Fields Fq,
                                                        int rows) {
   constexpr int FP = (NF + 1) / 2;
+  constexpr int M4_NH = FP == 1 ? 4 : 2;  // spans per workgroup barrier (16 KB of red)
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  __shared__ float red[2][8][64 * FP];
+  __shared__ float red[2][8][M4_NH * 64 * FP];
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
@@ -1161,14 +1189,27 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
           reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
               make_float2(0.f, 0.f);
 
+    // Spans go M4_NH at a time between workgroup barriers: each wave reduces
+    // its half-spans into red, then all threads combine the waves' sums.
     int buf = 0;
-    for (int ls = lblk; ls <= L; ls += M4_SPAN) {
-      if (lmin >= ls + M4_SPAN) {
-        // every chain of this wave is still exactly 0 in this span
+    for (int ls0 = lblk; ls0 <= L; ls0 += M4_NH * M4_SPAN) {
 #pragma unroll
-        for (int fp = 0; fp < FP; ++fp) red[buf][wv][lane * FP + fp] = 0.f;
-      } else {
-        // 1) each live chain's 16 values of the span -> column `lane` of its tile
+      for (int h = 0; h < M4_NH; ++h) {
+        const int ls = ls0 + h * M4_SPAN;
+        float* rw = &red[buf][wv][h * 64 * FP];
+        if (lmin >= ls + M4_SPAN || ls > L) {
+          // every chain of this wave is still exactly 0 in this span (or the
+          // span is past lmax)
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
+          continue;
+        }
+        // 1) each live chain's 16 values of the span -> column `lane` of its
+        // tile.  The span's alpha are read first, in one batch: read between
+        // the tile stores, each would wait for the stores before it.
+        typename P::ab_t al[M4_SPAN];
+#pragma unroll
+        for (int p = 0; p < M4_SPAN; ++p) al[p] = sab[ls - m + p + 2];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (lk[k] >= ls + M4_SPAN) continue;  // chain still 0 (wave-uniform)
@@ -1182,8 +1223,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
             }
             tk[p * M4_TROW] = (float)c[k].y0;
             tk[(p + 1) * M4_TROW] = (float)c[k].y1;
-            const auto a0 = sab[l - m + 2], a1 = sab[l - m + 3];
-            P::advance(a0, a1, c[k].ct, c[k].y0, c[k].y1);
+            P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
           }
         }
         wave_sync_lds();
@@ -1192,20 +1232,27 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
         const int arow = m4_off(lg, j) * M4_TROW + 16 * pg;  // lane j supplies row j
+        // the products as one group (no VALU work scheduled between them: on
+        // gfx950 they do not overlap it, and every switch costs issue time)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (lk[k] >= ls + M4_SPAN) continue;
           const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
+          float4 a4[4];
 #pragma unroll
-          for (int t = 0; t < 16; t += 4) {
-            const float4 a4 = *reinterpret_cast<const float4*>(tk + t);
-            const float a[4] = {a4.x, a4.y, a4.z, a4.w};
+          for (int t = 0; t < 4; ++t) a4[t] = *reinterpret_cast<const float4*>(tk + 4 * t);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float a[4] = {a4[t].x, a4[t].y, a4[t].z, a4[t].w};
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-              for (int fp = 0; fp < FP; ++fp) acc[fp] = mf441(a[u], W[k][t + u][fp], acc[fp]);
+              for (int fp = 0; fp < FP; ++fp)
+                acc[fp] = mf441(a[u], W[k][4 * t + u][fp], acc[fp]);
           }
         }
+        __builtin_amdgcn_sched_barrier(0);
         wave_sync_lds();  // the tiles are rewritten in the next span
         // 3) sum over the 4 pair-groups: lane bit 5 keeps rows {0,1} / {2,3},
         // lane bit 4 row {0} / {1} of those: lane 4b + j ends with row pg
@@ -1220,22 +1267,22 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
           }
           const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
                                                            __float_as_uint(v[1]), false, false);
-          red[buf][wv][lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
+          rw[lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
         }
       }
       __syncthreads();
-      if (wv == 0) {
-        const int l = ls + m4_off(lg, pg);
-#pragma unroll
-        for (int fp = 0; fp < FP; ++fp) {
-          float s = red[buf][0][lane * FP + fp];
-          for (int w = 1; w < nwaves; ++w) s += red[buf][w][lane * FP + fp];
-          const int f = 2 * fp + (j >> 1);
-          if (l <= L && f < NF && f0 + f < nfield) {
-            float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
-                         2 * (base + (l - m));
-            dst[j & 1] = s * T.cz[base + (l - m)];
-          }
+      // 4) the waves' sums, in wave order, one output per thread
+      for (int o = tid; o < M4_NH * 64 * FP; o += nthr) {
+        const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
+        const int jj = ln & 3, bb = ln >> 2;
+        const int l = ls0 + h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
+        const int f = 2 * fp + (jj >> 1);
+        if (l <= L && f < NF && f0 + f < nfield) {
+          float s = red[buf][0][o];
+          for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
+          float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
+                       2 * (base + (l - m));
+          dst[jj & 1] = s * T.cz[base + (l - m)];
         }
       }
       buf ^= 1;

@@ -168,7 +168,7 @@ struct HostTables {
   std::vector<float> trig;     // 2*nphi
   std::vector<double> x, wg;   // f64 Gauss nodes / weights
 };
-constexpr int kAbPad = 4;
+constexpr int kAbPad = 40;  // zero entries after the last row (kernels read alpha past a row)
 void build_host_tables(const tsphere_sht_config& cfg, HostTables& t);
 long long nlm_calc(int lmax, int mmax);
 
