@@ -427,7 +427,9 @@ def main() -> int:
         "batched": extras,
         "roofline": {"bound": "mfma", "kernel": dom,
                      "pipe": ("bf16 MFMA (split x3)" if args.legendre == "bf16x3"
-                              else "fp32 FMA on the vector ALU (157.3 TF peak = packed fp32)"), "achieved": round(achieved / 1e12, 3),
+                              else "fp32: recurrence on the vector ALU, sums on the fp32 matrix "
+                                   "cores as 4x4 blocks (157.3 TF peak, matrix = packed vector)"),
+                     "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
                      "flops_per_launch": 2 * f_leg},
