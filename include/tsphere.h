@@ -188,6 +188,22 @@ int tsphere_leg_synth_fields(tsphere_plan p, int nptr, int per, const float *con
                              long long qlm_stride, float *fm, tsphere_stream s);
 int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float *fm,
                               float *const *qlm, long long qlm_stride, tsphere_stream s);
+/* The spectral op: per m-row, the Legendre analysis of the scratch fm
+ * (nfield fields, as tsphere_leg_analys_multi), the diagonal implicit step
+ *   X_new = (X + s1[f] A_f) / (1 + s2[f] lam)     per real coefficient
+ * (bench/shtns/spec.h:166-169; the last lines of the models' .m files,
+ * Un = (U + dt*analys(..))
+ * ./ (1 + (dt*D)*lam), rounded as written) of state field species[f] in
+ * place, and the Legendre synthesis of the new state into fm (field slot =
+ * state index), ready for the next step's grid kernel: the analysis of step
+ * n and the synthesis of step n+1 share one generation of the recurrence.
+ * A(l,m) below every lane's significance point is 0, as in the analysis.
+ * FP32 and FP64-recurrence plans, nfield <= 4, up to 1024 latitude pairs.
+ * No reference counterpart of its own (it fuses encodeAnalysInto, the
+ * implicit-solve dispatch and encodeSynthInto, plan.ts:436-478). */
+int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float *fm, float *const *state,
+                              const float *lam, const float *s1, const float *s2,
+                              const int *species, tsphere_stream s);
 /* The plan's device (cos, sin)(2 pi k / nphi) table, 2*nphi floats, which
  * the row FFT of a generated grid kernel must read to round as the plan's
  * Fourier stages do. */

@@ -29,15 +29,17 @@ DEFAULTS = {
 }
 
 
-def make(key, lmax, nlat, nphi, params, batch=True, grid=True):
+def make(key, lmax, nlat, nphi, params, batch=True, grid=True, spectral=False,
+         legendre="fp32"):
     import tsphere
     from tsphere.model import GpuModel
 
     cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
     names, state, view = MODELS[key]
-    sht = tsphere.ShtPlan.create(cfg, max_fields=len(state))
+    sht = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=len(state))
     src = open(os.path.join(MFILES, key + ".m")).read()
-    gm = GpuModel.create(sht, cfg, src, names, state, view, batch=batch, grid=grid)
+    gm = GpuModel.create(sht, cfg, src, names, state, view, batch=batch, grid=grid,
+                         spectral=spectral)
     gm.set_params(params)
     return sht, gm
 
@@ -278,6 +280,47 @@ def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nl
             gm.upload(s, q)
         gm.step(20)
         outs.append([gm.read(s) for s in gm.state] + [gm.read(v) for v in gm.view])
+        gm.destroy()
+        sht.destroy()
+    for a, b in zip(*outs):
+        assert np.all(np.isfinite(b))
+        d = float(np.max(np.abs(a - b)) / max(1e-30, float(np.max(np.abs(a)))))
+        emit(f"  max rel diff {d:.2e}")
+        assert np.array_equal(a, b), d
+
+
+@pytest.mark.parametrize("key,lmax,nlat,nphi,legendre", [
+    ("brusselator", 31, 32, 64, "fp32"), ("schnakenberg", 15, 17, 32, "fp32"),
+    ("allencahn", 63, 64, 128, "fp32"), ("brusselator", 127, 128, 256, "f64rec"),
+    ("schnakenberg", 255, 256, 512, "fp32"), ("brusselator", 255, 257, 512, "f64rec")])
+def test_spectral_op_matches_grid_plan(gpu, key, lmax, nlat, nphi, legendre):
+    """The spectral op (Legendre analysis, the implicit solve and the next
+    step's Legendre synthesis in one kernel per step) gives the state and the
+    returned grid values of the plan that runs them as separate launches,
+    bit for bit: the same recurrence values, the same order of every sum and
+    the implicit step rounded as the generated kernel rounds it.  Steps run
+    as graph replays and as directly encoded remainders, over several calls
+    (each call re-synthesises from the state, so a state upload between
+    calls is honoured)."""
+    emit(f"spectral op {key} lmax {lmax} {nlat}x{nphi} {legendre}")
+    outs = []
+    for spectral in (False, True):
+        sht, gm = make(key, lmax, nlat, nphi, DEFAULTS[key], spectral=spectral,
+                       legendre=legendre)
+        kinds = [l.split()[0] for l in gm.describe()["step"]]
+        assert ("spectral" in kinds) == spectral, kinds
+        gm.graph_steps = 4
+        rng = np.random.default_rng(11)
+        for s in gm.state:
+            q = (rng.normal(size=2 * sht.nlm) * 0.02).astype(np.float32)
+            q[0] += 1.0
+            gm.upload(s, q)
+        gm.step(9)
+        gm.step(3)
+        mid = [gm.read(s) for s in gm.state]
+        gm.upload(gm.state[0], mid[0] * np.float32(0.5))
+        gm.step(6)
+        outs.append(mid + [gm.read(s) for s in gm.state] + [gm.read(v) for v in gm.view])
         gm.destroy()
         sht.destroy()
     for a, b in zip(*outs):

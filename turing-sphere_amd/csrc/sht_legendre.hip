@@ -102,19 +102,33 @@ __device__ __forceinline__ int first_live(const int (&lk)[K], int l) {
 // (rows = 2): workgroup b takes m = b and M - b, whose l-loops sum to
 // lmax + 2, laid out XCD by XCD.  Single (rows = 1): each workgroup takes one
 // row -- twice the workgroups, each with half the work, so more waves are
-// resident when the field count is small.  Rows go to the XCDs in groups of
-// 16 consecutive m (one 128-byte line of the Fourier scratch holds 16 m of a
-// latitude, so a group shares its lines in one L2), the groups dealt
-// boustrophedon over the 8 XCDs so that each gets long and short rows alike;
-// long rows are dispatched first.  (Workgroups go to XCD b % 8 in turn.)
-__device__ __forceinline__ int row_m(int r, int rows, int M) {
+// resident when the field count is small.  Workgroups go to XCD b % 8 in
+// turn and, in the first pass, to the XCD's CUs in turn, so the t-th
+// workgroup of an XCD shares a CU with the (t + C)-th, (t + 2C)-th and
+// (t + 3C)-th (C = a quarter of the XCD's workgroups).  The quarters take the
+// rows u, n - 1 - u, n/4 + u and 3n/4 - 1 - u (u = x C + t mod C on XCD x):
+// two complementary pairs, so every CU gets the same l-length in all, and
+// consecutive u on one XCD are consecutive m (a 128-byte line of the Fourier
+// scratch holds 16 m of a latitude, shared in one L2).  The longest rows are
+// dispatched first.
+// (Kernels with fewer resident workgroups per CU than four -- the analysis,
+// whose LDS tiles allow two -- keep the older dealing, quarters = false: rows
+// in groups of 16 consecutive m dealt boustrophedon over the XCDs, which
+// measured slightly ahead there.)
+__device__ __forceinline__ int row_m(int r, int rows, int M, bool quarters = true) {
   if (rows == 1) {
     if (r != 0) return -1;
     const int b = blockIdx.x, n = gridDim.x;
-    if (n % 256 != 0) return b;  // fewer than two groups per XCD: no balance
-    const int x = b & 7, t = b >> 3;             // XCD, position on it
-    const int gi = t >> 4, g = gi * 8 + ((gi & 1) ? 7 - x : x);
-    return g * 16 + (t & 15);
+    const int x = b & 7, t = b >> 3;  // XCD, position on it
+    if (!quarters) {
+      if (n % 256 != 0) return b;  // fewer than two groups per XCD: no balance
+      const int gi = t >> 4, g = gi * 8 + ((gi & 1) ? 7 - x : x);
+      return g * 16 + (t & 15);
+    }
+    if (n % 32 != 0) return b;  // too few workgroups to deal in quarters
+    const int C = n >> 5, Q = n >> 2;
+    const int q = t / C, u = x * C + (t - q * C);
+    return q == 0 ? u : q == 1 ? n - 1 - u : q == 2 ? Q + u : 3 * Q - 1 - u;
   }
   const int mp = xcd_chunk(blockIdx.x, gridDim.x);
   const int m = r == 0 ? mp : M - mp;
@@ -429,7 +443,7 @@ __global__ __launch_bounds__(256) void k_leg_analys(typename P::Tables T, int nf
   const int nthr = blockDim.x, nwaves = nthr >> 6;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = row_m(r, rows, M);
+    const int m = row_m(r, rows, M, false);
     if (m < 0) break;
     const long long base = lm_base(L, m);
     __syncthreads();
@@ -1120,7 +1134,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
   const int f0 = blockIdx.y * NF;
 
   for (int r = 0; r < 2; ++r) {
-    const int m = row_m(r, rows, M);
+    const int m = row_m(r, rows, M, false);
     if (m < 0) break;
     const long long base = lm_base(L, m);
     __syncthreads();
@@ -1288,6 +1302,303 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
       buf ^= 1;
     }
   }
+}
+
+// ------------------------------------------------------- the spectral op
+// Legendre analysis of step n, the diagonal implicit solve and the Legendre
+// synthesis of step n + 1 in one pass over each m-row: the recurrence value
+// of every (l, m, latitude pair) is generated once and used by both
+// contractions.  Per row, as k_leg_analys_m4 up to the waves' combined sums
+// A(l) of a group of spans; then, per coefficient (bench/shtns/spec.h:
+// 166-169's implicit step, the reference models' last lines)
+//   X_new = (X + s1 A) / (1 + s2 lam)     (each product and sum rounded as
+//                                          written: the generated kernel's
+//                                          arithmetic)
+// written to the state and, scaled by c_l, to LDS; then the synthesis block
+// products of those l with the chain values the wave kept in registers (the
+// synthesis of k_leg_synth_m4, in the same order over l).  The row's Fourier
+// scratch entries are read at the row's start and its synthesis written at
+// its end, so the scratch is updated in place.
+constexpr int SP_NH = 2;  // spans per workgroup barrier
+
+// The implicit step of one real coefficient, each product and sum rounded
+// as written (the generated kernel's (X + (s1 * A)) / (1 + (s2 * lam))).
+__device__ __forceinline__ float imex_step(float x, float s1, float a, float s2, float lam) {
+#pragma clang fp contract(off)
+  return (x + (s1 * a)) / (1.0f + (s2 * lam));
+}
+
+template <int NF, int K, class P>
+__global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, int nfield,
+                                                         float2* fm, Fields St,
+                                                         const float* __restrict__ lam,
+                                                         ImexCoef ic, int rows) {
+  constexpr int FP = (NF + 1) / 2;
+  constexpr int NO = SP_NH * 64 * FP;  // combined sums per span group
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ float red[2][8][NO];
+  __shared__ float qs[SP_NH * M4_SPAN][4 * FP];  // c_l X_new, (l, species, re/im)
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
+  const int j = lane & 3, blk = lane >> 2;
+  const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
+  const bool odd = lg >> 1;
+  float* tile = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1)) +
+                wv * (K * M4_SPAN * M4_TROW);
+
+  for (int r = 0; r < 2; ++r) {
+    const int m = row_m(r, rows, M, false);
+    if (m < 0) break;
+    const long long base = lm_base(L, m);
+    __syncthreads();
+    stage_row_t<NF, P>(T, m, 0, nfield, Fields{}, false, sab, nullptr, kPadAnalys);
+
+    ChainT<P> c[K];
+    int lk[K];
+    int lzmax = -1, lmin = 0x7fffffff;
+    float W[K][16][FP];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      load_chain_t<P>(T, m, pair_of(k), c[k]);
+      lk[k] = wave_min(c[k].ls);
+      lzmax = max(lzmax, last_seed(c[k].ls, L));
+      lmin = min(lmin, lk[k]);
+      const int jp = pair_of(k);
+      const int jc = min(jp, T.npair - 1);
+      const int js = nlat - 1 - jc;
+      const float w = jp < T.npair ? T.w[jc] : 0.f;
+      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane * (8 * FP);
+#pragma unroll
+      for (int f = 0; f < 2 * FP; ++f) {
+        const bool live = f < NF && f < nfield;
+        const int fld = min(min(f, NF - 1), nfield - 1);
+        const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
+        const float2 gN = g[(long long)jc * (M + 1)], gS = g[(long long)js * (M + 1)];
+        float4 we{0.f, 0.f, 0.f, 0.f};
+        if (live) {
+          if (js == jc)
+            we = make_float4(gN.x * w, gN.y * w, 0.f, 0.f);
+          else
+            we = make_float4((gN.x + gS.x) * w, (gN.y + gS.y) * w, (gN.x - gS.x) * w,
+                             (gN.y - gS.y) * w);
+        }
+        const int fp = f >> 1, c0 = 2 * (f & 1);
+        tw[(0 * FP + fp) * 4 + c0] = we.x;
+        tw[(0 * FP + fp) * 4 + c0 + 1] = we.y;
+        tw[(1 * FP + fp) * 4 + c0] = we.z;
+        tw[(1 * FP + fp) * 4 + c0 + 1] = we.w;
+      }
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp)
+          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) + (16 * pg + t) * (8 * FP) +
+                             ((odd ? FP : 0) + fp) * 4 + j];
+    wave_sync_lds();
+    if (lane == 0) smin[wv] = lmin;
+    __syncthreads();  // also publishes the staged row; every W is read
+    int lblk = smin[0];
+    for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+    lblk = min(lblk, L + 1);
+
+    // below every lane's significance point the analysed values are exactly
+    // 0: the implicit step alone (and no synthesis contribution)
+    for (int o = tid; o < (lblk - m) * 2 * NF; o += nthr) {
+      const int f = o % NF, q = o / NF;
+      if (f >= nfield) continue;
+      const long long i = 2 * base + q;
+      float* X = reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[f])));
+      X[i] = imex_step(X[i], ic.s1[f], 0.0f, ic.s2[f], lam[i]);
+    }
+
+    f4v accE[K][FP], accO[K][FP];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) accE[k][fp] = accO[k][fp] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    int buf = 0;
+    for (int ls0 = lblk; ls0 <= L; ls0 += SP_NH * M4_SPAN) {
+      float zk[SP_NH][K][M4_SPAN];  // the chains' values, kept for the synthesis
+      bool live[SP_NH];
+#pragma unroll
+      for (int h = 0; h < SP_NH; ++h) {
+        const int ls = ls0 + h * M4_SPAN;
+        float* rw = &red[buf][wv][h * 64 * FP];
+        live[h] = !(lmin >= ls + M4_SPAN || ls > L);
+        if (!live[h]) {
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
+          continue;
+        }
+        typename P::ab_t al[M4_SPAN];
+#pragma unroll
+        for (int p = 0; p < M4_SPAN; ++p) al[p] = sab[ls - m + p + 2];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
+#pragma unroll
+          for (int p = 0; p < M4_SPAN; p += 2) {
+            const int l = ls + p;
+            if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+            zk[h][k][p] = (float)c[k].y0;
+            zk[h][k][p + 1] = (float)c[k].y1;
+            tk[p * M4_TROW] = zk[h][k][p];
+            tk[(p + 1) * M4_TROW] = zk[h][k][p + 1];
+            P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
+          }
+        }
+        wave_sync_lds();
+        f4v acc[FP];
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+        const int arow = m4_off(lg, j) * M4_TROW + 16 * pg;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
+          float4 a4[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) a4[t] = *reinterpret_cast<const float4*>(tk + 4 * t);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float a[4] = {a4[t].x, a4[t].y, a4[t].z, a4[t].w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int fp = 0; fp < FP; ++fp)
+                acc[fp] = mf441(a[u], W[k][4 * t + u][fp], acc[fp]);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        wave_sync_lds();
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) {
+          float v[4] = {acc[fp][0], acc[fp][1], acc[fp][2], acc[fp][3]};
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
+                                                             __float_as_uint(v[i + 2]), false, false);
+            v[i] = __uint_as_float(s2[0]) + __uint_as_float(s2[1]);
+          }
+          const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
+                                                           __float_as_uint(v[1]), false, false);
+          rw[lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
+        }
+      }
+      __syncthreads();
+      // the waves' sums (in wave order) -> analysed value -> implicit step
+      for (int o = tid; o < NO; o += nthr) {
+        const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
+        const int jj = ln & 3, bb = ln >> 2;
+        const int lo = h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
+        const int l = ls0 + lo;
+        const int f = 2 * fp + (jj >> 1);
+        float qv = 0.f;
+        if (l <= L && f < NF && f < nfield) {
+          float s = red[buf][0][o];
+          for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
+          const float cz = T.cz[base + (l - m)];
+          const float a = s * cz;
+          const long long i = 2 * (base + (l - m)) + (jj & 1);
+          float* X = reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[f])));
+          const float xn = imex_step(X[i], ic.s1[f], a, ic.s2[f], lam[i]);
+          X[i] = xn;
+          qv = xn * cz;
+        }
+        if (f < NF) qs[lo][2 * (f < nfield ? ic.spc[f] : f) + (jj & 1)] = qv;
+      }
+      __syncthreads();
+      // synthesis block products of the group's l, in order of l
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int h = 0; h < SP_NH; ++h) {
+        if (!live[h]) continue;  // every chain value is still 0 (wave-uniform)
+#pragma unroll
+        for (int p = 0; p < M4_SPAN; p += 2) {
+          float bE[FP], bO[FP];
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) {
+            const float* qa = &qs[h * M4_SPAN + p][4 * fp];
+            const float* qb = &qs[h * M4_SPAN + p + 1][4 * fp];
+            bE[fp] = (NF == 1 && j >= 2) ? 0.f : qa[j];
+            bO[fp] = (NF == 1 && j >= 2) ? 0.f : qb[j];
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int fp = 0; fp < FP; ++fp) {
+              accE[k][fp] = mf441(zk[h][k][p], bE[fp], accE[k][fp]);
+              accO[k][fp] = mf441(zk[h][k][p + 1], bO[fp], accO[k][fp]);
+            }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      buf ^= 1;
+    }
+
+    // the synthesis of the row into the Fourier scratch (north = E + O,
+    // south = E - O), species s in field slot s
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int pb = pair_of(k) - lane + 4 * blk;
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) {
+        const int s = 2 * fp + (j >> 1);
+        if (s >= NF || s >= nfield) continue;
+        float* out = reinterpret_cast<float*>(fm + (long long)s * nlat * (M + 1) + m) + (j & 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int jp = pb + q;
+          if (jp >= T.npair) continue;
+          const float e = accE[k][fp][q], o = accO[k][fp][q];
+          out[2LL * jp * (M + 1)] = e + o;
+          out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+        }
+      }
+    }
+  }
+}
+
+template <int NF, int K, class P>
+hipError_t spectral_m4(const typename P::Tables& T, int nfield, int nwaves, float2* fm,
+                       const Fields& st, const float* lam, const ImexCoef& ic, hipStream_t s) {
+  dim3 grid(T.mmax + 1, 1);
+  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+                     sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
+  hipLaunchKernelGGL((k_leg_spectral_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
+                     st, lam, ic, 1);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t spectral_launch(const typename P::Tables& T, int nfield, float2* fm, const Fields& st,
+                           const float* lam, const ImexCoef& ic, hipStream_t s) {
+  if (nfield < 1 || nfield > 4) return hipErrorInvalidValue;
+  // one chain per wave when it covers the pairs, else two
+  const int npair = T.npair;
+  const int k = npair <= 64 * 8 ? 1 : npair <= 64 * 8 * 2 ? 2 : 0;
+  if (k == 0) return hipErrorInvalidValue;
+  const int nwaves = std::min(8, (npair + 64 * k - 1) / (64 * k));
+  if (nfield <= 2) {
+    if (nfield == 1)
+      return k == 1 ? spectral_m4<1, 1, P>(T, nfield, nwaves, fm, st, lam, ic, s)
+                    : spectral_m4<1, 2, P>(T, nfield, nwaves, fm, st, lam, ic, s);
+    return k == 1 ? spectral_m4<2, 1, P>(T, nfield, nwaves, fm, st, lam, ic, s)
+                  : spectral_m4<2, 2, P>(T, nfield, nwaves, fm, st, lam, ic, s);
+  }
+  return k == 1 ? spectral_m4<4, 1, P>(T, nfield, nwaves, fm, st, lam, ic, s)
+                : spectral_m4<4, 2, P>(T, nfield, nwaves, fm, st, lam, ic, s);
 }
 
 template <int NF, int K, class P>
@@ -1465,6 +1776,17 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
     case 2: return analys_nf<2, RecD>(T, nfield, t, fm, q, s);
     default: return analys_nf<4, RecD>(T, nfield, t, fm, q, s);
   }
+}
+
+hipError_t launch_leg_spectral(const LegTables& T, int nfield, float2* fm, const Fields& st,
+                               const float* lam, const ImexCoef& ic, hipStream_t s) {
+  return spectral_launch<RecF>(T, nfield, fm, st, lam, ic, s);
+}
+
+hipError_t launch_leg_spectral_f64r(const LegTablesZ64& T, int nfield, float2* fm,
+                                    const Fields& st, const float* lam, const ImexCoef& ic,
+                                    hipStream_t s) {
+  return spectral_launch<RecD>(T, nfield, fm, st, lam, ic, s);
 }
 
 }  // namespace tsphere

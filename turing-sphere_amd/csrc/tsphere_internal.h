@@ -186,6 +186,20 @@ hipError_t launch_leg_synth_mfma(const LegTables64& T, int nfield, const Fields&
                                  LegTile tile, hipStream_t s);
 hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2* fm,
                                   const Fields& q, LegTile tile, hipStream_t s);
+// The spectral op: Legendre analysis of the Fourier scratch (nfield fields),
+// the diagonal implicit step X_new = (X + s1 A) / (1 + s2 lam) of state field
+// spc[f] for analysed field f, and the Legendre synthesis of the new state
+// into the same scratch (field slot = state index).  FP32 and FP64-recurrence
+// forms; up to 4 fields.
+struct ImexCoef {
+  float s1[4], s2[4];  // per analysed field
+  int spc[4];          // the state field (species) each analysed field updates
+};
+hipError_t launch_leg_spectral(const LegTables& T, int nfield, float2* fm, const Fields& st,
+                               const float* lam, const ImexCoef& ic, hipStream_t s);
+hipError_t launch_leg_spectral_f64r(const LegTablesZ64& T, int nfield, float2* fm,
+                                    const Fields& st, const float* lam, const ImexCoef& ic,
+                                    hipStream_t s);
 // The FP64-recurrence form: the VALU kernels with an f64 recurrence.
 hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
                                  LegTile tile, hipStream_t s);

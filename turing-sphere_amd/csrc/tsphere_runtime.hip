@@ -733,6 +733,39 @@ int tsphere_leg_analys_multi(tsphere_plan p, int nfield, const float* fm, float*
   return ok();
 }
 
+int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float* fm, float* const* state,
+                              const float* lam, const float* s1, const float* s2,
+                              const int* species, tsphere_stream s) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1 || nfield > 4 || nfield > p->cfg.max_fields)
+    return fail(TSPHERE_ERR_INVALID, "nfield=%d: need 1..4 and <= max_fields=%d", nfield,
+                p->cfg.max_fields);
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    return fail(TSPHERE_ERR_INVALID, "the spectral op runs the FP32 and FP64-recurrence forms");
+  if (!fm || !state || !lam || !s1 || !s2 || !species)
+    return fail(TSPHERE_ERR_INVALID, "NULL argument");
+  ImexCoef ic{};
+  bool seen[4] = {false, false, false, false};
+  for (int f = 0; f < nfield; f++) {
+    if (!state[f]) return fail(TSPHERE_ERR_INVALID, "NULL state buffer %d", f);
+    if (species[f] < 0 || species[f] >= nfield || seen[species[f]])
+      return fail(TSPHERE_ERR_INVALID, "species must be a permutation of 0..%d", nfield - 1);
+    seen[species[f]] = true;
+    ic.s1[f] = s1[f];
+    ic.s2[f] = s2[f];
+    ic.spc[f] = species[f];
+  }
+  if (p->npair > 64 * 8 * 2)
+    return fail(TSPHERE_ERR_INVALID, "the spectral op covers up to 1024 latitude pairs");
+  const Fields st = listed((const void* const*)state, nfield);
+  float2* f2 = reinterpret_cast<float2*>(fm);
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+    TS_HIP(launch_leg_spectral_f64r(p->tablesz64(), nfield, f2, st, lam, ic, S(s)), "leg_spectral");
+  else
+    TS_HIP(launch_leg_spectral(p->tables(), nfield, f2, st, lam, ic, S(s)), "leg_spectral");
+  return ok();
+}
+
 int tsphere_plan_fourier_table(tsphere_plan p, const void** trig) {
   if (!p || !trig) return fail(TSPHERE_ERR_INVALID, "NULL argument");
   *trig = p->d_trig;

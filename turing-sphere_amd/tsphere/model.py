@@ -44,13 +44,18 @@ class GpuModel:
     @staticmethod
     def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
                stream: Stream | None = None, batch: bool = True,
-               file_name: str = "model.m", runs: int = 1, grid: bool = True) -> "GpuModel":
+               file_name: str = "model.m", runs: int = 1, grid: bool = True,
+               spectral: bool = False) -> "GpuModel":
         """GpuModel.create (model.ts:103-163).  ``runs`` > 1: an ensemble of
         that many independent runs of the model (same grid), each value
         holding the runs back to back; the plan needs
         max_fields >= runs * len(state).  ``grid`` lets the planner fuse the
         grid stage (FFT synthesis, element-wise statements, FFT analysis)
-        into one kernel where it can."""
+        into one kernel where it can; ``spectral`` (single runs) lets it fuse
+        the Legendre analysis, the implicit solve and the next step's
+        Legendre synthesis into one kernel where the step has that form
+        (off by default: bit-identical, but measured slower than the separate
+        launches at lmax 1023 so far)."""
         self = GpuModel()
         self.runs = int(runs)
         npts = cfg.nlat * cfg.nphi
@@ -80,7 +85,8 @@ class GpuModel:
         self._init = ModelPlan.create(sht, PlanSpec(init_fn, feedback), host, batch, "init",
                                       runs=R, shared=shared, grid=grid)
         self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step",
-                                      runs=R, shared=shared, grid=grid)
+                                      runs=R, shared=shared, grid=grid,
+                                      spectral=spectral and R == 1)
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm * R, "mgpu-state-stash")
         return self

@@ -164,6 +164,27 @@ class GridOp:
 
 
 @dataclass
+class SpectralOp:
+    """A step whose ops are one GridOp and the diagonal implicit solve of every
+    species, X = (X + s1 * analys(..)) ./ (1 + s2 * lam): each step runs the
+    grid op's kernel, then ONE library kernel (``tsphere_leg_spectral_imex``)
+    doing the grid op's Legendre analysis, the implicit solve and the
+    Legendre synthesis of the new state for the next step -- the recurrence
+    is generated once per step for both contractions.  A call of
+    ``encode_steps`` starts with the grid op's own Legendre synthesis, so it
+    needs nothing from earlier calls."""
+    kind: str
+    label: str
+    grid: GridOp
+    imex: object            # the KernelOp / FusedOp it replaces
+    species: list           # state c_names, in the grid op's synthesis order
+    spc: list               # analysed field f -> species index
+    s1: list                # scalar IR expressions per analysed field
+    s2: list
+    lam: str                # c_name of the eigenvalue array
+
+
+@dataclass
 class XformOp:
     kind: str               # 'synth' | 'analys'
     label: str
@@ -181,6 +202,8 @@ class CopyOp:
 
 
 def _reads(op) -> set:
+    if isinstance(op, SpectralOp):
+        return _reads(op.grid) | _reads(op.imex)
     if isinstance(op, KernelOp):
         return set(op.ins)
     if isinstance(op, FusedOp):
@@ -193,6 +216,8 @@ def _reads(op) -> set:
 
 
 def _writes(op) -> set:
+    if isinstance(op, SpectralOp):
+        return _writes(op.grid) | _writes(op.imex)
     if isinstance(op, KernelOp):
         return {op.out}
     if isinstance(op, FusedOp):
@@ -229,7 +254,7 @@ class ModelPlan:
     @staticmethod
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
                label: str = "step", runs: int = 1, shared=(), fuse: bool = True,
-               grid: bool = True) -> "ModelPlan":
+               grid: bool = True, spectral: bool = False) -> "ModelPlan":
         """ModelPlan.create (plan.ts:206-407).
 
         ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
@@ -242,7 +267,11 @@ class ModelPlan:
         ``grid`` (with ``fuse``, FFT Fourier stages): a synth ->
         element-wise -> analys run of ops becomes a GridOp, whose generated
         kernel keeps the grid values between the row FFTs in LDS and
-        registers."""
+        registers.
+
+        ``spectral`` (single runs, with ``grid``): a step made of one grid op
+        and the implicit solve of its species becomes a SpectralOp (the
+        analysis, the solve and the next step's synthesis in one kernel)."""
         self = ModelPlan()
         self._sht = sht
         self.runs = runs = int(runs)
@@ -353,6 +382,11 @@ class ModelPlan:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
             if grid:
                 ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
+                if spectral and runs == 1:
+                    ops = self._fuse_spectral(ops, {o.c_name for o in fn.outputs}, param_slots,
+                                              derived, host_of)
+        self._param_slots = param_slots
+        self._derived = derived
 
         # Buffers for every value the .m computes (reused across steps).
         for c, n in counts.items():
@@ -364,9 +398,10 @@ class ModelPlan:
             self._by_name[names[c]] = self._bufs[c]
         self.ops = ops
 
-        pending_kernels = [op for op in ops if isinstance(op, (KernelOp, FusedOp, GridOp))]
-        if any(isinstance(op, GridOp) for op in ops) and host.allocate:
-            nfm = runs * max(max(len(op.srcs), len(op.dsts)) for op in ops
+        flat = [x for op in ops for x in ((op.grid,) if isinstance(op, SpectralOp) else (op,))]
+        pending_kernels = [op for op in flat if isinstance(op, (KernelOp, FusedOp, GridOp))]
+        if any(isinstance(op, GridOp) for op in flat) and host.allocate:
+            nfm = runs * max(max(len(op.srcs), len(op.dsts)) for op in flat
                              if isinstance(op, GridOp))
             cfg = sht.cfg
             self._fm = DeviceBuffer(nfm * cfg.nlat * (cfg.mmax + 1) * 2, f"mgpu-{label}-fm")
@@ -479,6 +514,64 @@ class ModelPlan:
             i += 3
         return out
 
+    def _fuse_spectral(self, ops, outputs: set, param_slots: dict, derived: dict,
+                       host_of: dict) -> list:
+        """[GridOp, implicit solve] -> [SpectralOp] when every statement of the
+        solve is X = (X + s1 * A) ./ (1 + s2 * lam) with X a species the grid
+        op synthesises (the value written back to X's own buffer), A one of
+        its analysed values (read by nothing else), lam a host array and s1,
+        s2 scalar expressions."""
+        sht = self._sht
+        if (sht is None or getattr(sht, "legendre", None) not in ("fp32", "f64rec") or
+                len(ops) != 2 or not isinstance(ops[0], GridOp) or
+                not isinstance(ops[1], (KernelOp, FusedOp))):
+            return ops
+        G, E = ops
+        members = E.members if isinstance(E, FusedOp) else [E]
+        if not 1 <= len(G.dsts) <= 4 or len(members) != len(G.dsts) or \
+                len(G.srcs) != len(G.dsts):
+            return ops
+        scal = set(param_slots) | set(derived)
+        spc, s1, s2, lam = [None] * len(G.dsts), [None] * len(G.dsts), [None] * len(G.dsts), None
+        for mem in members:
+            m = _imex_pattern(mem.stmt.expr, scal)
+            if m is None:
+                return ops
+            x, a, lm, e1, e2 = m
+            if (x not in G.srcs or a not in G.dsts or a in outputs or lm not in host_of or
+                    self._bufs.get(mem.out) is not self._bufs.get(x) or
+                    (lam is not None and lm != lam)):
+                return ops
+            f = G.dsts.index(a)
+            if spc[f] is not None:
+                return ops
+            spc[f], s1[f], s2[f], lam = G.srcs.index(x), e1, e2, lm
+        if sorted(spc) != list(range(len(G.srcs))):
+            return ops
+        lbl = f"{G.label} => {E.label}"
+        return [SpectralOp("spectral", lbl, G, E, list(G.srcs), spc, s1, s2, lam)]
+
+    def _scalar(self, e) -> np.float32:
+        """A scalar IR expression of params and literals in f32, rounded per
+        operation as the generated kernels compute it."""
+        if e.kind == "NumLit":
+            return np.float32(e.value)
+        if e.kind == "Var":
+            if e.c_name in self._param_slots:
+                return np.float32(self._params[self._param_slots[e.c_name]])
+            return self._scalar(self._derived[e.c_name][1])
+        if e.kind == "Unary" and e.builtin in ("uminus",):
+            return np.float32(-self._scalar(e.operand))
+        a, b = self._scalar(e.left), self._scalar(e.right)
+        with np.errstate(all="ignore"):
+            if e.builtin == "plus":
+                return np.float32(a + b)
+            if e.builtin == "minus":
+                return np.float32(a - b)
+            if e.builtin in ("mtimes", "times"):
+                return np.float32(a * b)
+            return np.float32(a / b)  # rdivide / mrdivide (checked by _imex_pattern)
+
     def _fold_feedback(self, ops, host: HostBuffers, host_of: dict) -> list:
         """Let the kernel that produces a fed-back value write the host buffer
         directly when nothing after it reads that buffer's old contents."""
@@ -560,8 +653,25 @@ class ModelPlan:
         if ens:
             np_ = 0
             nq, ns = 2 * self._sht.nlm, self._sht.cfg.npts
+        spec = [op for op in self.ops if isinstance(op, SpectralOp)]
+        if spec:
+            # the grid op's own synthesis once; then each step's spectral
+            # kernel leaves the next step's synthesis in the scratch
+            G = spec[0].grid
+            _lib.check(L.tsphere_leg_synth_multi(self._sht.handle, len(G.srcs),
+                                                 self._ptrs(G.srcs), self._fm.ptr, sh), G.label)
         for _ in range(steps):
             for op in self.ops:
+                if isinstance(op, SpectralOp):
+                    self.launch_grid(op.grid, stream)
+                    n = len(op.species)
+                    s1 = (C.c_float * n)(*[float(self._scalar(e)) for e in op.s1])
+                    s2 = (C.c_float * n)(*[float(self._scalar(e)) for e in op.s2])
+                    spc = (C.c_int * n)(*op.spc)
+                    _lib.check(L.tsphere_leg_spectral_imex(
+                        self._sht.handle, n, self._fm.ptr, self._ptrs(op.species),
+                        self._bufs[op.lam].ptr, s1, s2, spc, sh), op.label)
+                    continue
                 if isinstance(op, (KernelOp, FusedOp)):
                     outs = op.outs if isinstance(op, FusedOp) else [op.out]
                     ptrs = self._ptrs(outs + op.ins + (["@prm"] if ens else []))
@@ -638,20 +748,37 @@ class ModelPlan:
     def describe(self) -> list[str]:
         """plan.ts:486-488: '<kind> <label>' per op (the fusion guard counts
         'kernel' lines)."""
-        return [f"{op.kind:<7} {op.label}" for op in self.ops]
+        out = []
+        for op in self.ops:
+            if isinstance(op, SpectralOp):
+                out.append(f"{op.grid.kind:<7} {op.grid.label}")
+                out.append(f"{op.kind:<7} {op.label.split(' => ', 3)[-1]} (with the "
+                           f"analysis before it and the next step's synthesis)")
+            else:
+                out.append(f"{op.kind:<7} {op.label}")
+        return out
+
+    def _flat_ops(self):
+        return [x for op in self.ops
+                for x in ((op.grid,) if isinstance(op, SpectralOp) else (op,))]
 
     def kernel_source(self) -> str:
-        return module_source([op.kernel for op in self.ops
+        return module_source([op.kernel for op in self._flat_ops()
                               if isinstance(op, (KernelOp, FusedOp, GridOp))])
 
     def generated_kernels(self) -> int:
         """Generated kernels launched per call."""
-        return sum(isinstance(op, (KernelOp, FusedOp, GridOp)) for op in self.ops)
+        return sum(isinstance(op, (KernelOp, FusedOp, GridOp)) for op in self._flat_ops())
 
     def elementwise_statements(self) -> int:
-        """Element-wise .m statements the generated kernels evaluate."""
-        return sum(1 if isinstance(op, KernelOp) else len(op.members) if isinstance(op, FusedOp)
-                   else op.nstmts if isinstance(op, GridOp) else 0 for op in self.ops)
+        """Element-wise .m statements a step evaluates (a spectral op's
+        implicit solve runs inside its library kernel)."""
+        def count(op):
+            if isinstance(op, SpectralOp):
+                return count(op.grid) + count(op.imex)
+            return (1 if isinstance(op, KernelOp) else len(op.members) if isinstance(op, FusedOp)
+                    else op.nstmts if isinstance(op, GridOp) else 0)
+        return sum(count(op) for op in self.ops)
 
     def destroy(self) -> None:
         self.invalidate()
@@ -670,6 +797,51 @@ class ModelPlan:
         if self._fm is not None:
             self._fm.free()
             self._fm = None
+
+
+def _is_scalar_expr(e, scal: set) -> bool:
+    if e.kind == "NumLit":
+        return True
+    if e.kind == "Var":
+        return e.c_name in scal and not is_tensor(e.ty)
+    if e.kind == "Unary":
+        return e.builtin == "uminus" and _is_scalar_expr(e.operand, scal)
+    if e.kind == "Binary":
+        return (e.builtin in ("plus", "minus", "mtimes", "times", "rdivide", "mrdivide") and
+                _is_scalar_expr(e.left, scal) and _is_scalar_expr(e.right, scal))
+    return False
+
+
+def _imex_pattern(e, scal: set):
+    """(X + s1 * A) ./ (1 + s2 * lam) -> (X, A, lam, s1, s2) c_names and
+    scalar expressions (products in either order), else None: the diagonal
+    implicit step of bench/shtns/spec.h:166-169 as the models write it."""
+    def tvar(v):
+        return v.c_name if v.kind == "Var" and is_tensor(v.ty) else None
+
+    def prod(b):  # scalar * tensor in either order -> (scalar expr, tensor c_name)
+        if b.kind != "Binary" or b.builtin not in ("mtimes", "times"):
+            return None
+        if _is_scalar_expr(b.left, scal) and tvar(b.right):
+            return b.left, tvar(b.right)
+        if _is_scalar_expr(b.right, scal) and tvar(b.left):
+            return b.right, tvar(b.left)
+        return None
+
+    if e.kind != "Binary" or e.builtin != "rdivide":
+        return None
+    n, d = e.left, e.right
+    if n.kind != "Binary" or n.builtin != "plus" or d.kind != "Binary" or d.builtin != "plus":
+        return None
+    x, pa = tvar(n.left), prod(n.right)
+    if x is None or pa is None:
+        return None
+    if not (d.left.kind == "NumLit" and float(d.left.value) == 1.0):
+        return None
+    pl = prod(d.right)
+    if pl is None:
+        return None
+    return x, pa[1], pl[1], pa[0], pl[0]
 
 
 def _ident(name: str) -> str:
