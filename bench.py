@@ -169,7 +169,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     if grid_ops:
         # the step runs the Fourier stages inside its generated grid kernel
         kms["grid_kernel"] = time_stage(lambda: gm.step_plan.launch_grid(grid_ops[0], s), s, 20)
-    if any(op.kind == "spectral" for op in gm.step_plan.ops) and legendre != "bf16x3":
+    if any(getattr(op, "mode", "") == "spectral" for op in gm.step_plan.ops):
         # the step's spectral kernel (analysis, implicit solve, next synthesis)
         # on scratch copies of the state
         lam = gm._host.get("lam")

@@ -188,6 +188,15 @@ int tsphere_leg_synth_fields(tsphere_plan p, int nptr, int per, const float *con
                              long long qlm_stride, float *fm, tsphere_stream s);
 int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float *fm,
                               float *const *qlm, long long qlm_stride, tsphere_stream s);
+/* The implicit step folded into the next synthesis: state field f is
+ * replaced by X_new = (X + s1[f] A_f) / (1 + s2[f] lam) per real coefficient
+ * (A_f = analysed[f], rounded as written, as tsphere_leg_spectral_imex), and
+ * the new state synthesised into fm (as tsphere_leg_synth_multi): the
+ * implicit-solve dispatch and the next step's encodeSynthInto (plan.ts:
+ * 436-478) as one launch.  FP32 and FP64-recurrence plans, nfield <= 4. */
+int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float *const *analysed,
+                           float *const *state, const float *lam, const float *s1,
+                           const float *s2, float *fm, tsphere_stream s);
 /* The spectral op: per m-row, the Legendre analysis of the scratch fm
  * (nfield fields, as tsphere_leg_analys_multi), the diagonal implicit step
  *   X_new = (X + s1[f] A_f) / (1 + s2[f] lam)     per real coefficient

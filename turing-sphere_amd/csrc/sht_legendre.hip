@@ -219,6 +219,45 @@ __device__ __forceinline__ void stage_row_t(const typename P::Tables& T, int m, 
   }
 }
 
+// The implicit step of one real coefficient, each product and sum rounded
+// as written (the generated kernel's (X + (s1 * A)) / (1 + (s2 * lam))).
+__device__ __forceinline__ float imex_step(float x, float s1, float a, float s2, float lam) {
+#pragma clang fp contract(off)
+  return (x + (s1 * a)) / (1.0f + (s2 * lam));
+}
+
+// stage_row_t with the implicit step in front: field f's state X is replaced
+// by X_new = (X + s1 A) / (1 + s2 lam) (A: its analysed values), written back
+// to the state, and c_l X_new staged as the synthesis coefficients.
+template <int NF, class P>
+__device__ __forceinline__ void stage_row_imex(const typename P::Tables& T, int m, int f0,
+                                               int nfield, const Fields& Fq, const Fields& Fa,
+                                               const float* lam, const ImexCoef& ic,
+                                               typename P::ab_t* sab, float2* sq, int pad) {
+  const int n = T.lmax - m + 1;
+  const long long base = lm_base(T.lmax, m);
+  for (int i = threadIdx.x; i < n + pad; i += blockDim.x) {
+    sab[i] = i < n ? T.ab[base + i] : P::zero_ab();
+    const float cz = i < n ? T.cz[base + i] : 0.f;
+    const float2 lm2 =
+        i < n ? reinterpret_cast<const float2*>(lam)[base + i] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int fld = f0 + f;
+      float2 xn = make_float2(0.f, 0.f);
+      if (i < n && fld < nfield) {
+        float2* X = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(fld))) + base + i;
+        const float2 x = *X;
+        const float2 a = reinterpret_cast<const float2*>(Fa.at(fld))[base + i];
+        xn.x = imex_step(x.x, ic.s1[fld], a.x, ic.s2[fld], lm2.x);
+        xn.y = imex_step(x.y, ic.s1[fld], a.y, ic.s2[fld], lm2.y);
+        *X = xn;
+      }
+      sq[i * NF + f] = make_float2(xn.x * cz, xn.y * cz);
+    }
+  }
+}
+
 // ------------------------------------------------------------- synthesis
 // kTile l-values of chains [K0, K) from l = m + i (INJ: lanes may take
 // their seed at l).
@@ -1031,9 +1070,10 @@ __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, cons
 // Synthesis: per chain and field pair, D_E / D_O (E: even l - m, O: odd)
 // accumulate P[pair][l] * Q[l][col] over l with the row's 4 pairs of a block
 // as rows; the lane of column j ends with the sums of the 4 pairs of its block.
-template <int NF, int K, class P>
+template <int NF, int K, class P, bool IMEX = false>
 __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int nfield, Fields Fq,
-                                                      float2* __restrict__ fm, int rows) {
+                                                      float2* __restrict__ fm, int rows,
+                                                      Fields Fa, const float* lam, ImexCoef ic) {
   constexpr int FP = (NF + 1) / 2;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
@@ -1047,7 +1087,10 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
     const int m = row_m(r, rows, M);
     if (m < 0) break;
     __syncthreads();  // the previous row's readers are done
-    stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
+    if constexpr (IMEX)
+      stage_row_imex<NF, P>(T, m, f0, nfield, Fq, Fa, lam, ic, sab, sq, kPadQ);
+    else
+      stage_row_t<NF, P>(T, m, f0, nfield, Fq, true, sab, sq, kPadQ);
     __syncthreads();
 
     ChainT<P> c[K];
@@ -1118,7 +1161,9 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
                                                        const float2* __restrict__ fm, Fields Fq,
                                                        int rows) {
   constexpr int FP = (NF + 1) / 2;
-  constexpr int M4_NH = FP == 1 ? 4 : 2;  // spans per workgroup barrier (16 KB of red)
+  // spans per workgroup barrier: as many as keep the workgroup's LDS (tiles,
+  // alpha, red) within a third of the CU's 160 KB, so three workgroups fit
+  constexpr int M4_NH = FP == 1 ? 4 : 2;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][8][M4_NH * 64 * FP];
   __shared__ int smin[8];
@@ -1320,13 +1365,6 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
 // scratch entries are read at the row's start and its synthesis written at
 // its end, so the scratch is updated in place.
 constexpr int SP_NH = 2;  // spans per workgroup barrier
-
-// The implicit step of one real coefficient, each product and sum rounded
-// as written (the generated kernel's (X + (s1 * A)) / (1 + (s2 * lam))).
-__device__ __forceinline__ float imex_step(float x, float s1, float a, float s2, float lam) {
-#pragma clang fp contract(off)
-  return (x + (s1 * a)) / (1.0f + (s2 * lam));
-}
 
 template <int NF, int K, class P>
 __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, int nfield,
@@ -1601,13 +1639,35 @@ hipError_t spectral_launch(const typename P::Tables& T, int nfield, float2* fm, 
                 : spectral_m4<4, 2, P>(T, nfield, nwaves, fm, st, lam, ic, s);
 }
 
-template <int NF, int K, class P>
+template <int NF, int K, class P, bool IMEX = false>
 hipError_t synth_m4(const typename P::Tables& T, int nfield, int nwaves, int rows, const Fields& q,
-                    float2* fm, hipStream_t s) {
+                    float2* fm, hipStream_t s, const Fields& a = Fields{},
+                    const float* lam = nullptr, const ImexCoef& ic = ImexCoef{}) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  hipLaunchKernelGGL((k_leg_synth_m4<NF, K, P>), grid, dim3(64 * nwaves),
-                     synth_lds<P>(T.lmax, NF), s, T, nfield, q, fm, rows);
+  hipLaunchKernelGGL((k_leg_synth_m4<NF, K, P, IMEX>), grid, dim3(64 * nwaves),
+                     synth_lds<P>(T.lmax, NF), s, T, nfield, q, fm, rows, a, lam, ic);
   return hipGetLastError();
+}
+
+// The implicit step of every field in front of the block-matrix synthesis
+// (the tile's waves, chains and rows; impl 3 only).
+template <class P>
+hipError_t imex_synth_launch(const typename P::Tables& T, int nfield, const LegTile& t,
+                             const Fields& q, const Fields& a, const float* lam,
+                             const ImexCoef& ic, float2* fm, hipStream_t s) {
+#define TS_IMEX_K(NF)                                                                        \
+  switch (t.k) {                                                                             \
+    case 1: return synth_m4<NF, 1, P, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic); \
+    case 2: return synth_m4<NF, 2, P, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic); \
+    case 4: return synth_m4<NF, 4, P, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic); \
+    default: return hipErrorInvalidValue;                                                    \
+  }
+  switch (t.nf) {
+    case 1: TS_IMEX_K(1)
+    case 2: TS_IMEX_K(2)
+    default: TS_IMEX_K(4)
+  }
+#undef TS_IMEX_K
 }
 
 template <int NF, int K, class P>
@@ -1776,6 +1836,25 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
     case 2: return analys_nf<2, RecD>(T, nfield, t, fm, q, s);
     default: return analys_nf<4, RecD>(T, nfield, t, fm, q, s);
   }
+}
+
+hipError_t launch_leg_imex_synth(const LegTables& T, int nfield, const Fields& q, const Fields& a,
+                                const float* lam, const ImexCoef& ic, float2* fm, LegTile tile,
+                                hipStream_t s) {
+  LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
+  if (t.impl != 3) t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, LegTile{0, 0, 0, 0, 3, 0});
+  if (!fp32_tile_ok(t, T.npair) || t.impl != 3) return hipErrorInvalidValue;
+  return imex_synth_launch<RecF>(T, nfield, t, q, a, lam, ic, fm, s);
+}
+
+hipError_t launch_leg_imex_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q,
+                                     const Fields& a, const float* lam, const ImexCoef& ic,
+                                     float2* fm, LegTile tile, hipStream_t s) {
+  LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 0, T.npair, nfield, tile);
+  if (t.impl != 3)
+    t = resolve_leg_tile(TSPHERE_LEG_F64REC, 0, T.npair, nfield, LegTile{0, 0, 0, 0, 3, 0});
+  if (!fp32_tile_ok(t, T.npair) || t.impl != 3) return hipErrorInvalidValue;
+  return imex_synth_launch<RecD>(T, nfield, t, q, a, lam, ic, fm, s);
 }
 
 hipError_t launch_leg_spectral(const LegTables& T, int nfield, float2* fm, const Fields& st,

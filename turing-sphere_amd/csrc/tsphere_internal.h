@@ -195,6 +195,16 @@ struct ImexCoef {
   float s1[4], s2[4];  // per analysed field
   int spc[4];          // the state field (species) each analysed field updates
 };
+// The implicit step (s1[f], s2[f] per field; spc unused) of every state
+// field q[f] from its analysed values a[f], in front of the Legendre
+// synthesis of the new state into fm: the next step's first stage with the
+// implicit solve of the last one folded in.  Block-matrix kernels (impl 3).
+hipError_t launch_leg_imex_synth(const LegTables& T, int nfield, const Fields& q, const Fields& a,
+                                const float* lam, const ImexCoef& ic, float2* fm, LegTile tile,
+                                hipStream_t s);
+hipError_t launch_leg_imex_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q,
+                                     const Fields& a, const float* lam, const ImexCoef& ic,
+                                     float2* fm, LegTile tile, hipStream_t s);
 hipError_t launch_leg_spectral(const LegTables& T, int nfield, float2* fm, const Fields& st,
                                const float* lam, const ImexCoef& ic, hipStream_t s);
 hipError_t launch_leg_spectral_f64r(const LegTablesZ64& T, int nfield, float2* fm,

@@ -733,6 +733,37 @@ int tsphere_leg_analys_multi(tsphere_plan p, int nfield, const float* fm, float*
   return ok();
 }
 
+int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float* const* analysed,
+                           float* const* state, const float* lam, const float* s1,
+                           const float* s2, float* fm, tsphere_stream s) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1 || nfield > 4 || nfield > p->cfg.max_fields)
+    return fail(TSPHERE_ERR_INVALID, "nfield=%d: need 1..4 and <= max_fields=%d", nfield,
+                p->cfg.max_fields);
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    return fail(TSPHERE_ERR_INVALID, "the fused implicit step runs the FP32 and FP64-recurrence "
+                                     "forms");
+  if (!analysed || !state || !lam || !s1 || !s2 || !fm)
+    return fail(TSPHERE_ERR_INVALID, "NULL argument");
+  ImexCoef ic{};
+  for (int f = 0; f < nfield; f++) {
+    if (!analysed[f] || !state[f]) return fail(TSPHERE_ERR_INVALID, "NULL buffer for field %d", f);
+    ic.s1[f] = s1[f];
+    ic.s2[f] = s2[f];
+    ic.spc[f] = f;
+  }
+  const Fields q = listed((const void* const*)state, nfield);
+  const Fields a = listed((const void* const*)analysed, nfield);
+  float2* f2 = reinterpret_cast<float2*>(fm);
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+    TS_HIP(launch_leg_imex_synth_f64r(p->tablesz64(), nfield, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
+  else
+    TS_HIP(launch_leg_imex_synth(p->tables(), nfield, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
+  return ok();
+}
+
 int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float* fm, float* const* state,
                               const float* lam, const float* s1, const float* s2,
                               const int* species, tsphere_stream s) {

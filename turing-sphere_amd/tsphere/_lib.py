@@ -99,6 +99,8 @@ SIGNATURES = {
     "tsphere_leg_analys": [_vp, _i, _vp, _vp, _ll, _vp],
     "tsphere_leg_synth_multi": [_vp, _i, C.POINTER(_vp), _vp, _vp],
     "tsphere_leg_analys_multi": [_vp, _i, _vp, C.POINTER(_vp), _vp],
+    "tsphere_leg_imex_synth": [_vp, _i, C.POINTER(_vp), C.POINTER(_vp), _vp,
+                               C.POINTER(C.c_float), C.POINTER(C.c_float), _vp, _vp],
     "tsphere_leg_spectral_imex": [_vp, _i, _vp, C.POINTER(_vp), _vp, C.POINTER(C.c_float),
                                   C.POINTER(C.c_float), C.POINTER(C.c_int), _vp],
     "tsphere_leg_synth_fields": [_vp, _i, _i, C.POINTER(_vp), _ll, _vp, _vp],

@@ -45,7 +45,7 @@ class GpuModel:
     def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
                stream: Stream | None = None, batch: bool = True,
                file_name: str = "model.m", runs: int = 1, grid: bool = True,
-               spectral: bool = False) -> "GpuModel":
+               spectral: bool = False, fold: bool = True) -> "GpuModel":
         """GpuModel.create (model.ts:103-163).  ``runs`` > 1: an ensemble of
         that many independent runs of the model (same grid), each value
         holding the runs back to back; the plan needs
@@ -55,7 +55,8 @@ class GpuModel:
         the Legendre analysis, the implicit solve and the next step's
         Legendre synthesis into one kernel where the step has that form
         (off by default: bit-identical, but measured slower than the separate
-        launches at lmax 1023 so far)."""
+        launches at lmax 1023 so far); ``fold`` (single runs, the default)
+        folds the implicit solve into the next step's synthesis launch."""
         self = GpuModel()
         self.runs = int(runs)
         npts = cfg.nlat * cfg.nphi
@@ -86,7 +87,7 @@ class GpuModel:
                                       runs=R, shared=shared, grid=grid)
         self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step",
                                       runs=R, shared=shared, grid=grid,
-                                      spectral=spectral and R == 1)
+                                      spectral=spectral and R == 1, fold=fold and R == 1)
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm * R, "mgpu-state-stash")
         return self

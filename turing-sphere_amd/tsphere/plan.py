@@ -182,6 +182,8 @@ class SpectralOp:
     s1: list                # scalar IR expressions per analysed field
     s2: list
     lam: str                # c_name of the eigenvalue array
+    mode: str = "spectral"  # or "fold": the grid op's analysis separate, the
+                            # implicit solve folded into the next synthesis
 
 
 @dataclass
@@ -254,7 +256,8 @@ class ModelPlan:
     @staticmethod
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
                label: str = "step", runs: int = 1, shared=(), fuse: bool = True,
-               grid: bool = True, spectral: bool = False) -> "ModelPlan":
+               grid: bool = True, spectral: bool = False,
+               fold: bool = False) -> "ModelPlan":
         """ModelPlan.create (plan.ts:206-407).
 
         ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
@@ -271,7 +274,10 @@ class ModelPlan:
 
         ``spectral`` (single runs, with ``grid``): a step made of one grid op
         and the implicit solve of its species becomes a SpectralOp (the
-        analysis, the solve and the next step's synthesis in one kernel)."""
+        analysis, the solve and the next step's synthesis in one kernel).
+        ``fold`` (single runs, with ``grid``, when not ``spectral``): the same
+        step keeps the analysis as its own launch and folds the implicit
+        solve into the next step's synthesis launch."""
         self = ModelPlan()
         self._sht = sht
         self.runs = runs = int(runs)
@@ -382,9 +388,10 @@ class ModelPlan:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
             if grid:
                 ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
-                if spectral and runs == 1:
+                if (spectral or fold) and runs == 1:
                     ops = self._fuse_spectral(ops, {o.c_name for o in fn.outputs}, param_slots,
-                                              derived, host_of)
+                                              derived, host_of,
+                                              "spectral" if spectral else "fold")
         self._param_slots = param_slots
         self._derived = derived
 
@@ -515,7 +522,7 @@ class ModelPlan:
         return out
 
     def _fuse_spectral(self, ops, outputs: set, param_slots: dict, derived: dict,
-                       host_of: dict) -> list:
+                       host_of: dict, mode: str = "spectral") -> list:
         """[GridOp, implicit solve] -> [SpectralOp] when every statement of the
         solve is X = (X + s1 * A) ./ (1 + s2 * lam) with X a species the grid
         op synthesises (the value written back to X's own buffer), A one of
@@ -549,7 +556,7 @@ class ModelPlan:
         if sorted(spc) != list(range(len(G.srcs))):
             return ops
         lbl = f"{G.label} => {E.label}"
-        return [SpectralOp("spectral", lbl, G, E, list(G.srcs), spc, s1, s2, lam)]
+        return [SpectralOp("spectral", lbl, G, E, list(G.srcs), spc, s1, s2, lam, mode)]
 
     def _scalar(self, e) -> np.float32:
         """A scalar IR expression of params and literals in f32, rounded per
@@ -665,12 +672,28 @@ class ModelPlan:
                 if isinstance(op, SpectralOp):
                     self.launch_grid(op.grid, stream)
                     n = len(op.species)
-                    s1 = (C.c_float * n)(*[float(self._scalar(e)) for e in op.s1])
-                    s2 = (C.c_float * n)(*[float(self._scalar(e)) for e in op.s2])
-                    spc = (C.c_int * n)(*op.spc)
-                    _lib.check(L.tsphere_leg_spectral_imex(
-                        self._sht.handle, n, self._fm.ptr, self._ptrs(op.species),
-                        self._bufs[op.lam].ptr, s1, s2, spc, sh), op.label)
+                    v1 = [float(self._scalar(e)) for e in op.s1]
+                    v2 = [float(self._scalar(e)) for e in op.s2]
+                    if op.mode == "spectral":
+                        _lib.check(L.tsphere_leg_spectral_imex(
+                            self._sht.handle, n, self._fm.ptr, self._ptrs(op.species),
+                            self._bufs[op.lam].ptr, (C.c_float * n)(*v1), (C.c_float * n)(*v2),
+                            (C.c_int * n)(*op.spc), sh), op.label)
+                        continue
+                    # fold: the analysis, then the implicit solve of each
+                    # species (its analysed value, its scalars) in front of
+                    # the next step's synthesis
+                    G = op.grid
+                    _lib.check(L.tsphere_leg_analys_multi(self._sht.handle, len(G.dsts),
+                                                          self._fm.ptr, self._ptrs(G.dsts), sh),
+                               G.label)
+                    by = {sp: f for f, sp in enumerate(op.spc)}
+                    ana = [G.dsts[by[sp]] for sp in range(n)]
+                    _lib.check(L.tsphere_leg_imex_synth(
+                        self._sht.handle, n, self._ptrs(ana), self._ptrs(op.species),
+                        self._bufs[op.lam].ptr, (C.c_float * n)(*[v1[by[sp]] for sp in range(n)]),
+                        (C.c_float * n)(*[v2[by[sp]] for sp in range(n)]), self._fm.ptr, sh),
+                        op.label)
                     continue
                 if isinstance(op, (KernelOp, FusedOp)):
                     outs = op.outs if isinstance(op, FusedOp) else [op.out]
@@ -750,7 +773,11 @@ class ModelPlan:
         'kernel' lines)."""
         out = []
         for op in self.ops:
-            if isinstance(op, SpectralOp):
+            if isinstance(op, SpectralOp) and op.mode == "fold":
+                out.append(f"{op.grid.kind:<7} {op.grid.label}")
+                out.append(f"{'imex':<7} {op.label.split(' => ', 3)[-1]} (folded into the "
+                           f"next step's synthesis)")
+            elif isinstance(op, SpectralOp):
                 out.append(f"{op.grid.kind:<7} {op.grid.label}")
                 out.append(f"{op.kind:<7} {op.label.split(' => ', 3)[-1]} (with the "
                            f"analysis before it and the next step's synthesis)")
