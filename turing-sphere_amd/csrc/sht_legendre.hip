@@ -1174,8 +1174,10 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
   const int j = lane & 3, blk = lane >> 2;
   const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
   const bool odd = lg >> 1;  // this block's parity of l - m
-  float* tile = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1)) +
-                wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
+  // the row's c_l, staged with alpha (the combine step reads them after a
+  // barrier, where a global load's latency would stall the workgroup)
+  float* scz = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
+  float* tile = scz + ((L + 1 + 3) & ~3) + wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
   const int f0 = blockIdx.y * NF;
 
   for (int r = 0; r < 2; ++r) {
@@ -1184,6 +1186,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
     const long long base = lm_base(L, m);
     __syncthreads();
     stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+    for (int i = tid; i <= L - m; i += nthr) scz[i] = T.cz[base + i];
 
     ChainT<P> c[K];
     int lk[K];
@@ -1341,7 +1344,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
           for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
-          dst[jj & 1] = s * T.cz[base + (l - m)];
+          dst[jj & 1] = s * scz[l - m];
         }
       }
       buf ^= 1;
@@ -1675,6 +1678,7 @@ hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int ro
                      const float2* fm, const Fields& q, hipStream_t s) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+                     sizeof(float) * ((T.lmax + 1 + 3) & ~3) +
                      sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
   hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q, rows);
