@@ -166,10 +166,15 @@ def test_graph_replay_matches_eager(gpu):
     gm.snapshot_state()
     gm._step.encode_steps(gm.stream, 7)
     eager = gm.read("U")
-    for rep in range(3):
+    for rep in range(5):
         gm.restore_state()
-        gm.graph_steps = 1 if rep == 0 else 3
-        gm.step(7)
+        # 0: graphs of 4, 2 and 1 steps; 1, 3: one graph replayed, rest eager
+        gm.graph_steps = (1, 3, 0, 0, 3)[rep]
+        if rep == 3:
+            gm.step(3)
+            gm.step(4)
+        else:
+            gm.step(7)
         got = gm.read("U")
         emit(f"replay {rep}: max diff {np.max(np.abs(got - eager)):.1e}")
         assert np.array_equal(got, eager)

@@ -38,7 +38,7 @@ class GpuModel:
         self._last_ran = "init"
         self._stashed_ran = "init"
         self._destroyed = False
-        self.graph_steps = 1
+        self.graph_steps = 0           # 0: power-of-two graphs up to 64 steps
         self.runs = 1
 
     @staticmethod

@@ -244,8 +244,7 @@ class ModelPlan:
         self._kernels: list = []
         self._params = np.zeros(1, np.float32)
         self._sht = None
-        self._graph = None
-        self._graph_steps = 0
+        self._graphs: dict[int, int] = {}      # steps per graph -> hipGraphExec
         self._ptr_cache = {}
         self.runs = 1
         self._prm = None       # ensembles: device copy of the per-run scalars
@@ -626,10 +625,9 @@ class ModelPlan:
             self._prm.upload(self._params, stream)
 
     def invalidate(self) -> None:
-        if self._graph is not None:
-            _lib.call("tsphere_graph_destroy", self._graph)
-        self._graph = None
-        self._graph_steps = 0
+        for g in self._graphs.values():
+            _lib.call("tsphere_graph_destroy", g)
+        self._graphs = {}
 
     def buffer(self, name: str):
         return self._by_name.get(name)
@@ -746,27 +744,44 @@ class ModelPlan:
             op.handle, op.blocks, self.runs, op.block, len(ptrs), ptrs,
             0 if ens else len(self.param_names), pvals, stream.handle), op.label)
 
-    def run(self, stream: Stream, steps: int = 1, graph_steps: int = 1) -> None:
-        """Replay `steps` timesteps through a HIP graph holding `graph_steps`
-        of them (captured on first use or after a parameter change)."""
-        if steps <= 0:
-            return
-        if self._graph is None or self._graph_steps != graph_steps:
-            self.invalidate()
+    def _graph(self, stream: Stream, k: int) -> int:
+        """The HIP graph of `k` timesteps (captured on first use or after a
+        parameter change)."""
+        g = self._graphs.get(k)
+        if g is None:
             _lib.call("tsphere_graph_begin", stream.handle)
             try:
-                self.encode_steps(stream, graph_steps)
+                self.encode_steps(stream, k)
             finally:
-                g = C.c_void_p()
-                rc = _lib.load().tsphere_graph_end(stream.handle, C.byref(g))
+                h = C.c_void_p()
+                rc = _lib.load().tsphere_graph_end(stream.handle, C.byref(h))
             _lib.check(rc, "graph capture")
-            self._graph = g.value
-            self._graph_steps = graph_steps
-        full, rest = divmod(steps, graph_steps)
-        for _ in range(full):
-            _lib.call("tsphere_graph_launch", self._graph, stream.handle)
-        if rest:
-            self.encode_steps(stream, rest)
+            g = self._graphs[k] = h.value
+        return g
+
+    def run(self, stream: Stream, steps: int = 1, graph_steps: int = 0) -> None:
+        """Replay `steps` timesteps through HIP graphs.  `graph_steps` > 0:
+        one graph of that many steps, the remainder recorded directly;
+        0: graphs of power-of-two step counts up to 64, largest first (at most
+        seven kept), so that `steps` steps cost O(log) graph launches and a
+        step count that repeats replays the same graphs."""
+        if steps <= 0:
+            return
+        if graph_steps > 0:
+            if any(k != graph_steps for k in self._graphs):
+                self.invalidate()
+            full, rest = divmod(steps, graph_steps)
+            if full:
+                g = self._graph(stream, graph_steps)
+                for _ in range(full):
+                    _lib.call("tsphere_graph_launch", g, stream.handle)
+            if rest:
+                self.encode_steps(stream, rest)
+            return
+        while steps > 0:
+            k = 1 << (min(steps, 64).bit_length() - 1)
+            _lib.call("tsphere_graph_launch", self._graph(stream, k), stream.handle)
+            steps -= k
 
     def describe(self) -> list[str]:
         """plan.ts:486-488: '<kind> <label>' per op (the fusion guard counts
