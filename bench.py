@@ -371,8 +371,8 @@ def main() -> int:
     if args.hbm_calibration:
         return hbm_calibration()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
-              f"step on a single MI355X -- see DESIGN.md for how it would shard.",
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the step runs on a "
+              f"single MI355X; across GPUs it would shard by spectral rows m and latitude rows.",
               file=sys.stderr)
         return 2
 

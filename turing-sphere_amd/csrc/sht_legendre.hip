@@ -45,6 +45,9 @@ namespace {
 constexpr int kTile = 4;   // l-values between seed injections (seeds are 4-aligned)
 constexpr int kPadQ = 24;  // zero entries after a staged row (reads run past lmax)
 constexpr int kPadAnalys = 40;  // analysis spans read up to 32 + 3 entries past lmax
+// entries of the staged alpha row in an analysis workgroup's LDS, rounded to
+// 4 so that the arrays after it start 16-byte aligned (their 16-byte loads)
+__host__ __device__ constexpr int analys_ab_len(int lmax) { return (lmax + 1 + kPadAnalys + 3) & ~3; }
 
 __device__ __forceinline__ long long lm_base(int lmax, int m) {
   return (long long)m * (lmax + 1) - ((long long)m * (m - 1)) / 2;
@@ -769,7 +772,7 @@ __global__ __launch_bounds__(256) void k_leg_analys_mf(typename P::Tables T, int
   const int nthr = blockDim.x, nwaves = nthr >> 6;
   const int r16 = lane & 15, kg = lane >> 4;
   auto* sab = reinterpret_cast<typename P::ab_t*>(smem);
-  float* tiles = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
+  float* tiles = reinterpret_cast<float*>(sab + analys_ab_len(L));
   float* tl = tiles + wv * (2 * MF_TL * TROW);  // two tiles per wave
   float* red = tiles + 4 * 2 * MF_TL * TROW;     // [4][256]
   const int f0 = blockIdx.y * NF;
@@ -912,7 +915,7 @@ template <int NF, int KC, class P>
 hipError_t analys_mf(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
                      const Fields& q, hipStream_t s) {
   dim3 grid(T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+  const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (4 * 2 * MF_TL * (size_t)(64 * KC + 4) + 4 * 256);
   hipLaunchKernelGGL((k_leg_analys_mf<NF, KC, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q);
@@ -1151,19 +1154,31 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
 // two blocks of an 8-lane LDS pass read rows 4 apart in bank terms.
 constexpr int M4_SPAN = 16;
 constexpr int M4_TROW = 64 + 4;  // floats per l row of the tile
+constexpr int M4_WROW = 64 + 2;  // floats per column row of the row's staged weights
+static_assert(8 * 2 * M4_WROW <= M4_SPAN * M4_TROW, "two field pairs' weights fit a tile");
 
 __device__ __forceinline__ int m4_lgroup(int blk) { return ((blk & 1) << 1) | ((blk >> 1) & 1); }
 // l offset in the span of row r of l-group g
 __device__ __forceinline__ int m4_off(int g, int r) { return (g >> 1) + 2 * r + 8 * (g & 1); }
 
+// Three 8-wave workgroups per CU (6 waves per SIMD) need at most 80 VGPRs and
+// a third of the CU's 160 KB of LDS per workgroup: the FP32-recurrence,
+// one-chain, one-field-pair form is held to that register count (measured ~7%
+// faster than at 84 VGPRs and two workgroups); the FP64 recurrence and wider
+// forms would spill under it.
 template <int NF, int K, class P>
-__global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int nfield,
-                                                       const float2* __restrict__ fm, Fields Fq,
-                                                       int rows) {
+constexpr int m4_analys_waves_per_eu() {
+  return std::is_same<typename P::ab_t, float>::value && (NF + 1) / 2 == 1 && K == 1 ? 6 : 1;
+}
+
+template <int NF, int K, class P>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_analys_waves_per_eu<NF, K, P>())))
+void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict__ fm, Fields Fq,
+                     int rows) {
   constexpr int FP = (NF + 1) / 2;
-  // spans per workgroup barrier: as many as keep the workgroup's LDS (tiles,
-  // alpha, red) within a third of the CU's 160 KB, so three workgroups fit
-  constexpr int M4_NH = FP == 1 ? 4 : 2;
+  // spans per workgroup barrier: two keep the workgroup's LDS (tiles, alpha,
+  // red) within a third of the CU's 160 KB at lmax 1023 with FP64 alpha
+  constexpr int M4_NH = 2;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][8][M4_NH * 64 * FP];
   __shared__ int smin[8];
@@ -1174,10 +1189,8 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
   const int j = lane & 3, blk = lane >> 2;
   const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
   const bool odd = lg >> 1;  // this block's parity of l - m
-  // the row's c_l, staged with alpha (the combine step reads them after a
-  // barrier, where a global load's latency would stall the workgroup)
-  float* scz = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1));
-  float* tile = scz + ((L + 1 + 3) & ~3) + wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
+  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) +
+               wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
   const int f0 = blockIdx.y * NF;
 
   for (int r = 0; r < 2; ++r) {
@@ -1186,14 +1199,16 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
     const long long base = lm_base(L, m);
     __syncthreads();
     stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
-    for (int i = tid; i <= L - m; i += nthr) scz[i] = T.cz[base + i];
 
     ChainT<P> c[K];
     int lk[K];
     int lzmax = -1, lmin = 0x7fffffff;
     // B operands for the whole row: each lane computes W of its own pair
     // (every column, both parities) into the chain's tile, then gathers the
-    // 16 of its block: pair 16 pg + t, column j, the block's parity.
+    // 16 of its block: pair 16 pg + t, column j, the block's parity.  The
+    // tile holds them as rows q = (parity FP + fp) 4 + column of M4_WROW
+    // floats, one per pair: the writes are consecutive, and the gather's
+    // 32 distinct addresses (q of the block, 16 pg + t) fall in 32 banks.
     float W[K][16][FP];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1205,7 +1220,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
       const int jc = min(jp, T.npair - 1);
       const int js = nlat - 1 - jc;
       const float w = jp < T.npair ? T.w[jc] : 0.f;
-      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane * (8 * FP);  // [pair][parity][fp][col]
+      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane;  // [parity][fp][col][pair]
 #pragma unroll
       for (int f = 0; f < 2 * FP; ++f) {
         const bool live = f < NF && f0 + f < nfield;
@@ -1221,10 +1236,10 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
                              (gN.y - gS.y) * w);
         }
         const int fp = f >> 1, c0 = 2 * (f & 1);
-        tw[(0 * FP + fp) * 4 + c0] = we.x;
-        tw[(0 * FP + fp) * 4 + c0 + 1] = we.y;
-        tw[(1 * FP + fp) * 4 + c0] = we.z;
-        tw[(1 * FP + fp) * 4 + c0 + 1] = we.w;
+        tw[((0 * FP + fp) * 4 + c0) * M4_WROW] = we.x;
+        tw[((0 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.y;
+        tw[((1 * FP + fp) * 4 + c0) * M4_WROW] = we.z;
+        tw[((1 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.w;
       }
     }
     wave_sync_lds();
@@ -1234,8 +1249,8 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
       for (int t = 0; t < 16; ++t)
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp)
-          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) + (16 * pg + t) * (8 * FP) +
-                             ((odd ? FP : 0) + fp) * 4 + j];
+          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) +
+                             (((odd ? FP : 0) + fp) * 4 + j) * M4_WROW + 16 * pg + t];
     wave_sync_lds();  // the tiles are rewritten by the spans
     if (lane == 0) smin[wv] = lmin;
     __syncthreads();  // also publishes the staged row
@@ -1344,7 +1359,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_m4(typename P::Tables T, int
           for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
-          dst[jj & 1] = s * scz[l - m];
+          dst[jj & 1] = s * T.cz[base + (l - m)];
         }
       }
       buf ^= 1;
@@ -1387,7 +1402,7 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
   const int j = lane & 3, blk = lane >> 2;
   const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
   const bool odd = lg >> 1;
-  float* tile = reinterpret_cast<float*>(sab + ((L + 1 + kPadAnalys + 1) & ~1)) +
+  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) +
                 wv * (K * M4_SPAN * M4_TROW);
 
   for (int r = 0; r < 2; ++r) {
@@ -1411,7 +1426,7 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
       const int jc = min(jp, T.npair - 1);
       const int js = nlat - 1 - jc;
       const float w = jp < T.npair ? T.w[jc] : 0.f;
-      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane * (8 * FP);
+      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane;  // [parity][fp][col][pair]
 #pragma unroll
       for (int f = 0; f < 2 * FP; ++f) {
         const bool live = f < NF && f < nfield;
@@ -1427,10 +1442,10 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
                              (gN.y - gS.y) * w);
         }
         const int fp = f >> 1, c0 = 2 * (f & 1);
-        tw[(0 * FP + fp) * 4 + c0] = we.x;
-        tw[(0 * FP + fp) * 4 + c0 + 1] = we.y;
-        tw[(1 * FP + fp) * 4 + c0] = we.z;
-        tw[(1 * FP + fp) * 4 + c0 + 1] = we.w;
+        tw[((0 * FP + fp) * 4 + c0) * M4_WROW] = we.x;
+        tw[((0 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.y;
+        tw[((1 * FP + fp) * 4 + c0) * M4_WROW] = we.z;
+        tw[((1 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.w;
       }
     }
     wave_sync_lds();
@@ -1440,8 +1455,8 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
       for (int t = 0; t < 16; ++t)
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp)
-          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) + (16 * pg + t) * (8 * FP) +
-                             ((odd ? FP : 0) + fp) * 4 + j];
+          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) +
+                             (((odd ? FP : 0) + fp) * 4 + j) * M4_WROW + 16 * pg + t];
     wave_sync_lds();
     if (lane == 0) smin[wv] = lmin;
     __syncthreads();  // also publishes the staged row; every W is read
@@ -1615,7 +1630,7 @@ template <int NF, int K, class P>
 hipError_t spectral_m4(const typename P::Tables& T, int nfield, int nwaves, float2* fm,
                        const Fields& st, const float* lam, const ImexCoef& ic, hipStream_t s) {
   dim3 grid(T.mmax + 1, 1);
-  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
+  const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
   hipLaunchKernelGGL((k_leg_spectral_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      st, lam, ic, 1);
@@ -1677,8 +1692,7 @@ template <int NF, int K, class P>
 hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int rows,
                      const float2* fm, const Fields& q, hipStream_t s) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
-  const size_t lds = sizeof(typename P::ab_t) * ((T.lmax + 1 + kPadAnalys + 1) & ~1) +
-                     sizeof(float) * ((T.lmax + 1 + 3) & ~3) +
+  const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
   hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q, rows);
