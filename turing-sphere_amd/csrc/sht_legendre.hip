@@ -1150,8 +1150,14 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
 // 4b + j then sums P W over the pair-group for l_r, so a span ends with a sum
 // over the 4 pair-groups (lane bits 4, 5: two swaps) and the waves' sums
 // meeting in LDS.  Seeds sit at l - m = 0 mod 4, so a span's offsets keep the
-// parity of l - m.  Lane bits 2, 3 map to l-groups (0, 2, 1, 3) so that the
-// two blocks of an 8-lane LDS pass read rows 4 apart in bank terms.
+// parity of l - m.  Lane bits 2, 3 map to l-groups (0, 2, 1, 3), and the
+// tile stores offset p in physical row m4_row(p), so that block q of a
+// pair-group reads physical rows 4q .. 4q + 3: with rows 68 floats apart, the
+// 16-byte load of lane 16 pg + 4 q + j starts in 4-bank group
+// (4 q + j + 4 pg + t) mod 16, distinct over any 16 lanes that differ in
+// (q, j) or in (j, pg), whether an LDS pass takes 16 consecutive lanes or
+// 4 lanes from each 16 (the kernel's measured bank-conflict count did not
+// change with this placement: those cycles come from elsewhere).
 constexpr int M4_SPAN = 16;
 constexpr int M4_TROW = 64 + 4;  // floats per l row of the tile
 constexpr int M4_WROW = 64 + 2;  // floats per column row of the row's staged weights
@@ -1160,6 +1166,11 @@ static_assert(8 * 2 * M4_WROW <= M4_SPAN * M4_TROW, "two field pairs' weights fi
 __device__ __forceinline__ int m4_lgroup(int blk) { return ((blk & 1) << 1) | ((blk >> 1) & 1); }
 // l offset in the span of row r of l-group g
 __device__ __forceinline__ int m4_off(int g, int r) { return (g >> 1) + 2 * r + 8 * (g & 1); }
+// physical tile row of span offset p: m4_row(m4_off(m4_lgroup(q), r)) = 4 q + r
+__host__ __device__ constexpr int m4_row(int p) { return (p & 8) | ((p & 1) << 2) | ((p & 7) >> 1); }
+static_assert(m4_row(0) == 0 && m4_row(2) == 1 && m4_row(1) == 4 && m4_row(7) == 7 &&
+                  m4_row(8) == 8 && m4_row(14) == 11 && m4_row(9) == 12 && m4_row(15) == 15,
+              "block q's rows are physical rows 4q .. 4q + 3");
 
 // Three 8-wave workgroups per CU (6 waves per SIMD) need at most 80 VGPRs and
 // a third of the CU's 160 KB of LDS per workgroup: the FP32-recurrence,
@@ -1298,8 +1309,8 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
               c[k].y0 = c[k].s0;
               c[k].y1 = c[k].s1;
             }
-            tk[p * M4_TROW] = (float)c[k].y0;
-            tk[(p + 1) * M4_TROW] = (float)c[k].y1;
+            tk[m4_row(p) * M4_TROW] = (float)c[k].y0;
+            tk[m4_row(p + 1) * M4_TROW] = (float)c[k].y1;
             P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
           }
         }
@@ -1308,7 +1319,7 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
         f4v acc[FP];
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
-        const int arow = m4_off(lg, j) * M4_TROW + 16 * pg;  // lane j supplies row j
+        const int arow = (4 * (blk & 3) + j) * M4_TROW + 16 * pg;  // lane j: row j of lg
         // the products as one group (no VALU work scheduled between them: on
         // gfx950 they do not overlap it, and every switch costs issue time)
         __builtin_amdgcn_sched_barrier(0);
@@ -1509,8 +1520,8 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
             }
             zk[h][k][p] = (float)c[k].y0;
             zk[h][k][p + 1] = (float)c[k].y1;
-            tk[p * M4_TROW] = zk[h][k][p];
-            tk[(p + 1) * M4_TROW] = zk[h][k][p + 1];
+            tk[m4_row(p) * M4_TROW] = zk[h][k][p];
+            tk[m4_row(p + 1) * M4_TROW] = zk[h][k][p + 1];
             P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
           }
         }
@@ -1518,7 +1529,7 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
         f4v acc[FP];
 #pragma unroll
         for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
-        const int arow = m4_off(lg, j) * M4_TROW + 16 * pg;
+        const int arow = (4 * (blk & 3) + j) * M4_TROW + 16 * pg;
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
