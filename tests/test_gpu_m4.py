@@ -19,8 +19,10 @@ def leg_tol(lmax, legendre):
     return 1e-5 if legendre == "f64rec" else max(3e-6, 2e-10 * (lmax + 1) ** 2)
 
 
+# lmax 12 and 101: lmax + 1 = 1 and 2 mod 4, rows whose staged alpha length
+# is rounded up before the 16-byte aligned tiles that follow it in LDS
 _SMALL = [(15, 16, 32), (15, 17, 32), (12, 13, 26), (63, 64, 128), (127, 128, 256),
-          (255, 256, 512)]
+          (255, 256, 512), (101, 102, 204)]
 _CASES = ([g + (nf, "fp32") for g in _SMALL for nf in (1, 2, 3, 5)]
           + [g + (nf, "f64rec") for g in _SMALL[:4] for nf in (1, 2, 5)]
           + [(511, 512, 1024, 2, "fp32"), (1023, 1024, 2048, 2, "fp32"),
