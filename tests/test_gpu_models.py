@@ -29,7 +29,7 @@ DEFAULTS = {
 }
 
 
-def make(key, lmax, nlat, nphi, params, batch=True, grid=True, spectral=False,
+def make(key, lmax, nlat, nphi, params, batch=True, grid=True, spectral=True,
          legendre="fp32", fold=True):
     import tsphere
     from tsphere.model import GpuModel

@@ -1,0 +1,87 @@
+"""The spectral op's C entry point (tsphere_leg_spectral_imex: Legendre
+analysis, the diagonal implicit step and Legendre synthesis of the new state
+in one pass per m-row) against the three launches it replaces, called
+through the same C ABI: tsphere_leg_analys, the implicit step
+X_new = (X + s1 A) / (1 + s2 lam) evaluated op by op in float32 on the host
+(IEEE single rounding of each product, sum and quotient, as the kernels
+round it), and tsphere_leg_synth of X_new.  The two analyses may sum the
+latitude pairs in different orders (the fused kernel picks its own
+workgroup shape), so the comparison is to 2e-6 relative, not bit for bit
+(the planner-level tests compare the fused and separate step plans
+exactly).  One to four fields: three and four fields at lmax 15 have fewer
+latitude pairs than one wave, and run in the two-wave workgroup the fused
+kernel's combine needs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import emit
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi", [(15, 16, 32), (63, 64, 128), (255, 256, 512)])
+@pytest.mark.parametrize("nfield", [1, 2, 3, 4])
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
+def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield, legendre):
+    import tsphere
+    from tsphere import _lib
+    from tsphere.device import DeviceBuffer
+
+    emit(f"spectral abi {legendre} lmax {lmax} {nlat}x{nphi} nfield {nfield}")
+    L = _lib.load()
+    cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
+    plan = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=4)
+    s = plan.stream
+    nlm, M = plan.nlm, lmax
+    rng = np.random.default_rng(1000 + 10 * lmax + nfield)
+    nfm = nfield * 2 * (M + 1) * nlat
+    fm0 = rng.normal(size=nfm).astype(np.float32)
+    x0 = (0.1 * rng.normal(size=(nfield, 2 * nlm))).astype(np.float32)
+    ell = np.repeat(np.concatenate([np.arange(m, lmax + 1) for m in range(M + 1)]), 2)
+    lam = (ell * (ell + 1)).astype(np.float32)
+    s1 = np.float32([0.002, 0.003, 0.001, 0.004])[:nfield]
+    s2 = np.float32([4e-7, 3e-6, 1e-5, 2e-6])[:nfield]
+
+    # fused: one launch over the scratch and the state
+    fmA = DeviceBuffer(nfm, "fmA")
+    fmA.upload(fm0, s)
+    xA = [DeviceBuffer(2 * nlm, f"xA{f}") for f in range(nfield)]
+    for f in range(nfield):
+        xA[f].upload(x0[f], s)
+    lamb = DeviceBuffer(2 * nlm, "lam")
+    lamb.upload(lam, s)
+    ptrs = (C.c_void_p * nfield)(*[b.ptr for b in xA])
+    cs1 = (C.c_float * nfield)(*s1.tolist())
+    cs2 = (C.c_float * nfield)(*s2.tolist())
+    spc = (C.c_int * nfield)(*range(nfield))
+    _lib.check(L.tsphere_leg_spectral_imex(plan.handle, nfield, fmA.ptr, ptrs, lamb.ptr, cs1, cs2,
+                                           spc, s.handle), "spectral")
+    got_x = np.stack([b.download(s) for b in xA])
+    got_fm = fmA.download(s)
+
+    # separate: analysis, host implicit step, synthesis
+    fmB = DeviceBuffer(nfm, "fmB")
+    fmB.upload(fm0, s)
+    ab = DeviceBuffer(nfield * 2 * nlm, "A")
+    _lib.check(L.tsphere_leg_analys(plan.handle, nfield, fmB.ptr, ab.ptr, 2 * nlm, s.handle),
+               "analys")
+    A = ab.download(s).reshape(nfield, 2 * nlm)
+    xn = np.empty_like(x0)
+    for f in range(nfield):
+        num = x0[f] + s1[f] * A[f]           # float32 ops, each rounded once
+        den = np.float32(1.0) + s2[f] * lam
+        xn[f] = num / den
+    xb = DeviceBuffer(nfield * 2 * nlm, "xn")
+    xb.upload(xn.reshape(-1), s)
+    _lib.check(L.tsphere_leg_synth(plan.handle, nfield, xb.ptr, 2 * nlm, fmB.ptr, s.handle),
+               "synth")
+    want_fm = fmB.download(s)
+
+    ex = O.rel_l2(got_x, xn)
+    ef = O.rel_l2(got_fm, want_fm)
+    emit(f"  state {ex:.2e}  scratch {ef:.2e}")
+    assert np.all(np.isfinite(got_fm)) and ex < 2e-6 and ef < 2e-6
+    plan.destroy()

@@ -1390,21 +1390,31 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
 //                                          arithmetic)
 // written to the state and, scaled by c_l, to LDS; then the synthesis block
 // products of those l with the chain values the wave kept in registers (the
-// synthesis of k_leg_synth_m4, in the same order over l).  The row's Fourier
-// scratch entries are read at the row's start and its synthesis written at
-// its end, so the scratch is updated in place.
-constexpr int SP_NH = 2;  // spans per workgroup barrier
+// synthesis of k_leg_synth_m4, in the same order over l).  The spans run as a
+// pipeline with one workgroup barrier each: iteration i takes span i's
+// analysis up to the waves' sums, the barrier, then the combine and implicit
+// step of span i (the first 64 FP threads, whose state, lam and c_l loads
+// were issued before the analysis) beside every wave's synthesis products of
+// span i - 1 (solved in the previous iteration, published by this barrier).
+// The row's Fourier scratch entries are read at the row's start and its
+// synthesis written at its end, so the scratch is updated in place.
+// two 8-wave workgroups per CU need at most 128 VGPRs: the one-chain,
+// two-field form is held to that (it compiles to 132 with the FP32 recurrence
+// and to 146 with the FP64 one, which then spills 120 bytes per lane and
+// still measured 7% faster per step than one workgroup per CU; the one- and
+// four-field forms would spill more)
+template <int NF, int K, class P>
+constexpr int m4_spectral_waves_per_eu() { return NF == 2 && K == 1 ? 4 : 1; }
 
 template <int NF, int K, class P>
-__global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, int nfield,
-                                                         float2* fm, Fields St,
-                                                         const float* __restrict__ lam,
-                                                         ImexCoef ic, int rows) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_spectral_waves_per_eu<NF, K, P>())))
+void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
+                       const float* __restrict__ lam, ImexCoef ic, int rows) {
   constexpr int FP = (NF + 1) / 2;
-  constexpr int NO = SP_NH * 64 * FP;  // combined sums per span group
+  constexpr int NO = 64 * FP;  // combined sums per span
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][8][NO];
-  __shared__ float qs[SP_NH * M4_SPAN][4 * FP];  // c_l X_new, (l, species, re/im)
+  __shared__ float qs[2][M4_SPAN][4 * FP];  // c_l X_new, (l, species, re/im)
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
@@ -1491,127 +1501,152 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
 #pragma unroll
       for (int fp = 0; fp < FP; ++fp) accE[k][fp] = accO[k][fp] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    int buf = 0;
-    for (int ls0 = lblk; ls0 <= L; ls0 += SP_NH * M4_SPAN) {
-      float zk[SP_NH][K][M4_SPAN];  // the chains' values, kept for the synthesis
-      bool live[SP_NH];
+    // span ls's analysis into this wave's sums red[b][wv] and the chain
+    // values zc; false when every chain of the wave is still 0 there
+    auto analyse = [&](int ls, int b, float (&zc)[K][M4_SPAN]) -> bool {
+      float* rw = &red[b][wv][0];
+      if (lmin >= ls + M4_SPAN) {
 #pragma unroll
-      for (int h = 0; h < SP_NH; ++h) {
-        const int ls = ls0 + h * M4_SPAN;
-        float* rw = &red[buf][wv][h * 64 * FP];
-        live[h] = !(lmin >= ls + M4_SPAN || ls > L);
-        if (!live[h]) {
-#pragma unroll
-          for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
-          continue;
-        }
-        typename P::ab_t al[M4_SPAN];
-#pragma unroll
-        for (int p = 0; p < M4_SPAN; ++p) al[p] = sab[ls - m + p + 2];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
-#pragma unroll
-          for (int p = 0; p < M4_SPAN; p += 2) {
-            const int l = ls + p;
-            if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
-              c[k].y0 = c[k].s0;
-              c[k].y1 = c[k].s1;
-            }
-            zk[h][k][p] = (float)c[k].y0;
-            zk[h][k][p + 1] = (float)c[k].y1;
-            tk[m4_row(p) * M4_TROW] = zk[h][k][p];
-            tk[m4_row(p + 1) * M4_TROW] = zk[h][k][p + 1];
-            P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
-          }
-        }
-        wave_sync_lds();
-        f4v acc[FP];
-#pragma unroll
-        for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
-        const int arow = (4 * (blk & 3) + j) * M4_TROW + 16 * pg;
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
-          float4 a4[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) a4[t] = *reinterpret_cast<const float4*>(tk + 4 * t);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const float a[4] = {a4[t].x, a4[t].y, a4[t].z, a4[t].w};
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-              for (int fp = 0; fp < FP; ++fp)
-                acc[fp] = mf441(a[u], W[k][4 * t + u][fp], acc[fp]);
-          }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        wave_sync_lds();
-#pragma unroll
-        for (int fp = 0; fp < FP; ++fp) {
-          float v[4] = {acc[fp][0], acc[fp][1], acc[fp][2], acc[fp][3]};
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            const auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
-                                                             __float_as_uint(v[i + 2]), false, false);
-            v[i] = __uint_as_float(s2[0]) + __uint_as_float(s2[1]);
-          }
-          const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
-                                                           __float_as_uint(v[1]), false, false);
-          rw[lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
-        }
+        for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
+        return false;
       }
-      __syncthreads();
-      // the waves' sums (in wave order) -> analysed value -> implicit step
-      for (int o = tid; o < NO; o += nthr) {
-        const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
-        const int jj = ln & 3, bb = ln >> 2;
-        const int lo = h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
-        const int l = ls0 + lo;
-        const int f = 2 * fp + (jj >> 1);
-        float qv = 0.f;
-        if (l <= L && f < NF && f < nfield) {
-          float s = red[buf][0][o];
-          for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
-          const float cz = T.cz[base + (l - m)];
-          const float a = s * cz;
-          const long long i = 2 * (base + (l - m)) + (jj & 1);
-          float* X = reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[f])));
-          const float xn = imex_step(X[i], ic.s1[f], a, ic.s2[f], lam[i]);
-          X[i] = xn;
-          qv = xn * cz;
-        }
-        if (f < NF) qs[lo][2 * (f < nfield ? ic.spc[f] : f) + (jj & 1)] = qv;
-      }
-      __syncthreads();
-      // synthesis block products of the group's l, in order of l
-      __builtin_amdgcn_sched_barrier(0);
+      typename P::ab_t al[M4_SPAN];
 #pragma unroll
-      for (int h = 0; h < SP_NH; ++h) {
-        if (!live[h]) continue;  // every chain value is still 0 (wave-uniform)
+      for (int p = 0; p < M4_SPAN; ++p) al[p] = sab[ls - m + p + 2];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
 #pragma unroll
         for (int p = 0; p < M4_SPAN; p += 2) {
-          float bE[FP], bO[FP];
-#pragma unroll
-          for (int fp = 0; fp < FP; ++fp) {
-            const float* qa = &qs[h * M4_SPAN + p][4 * fp];
-            const float* qb = &qs[h * M4_SPAN + p + 1][4 * fp];
-            bE[fp] = (NF == 1 && j >= 2) ? 0.f : qa[j];
-            bO[fp] = (NF == 1 && j >= 2) ? 0.f : qb[j];
+          const int l = ls + p;
+          if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
+            c[k].y0 = c[k].s0;
+            c[k].y1 = c[k].s1;
           }
+          zc[k][p] = (float)c[k].y0;
+          zc[k][p + 1] = (float)c[k].y1;
+          tk[m4_row(p) * M4_TROW] = zc[k][p];
+          tk[m4_row(p + 1) * M4_TROW] = zc[k][p + 1];
+          P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
+        }
+      }
+      wave_sync_lds();
+      f4v acc[FP];
 #pragma unroll
-          for (int k = 0; k < K; ++k)
+      for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+      const int arow = (4 * (blk & 3) + j) * M4_TROW + 16 * pg;
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int fp = 0; fp < FP; ++fp) {
-              accE[k][fp] = mf441(zk[h][k][p], bE[fp], accE[k][fp]);
-              accO[k][fp] = mf441(zk[h][k][p + 1], bO[fp], accO[k][fp]);
-            }
+      for (int k = 0; k < K; ++k) {
+        const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
+        float4 a4[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a4[t] = *reinterpret_cast<const float4*>(tk + 4 * t);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float a[4] = {a4[t].x, a4[t].y, a4[t].z, a4[t].w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int fp = 0; fp < FP; ++fp) acc[fp] = mf441(a[u], W[k][4 * t + u][fp], acc[fp]);
         }
       }
       __builtin_amdgcn_sched_barrier(0);
-      buf ^= 1;
+      wave_sync_lds();  // the tile is rewritten by the next span
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) {
+        float v[4] = {acc[fp][0], acc[fp][1], acc[fp][2], acc[fp][3]};
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]),
+                                                           __float_as_uint(v[i + 2]), false, false);
+          v[i] = __uint_as_float(s2[0]) + __uint_as_float(s2[1]);
+        }
+        const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
+                                                         __float_as_uint(v[1]), false, false);
+        rw[lane * FP + fp] = __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
+      }
+      return true;
+    };
+    // synthesis block products of one span's l, in order of l
+    auto synthesise = [&](const float (&zp)[K][M4_SPAN], int b) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int p = 0; p < M4_SPAN; p += 2) {
+        float bE[FP], bO[FP];
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) {
+          const float* qa = &qs[b][p][4 * fp];
+          const float* qb = &qs[b][p + 1][4 * fp];
+          bE[fp] = (NF == 1 && j >= 2) ? 0.f : qa[j];
+          bO[fp] = (NF == 1 && j >= 2) ? 0.f : qb[j];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) {
+            accE[k][fp] = mf441(zp[k][p], bE[fp], accE[k][fp]);
+            accO[k][fp] = mf441(zp[k][p + 1], bO[fp], accO[k][fp]);
+          }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    // this thread's coefficient of a span's combine (threads below NO)
+    const int cfp = tid % FP, cln = (tid / FP) & 63;
+    const int cjj = cln & 3, cbb = cln >> 2;
+    const int clo = m4_off(m4_lgroup(cbb & 3), cbb >> 2);
+    const int cf = 2 * cfp + (cjj >> 1);
+    const bool cthr = tid < NO && cf < NF;
+    const bool cfield = cthr && cf < nfield;
+    float* const cX = cfield ? reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[cf])))
+                             : nullptr;
+    // one pipeline iteration: span ls's analysis, the barrier, its combine and
+    // implicit step beside the synthesis of the span before it (if any)
+    auto iteration = [&](int ls, int b, float (&zc)[K][M4_SPAN], const float (&zp)[K][M4_SPAN],
+                         bool livep) -> bool {
+      const int l = ls + clo;
+      const bool has = cfield && l <= L;
+      long long ci = 0;
+      float xo = 0.f, lv = 0.f, cz = 0.f;
+      if (has) {  // issued before the analysis, used after the barrier
+        ci = 2 * (base + (l - m)) + (cjj & 1);
+        xo = cX[ci];
+        lv = lam[ci];
+        cz = T.cz[base + (l - m)];
+      }
+      const bool live = analyse(ls, b, zc);
+      __syncthreads();  // the waves' sums of span ls; qs[b ^ 1] of the span before
+      if (cthr) {
+        float qv = 0.f;
+        if (has) {
+          float s = red[b][0][tid];
+          for (int w = 1; w < nwaves; ++w) s += red[b][w][tid];
+          const float xn = imex_step(xo, ic.s1[cf], s * cz, ic.s2[cf], lv);
+          cX[ci] = xn;
+          qv = xn * cz;
+        }
+        qs[b][clo][2 * (cf < nfield ? ic.spc[cf] : cf) + (cjj & 1)] = qv;
+      }
+      if (livep) synthesise(zp, b ^ 1);
+      return live;
+    };
+    float zA[K][M4_SPAN], zB[K][M4_SPAN];
+    bool liveA = false, liveB = false, lastA = false;
+    // spans two at a time, the chain values alternating between zA and zB
+    for (int ls = lblk; ls <= L; ls += 2 * M4_SPAN) {
+      liveA = iteration(ls, 0, zA, zB, liveB);
+      lastA = true;
+      if (ls + M4_SPAN > L) break;
+      liveB = iteration(ls + M4_SPAN, 1, zB, zA, liveA);
+      lastA = false;
+    }
+    if (lblk <= L) {
+      __syncthreads();  // qs of the last span
+      if (lastA) {
+        if (liveA) synthesise(zA, 0);
+      } else if (liveB) {
+        synthesise(zB, 1);
+      }
     }
 
     // the synthesis of the row into the Fourier scratch (north = E + O,
@@ -1640,6 +1675,7 @@ __global__ __launch_bounds__(512) void k_leg_spectral_m4(typename P::Tables T, i
 template <int NF, int K, class P>
 hipError_t spectral_m4(const typename P::Tables& T, int nfield, int nwaves, float2* fm,
                        const Fields& st, const float* lam, const ImexCoef& ic, hipStream_t s) {
+  if (nwaves < (NF + 1) / 2 || nwaves > 8) return hipErrorInvalidValue;
   dim3 grid(T.mmax + 1, 1);
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
@@ -1656,7 +1692,8 @@ hipError_t spectral_launch(const typename P::Tables& T, int nfield, float2* fm, 
   const int npair = T.npair;
   const int k = npair <= 64 * 8 ? 1 : npair <= 64 * 8 * 2 ? 2 : 0;
   if (k == 0) return hipErrorInvalidValue;
-  const int nwaves = std::min(8, (npair + 64 * k - 1) / (64 * k));
+  // the combine takes one thread per sum of a span: 64 per field pair
+  const int nwaves = std::max(std::min(8, (npair + 64 * k - 1) / (64 * k)), nfield > 2 ? 2 : 1);
   if (nfield <= 2) {
     if (nfield == 1)
       return k == 1 ? spectral_m4<1, 1, P>(T, nfield, nwaves, fm, st, lam, ic, s)

@@ -528,6 +528,8 @@ class ModelPlan:
         its analysed values (read by nothing else), lam a host array and s1,
         s2 scalar expressions."""
         sht = self._sht
+        if mode == "spectral" and sht is not None and (sht.cfg.nlat + 1) // 2 > 1024:
+            mode = "fold"  # the spectral kernel covers up to 1024 latitude pairs
         if (sht is None or getattr(sht, "legendre", None) not in ("fp32", "f64rec") or
                 len(ops) != 2 or not isinstance(ops[0], GridOp) or
                 not isinstance(ops[1], (KernelOp, FusedOp))):
