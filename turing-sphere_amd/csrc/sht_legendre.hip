@@ -1399,10 +1399,10 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
 // The row's Fourier scratch entries are read at the row's start and its
 // synthesis written at its end, so the scratch is updated in place.
 // two 8-wave workgroups per CU need at most 128 VGPRs: the one-chain,
-// two-field form is held to that (it compiles to 132 with the FP32 recurrence
-// and to 146 with the FP64 one, which then spills 120 bytes per lane and
-// still measured 7% faster per step than one workgroup per CU; the one- and
-// four-field forms would spill more)
+// two-field form is held to that (it compiles to 132 with the FP32 recurrence;
+// the FP64 one spills 28 bytes per lane there and still runs its step
+// faster than at one workgroup per CU; the one- and four-field forms would
+// spill more)
 template <int NF, int K, class P>
 constexpr int m4_spectral_waves_per_eu() { return NF == 2 && K == 1 ? 4 : 1; }
 
@@ -1510,24 +1510,32 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
         for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
         return false;
       }
-      typename P::ab_t al[M4_SPAN];
+      // the span's alpha in batches read ahead of the tile stores: all 16 of
+      // the f32 form at once, the f64 form's four at a time (16 doubles would
+      // take 32 VGPRs of a kernel held to 128; with 4 it spills 28 bytes per
+      // lane instead of 120, and its step measured 152 -> 140 us at C2)
+      constexpr int AB = sizeof(typename P::ab_t) == 8 ? M4_SPAN / 4 : M4_SPAN;
 #pragma unroll
-      for (int p = 0; p < M4_SPAN; ++p) al[p] = sab[ls - m + p + 2];
+      for (int pb = 0; pb < M4_SPAN; pb += AB) {
+        typename P::ab_t al[AB];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
+        for (int p = 0; p < AB; ++p) al[p] = sab[ls - m + pb + p + 2];
 #pragma unroll
-        for (int p = 0; p < M4_SPAN; p += 2) {
-          const int l = ls + p;
-          if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
-            c[k].y0 = c[k].s0;
-            c[k].y1 = c[k].s1;
+        for (int k = 0; k < K; ++k) {
+          float* tk = tile + k * (M4_SPAN * M4_TROW) + lane;
+#pragma unroll
+          for (int p = pb; p < pb + AB; p += 2) {
+            const int l = ls + p;
+            if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
+              c[k].y0 = c[k].s0;
+              c[k].y1 = c[k].s1;
+            }
+            zc[k][p] = (float)c[k].y0;
+            zc[k][p + 1] = (float)c[k].y1;
+            tk[m4_row(p) * M4_TROW] = zc[k][p];
+            tk[m4_row(p + 1) * M4_TROW] = zc[k][p + 1];
+            P::advance(al[p - pb], al[p - pb + 1], c[k].ct, c[k].y0, c[k].y1);
           }
-          zc[k][p] = (float)c[k].y0;
-          zc[k][p + 1] = (float)c[k].y1;
-          tk[m4_row(p) * M4_TROW] = zc[k][p];
-          tk[m4_row(p + 1) * M4_TROW] = zc[k][p + 1];
-          P::advance(al[p], al[p + 1], c[k].ct, c[k].y0, c[k].y1);
         }
       }
       wave_sync_lds();

@@ -56,7 +56,7 @@ class GpuModel:
         the next step's Legendre synthesis into one kernel where the step has
         that form and the Legendre form is FP32 or FP64-recurrence
         (bit-identical to the separate launches; at lmax 1023 129 against
-        131-133 us per FP32 step, 152 against 163 with the FP64 recurrence);
+        131-133 us per FP32 step, 140 against 163 with the FP64 recurrence);
         ``fold`` (single runs, used where the spectral op is off or does not
         apply, e.g. above 1024 latitude pairs) folds the implicit solve into
         the next step's synthesis launch."""
