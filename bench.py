@@ -402,6 +402,14 @@ def main() -> int:
         m = re.search(r"^hbm_bytes_per_launch: (\d+)", open(prof).read(), re.M)
         traffic = int(m.group(1)) if m else None
 
+    # achieved HBM rate of the Fourier-stage kernels on their algorithmic
+    # bytes: a field's Fourier scratch is (M+1) nlat complex, its grid nlat
+    # nphi reals; the grid kernel reads the scratch, writes the returned grids
+    # and the analysed scratch
+    fmb, spb = 2 * 8 * (LMAX + 1) * NLAT, 2 * 4 * NLAT * NPHI
+    stage_bytes = {"fft_synth": fmb + spb, "fft_analys": spb + fmb, "grid_kernel": 2 * fmb + spb}
+    hbm_gbs = {k: round(b / (kms[k] * 1e-3) / 1e9, 1) for k, b in stage_bytes.items() if k in kms}
+
     extras = {}
     if not args.quick:
         for leg in ("fp32", "f64rec", "bf16x3"):
@@ -439,6 +447,7 @@ def main() -> int:
         "sht_gflops": round(sht_flops_step * steps_s / 1e9, 2),
         "transforms_per_s": round(4 * steps_s, 2),
         "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
+        "hbm_gbs": hbm_gbs,
         "finite": main_v["finite"],
         "variants": {k: {kk: (round(vv, 8) if isinstance(vv, float) else vv)
                          for kk, vv in v.items() if kk != "kernel_ms"}

@@ -1156,8 +1156,9 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
 // 16-byte load of lane 16 pg + 4 q + j starts in 4-bank group
 // (4 q + j + 4 pg + t) mod 16, distinct over any 16 lanes that differ in
 // (q, j) or in (j, pg), whether an LDS pass takes 16 consecutive lanes or
-// 4 lanes from each 16 (the kernel's measured bank-conflict count did not
-// change with this placement: those cycles come from elsewhere).
+// 4 lanes from each 16.  (The bank-conflict counter reads 4 cycles per
+// 16-byte operand load under this placement and the earlier one alike: the
+// passes of a full-wave 16-byte load, not avoidable conflicts.)
 constexpr int M4_SPAN = 16;
 constexpr int M4_TROW = 64 + 4;  // floats per l row of the tile
 constexpr int M4_WROW = 64 + 2;  // floats per column row of the row's staged weights
