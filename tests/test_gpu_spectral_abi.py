@@ -12,6 +12,10 @@ exactly).  One to four fields: three and four fields at lmax 15 have fewer
 latitude pairs than one wave, and run in the two-wave workgroup the fused
 kernel's combine needs."""
 import ctypes as C
+import json
+import os
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -84,4 +88,75 @@ def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield,
     ef = O.rel_l2(got_fm, want_fm)
     emit(f"  state {ex:.2e}  scratch {ef:.2e}")
     assert np.all(np.isfinite(got_fm)) and ex < 2e-6 and ef < 2e-6
+    plan.destroy()
+
+
+def test_napi_spectral_entry_matches_ctypes(gpu, tmp_path):
+    """legSpectralImex through the N-API addon, in a node child process, gives
+    exactly what the ctypes call gives on the same inputs (an FP64-recurrence
+    plan, so the addon's legendre option is exercised too)."""
+    import tsphere
+    from tsphere import _lib
+    from tsphere.device import DeviceBuffer
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    addon = os.path.join(root, "turing-sphere_amd", "napi", "tsphere_napi.node")
+    node = shutil.which("node")
+    if node is None or not os.path.exists(addon):
+        pytest.skip("node or the built addon is not available")
+    lmax, nlat, nphi, nfield = 63, 64, 128, 2
+    emit(f"N-API legSpectralImex f64rec lmax {lmax} (node child process)")
+    cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
+    plan = tsphere.ShtPlan.create(cfg, legendre="f64rec", max_fields=nfield)
+    s = plan.stream
+    nlm = plan.nlm
+    rng = np.random.default_rng(77)
+    fm0 = rng.normal(size=nfield * 2 * (lmax + 1) * nlat).astype(np.float32)
+    x0 = (0.1 * rng.normal(size=(nfield, 2 * nlm))).astype(np.float32)
+    ell = np.repeat(np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)]), 2)
+    lam = (ell * (ell + 1)).astype(np.float32)
+    s1 = np.float32([0.002, 0.003])
+    s2 = np.float32([4e-7, 3e-6])
+    files = {k: str(tmp_path / f"{k}.f32") for k in ("fm", "x0", "x1", "lam", "ofm", "ox0", "ox1")}
+    fm0.tofile(files["fm"])
+    x0[0].tofile(files["x0"])
+    x0[1].tofile(files["x1"])
+    lam.tofile(files["lam"])
+    js = f"""
+const fs = require('fs');
+const ts = require({json.dumps(addon)});
+const f = {json.dumps(files)};
+const rd = p => new Float32Array(fs.readFileSync(p).buffer.slice(0));
+const plan = ts.planCreate({{lmax: {lmax}, mmax: {lmax}, nlat: {nlat}, nphi: {nphi},
+                             legendre: 'f64rec', maxFields: {nfield}}});
+const h = {{}};
+for (const k of ['fm', 'x0', 'x1', 'lam']) {{
+  const a = rd(f[k]); h[k] = [ts.malloc(4 * a.length), a.length]; ts.upload(h[k][0], a);
+}}
+ts.legSpectralImex(plan, h.fm[0], [h.x0[0], h.x1[0]], h.lam[0],
+                   [{float(s1[0])!r}, {float(s1[1])!r}], [{float(s2[0])!r}, {float(s2[1])!r}], [0, 1]);
+for (const [k, o] of [['fm', 'ofm'], ['x0', 'ox0'], ['x1', 'ox1']])
+  fs.writeFileSync(f[o], Buffer.from(ts.download(h[k][0], h[k][1]).buffer));
+for (const k of Object.keys(h)) ts.free(h[k][0]);
+ts.planDestroy(plan);
+console.log('ok');
+"""
+    r = subprocess.run([node, "-e", js], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    fmA = DeviceBuffer(fm0.size, "fmA")
+    fmA.upload(fm0, s)
+    xA = [DeviceBuffer(2 * nlm, f"xA{f}") for f in range(nfield)]
+    for f in range(nfield):
+        xA[f].upload(x0[f], s)
+    lamb = DeviceBuffer(2 * nlm, "lam")
+    lamb.upload(lam, s)
+    L = _lib.load()
+    _lib.check(L.tsphere_leg_spectral_imex(
+        plan.handle, nfield, fmA.ptr, (C.c_void_p * nfield)(*[b.ptr for b in xA]), lamb.ptr,
+        (C.c_float * nfield)(*s1.tolist()), (C.c_float * nfield)(*s2.tolist()),
+        (C.c_int * nfield)(0, 1), s.handle), "spectral")
+    assert np.array_equal(np.fromfile(files["ofm"], np.float32), fmA.download(s))
+    for f in range(nfield):
+        assert np.array_equal(np.fromfile(files[f"ox{f}"], np.float32), xA[f].download(s))
     plan.destroy()

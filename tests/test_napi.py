@@ -32,7 +32,8 @@ def test_addon_loads_and_exports():
     for name in ("planCreate", "planQuery", "planDestroy", "synth", "analys", "malloc", "free",
                  "upload", "download", "moduleCompile", "moduleKernel", "kernelLaunch",
                  "graphBegin", "graphEnd", "graphLaunch", "streamCreate", "streamSync",
-                 "legSynth", "legAnalys", "fourierTable", "kernelLaunchGrid"):
+                 "legSynth", "legAnalys", "fourierTable", "kernelLaunchGrid", "legImexSynth",
+                 "legSpectralImex"):
         assert name in out["names"]
 
 
@@ -46,8 +47,13 @@ def test_addon_config_errors():
       }}
       let te = null;
       try {{ a.synth(1, 2, 3); }} catch (e) {{ te = e.name; }}
-      console.log(JSON.stringify({{r, te}}));""")
+      let le = null;
+      try {{ a.planCreate({{lmax: 31, mmax: 31, nlat: 32, nphi: 64, legendre: 'fp16'}}); }}
+      catch (e) {{ le = [e.name, e.message]; }}
+      console.log(JSON.stringify({{r, te, le}}));""")
     (n1, m1, c1), (n2, m2, c2) = out["r"]
     assert n1 == n2 == "ShtConfigError" and c1 == c2 == 1
     assert "nlat must be an integer > lmax" in m1 and "nphi must be an integer >= 2*mmax+1" in m2
     assert out["te"] == "TypeError"   # plain numbers are not handles
+    # an unknown Legendre form is refused, not mapped to FP32
+    assert out["le"][0] == "TypeError" and "legendre must be" in out["le"][1]

@@ -140,7 +140,14 @@ napi_value PlanCreate(napi_env env, napi_callback_info info) {
   const std::string f = str_prop("fourier", "auto"), l = str_prop("legendre", "fp32");
   c.fourier = f == "fft" ? TSPHERE_FOURIER_FFT : f == "dft" ? TSPHERE_FOURIER_DFT
                                                             : TSPHERE_FOURIER_AUTO;
-  c.legendre = l == "bf16x3" ? TSPHERE_LEG_BF16X3 : TSPHERE_LEG_FP32;
+  if (l == "fp32")
+    c.legendre = TSPHERE_LEG_FP32;
+  else if (l == "f64rec")
+    c.legendre = TSPHERE_LEG_F64REC;
+  else if (l == "bf16x3")
+    c.legendre = TSPHERE_LEG_BF16X3;
+  else
+    return type_error(env, "planCreate: legendre must be 'fp32', 'f64rec' or 'bf16x3'");
   tsphere_plan p = nullptr;
   if (int rc = tsphere_plan_create(&c, &p)) return throw_error(env, rc, "ShtPlan.create");
   return make_ptr(env, p);
@@ -372,6 +379,60 @@ napi_value LegStage(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// legImexSynth(plan, [analysed...], [state...], lam, [s1...], [s2...], fm, stream?):
+// each state field's implicit step X = (X + s1 A) / (1 + s2 lam) from its
+// analysed values A, then the Legendre synthesis of the new state into fm
+// (the planner's "fold" form of a step, after the grid op and legAnalys)
+napi_value LegImexSynth(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void *p = nullptr, *lam = nullptr, *fm = nullptr, *s = nullptr;
+  std::vector<void*> an, st;
+  std::vector<float> s1, s2;
+  if (a.argc < 7 || !get_ptr(env, a.v[0], &p) || !get_ptrs(env, a.v[1], an) ||
+      !get_ptrs(env, a.v[2], st) || !get_ptr(env, a.v[3], &lam) || !get_floats(env, a.v[4], s1) ||
+      !get_floats(env, a.v[5], s2) || !get_ptr(env, a.v[6], &fm))
+    return type_error(env, "legImexSynth(plan, analysedPtrs, statePtrs, lam, s1, s2, fm)");
+  const size_t n = st.size();
+  if (an.size() != n || s1.size() != n || s2.size() != n)
+    return type_error(env, "legImexSynth: one analysed pointer, s1 and s2 per state field");
+  if (a.argc >= 8) get_ptr(env, a.v[7], &s);
+  if (int rc = tsphere_leg_imex_synth(static_cast<tsphere_plan>(p), static_cast<int>(n),
+                                      reinterpret_cast<const float* const*>(an.data()),
+                                      reinterpret_cast<float* const*>(st.data()),
+                                      static_cast<const float*>(lam), s1.data(), s2.data(),
+                                      static_cast<float*>(fm), s))
+    return throw_error(env, rc, "legImexSynth");
+  return nullptr;
+}
+
+// legSpectralImex(plan, fm, [state...], lam, [s1...], [s2...], [species...], stream?):
+// the Legendre analysis of fm, each field's implicit step into state
+// species[f] and the Legendre synthesis of the new state into fm, one launch
+// (the planner's default "spectral" form of a step, after the grid op)
+napi_value LegSpectralImex(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void *p = nullptr, *fm = nullptr, *lam = nullptr, *s = nullptr;
+  std::vector<void*> st;
+  std::vector<float> s1, s2, spf;
+  if (a.argc < 7 || !get_ptr(env, a.v[0], &p) || !get_ptr(env, a.v[1], &fm) ||
+      !get_ptrs(env, a.v[2], st) || !get_ptr(env, a.v[3], &lam) || !get_floats(env, a.v[4], s1) ||
+      !get_floats(env, a.v[5], s2) || !get_floats(env, a.v[6], spf))
+    return type_error(env, "legSpectralImex(plan, fm, statePtrs, lam, s1, s2, species)");
+  const size_t n = st.size();
+  if (s1.size() != n || s2.size() != n || spf.size() != n)
+    return type_error(env, "legSpectralImex: one s1, s2 and species entry per state field");
+  std::vector<int> spc(n);
+  for (size_t i = 0; i < n; i++) spc[i] = static_cast<int>(spf[i]);
+  if (a.argc >= 8) get_ptr(env, a.v[7], &s);
+  if (int rc = tsphere_leg_spectral_imex(static_cast<tsphere_plan>(p), static_cast<int>(n),
+                                         static_cast<float*>(fm),
+                                         reinterpret_cast<float* const*>(st.data()),
+                                         static_cast<const float*>(lam), s1.data(), s2.data(),
+                                         spc.data(), s))
+    return throw_error(env, rc, "legSpectralImex");
+  return nullptr;
+}
+
 // fourierTable(plan) -> device pointer of the plan's twiddle table
 napi_value FourierTable(napi_env env, napi_callback_info info) {
   Args a(env, info);
@@ -434,6 +495,9 @@ NAPI_MODULE_INIT() {
        nullptr},
       {"legSynth", nullptr, LegStage<true>, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"legAnalys", nullptr, LegStage<false>, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"legImexSynth", nullptr, LegImexSynth, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"legSpectralImex", nullptr, LegSpectralImex, nullptr, nullptr, nullptr, napi_default,
+       nullptr},
       {"fourierTable", nullptr, FourierTable, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"graphBegin", nullptr, GraphBegin, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"graphEnd", nullptr, GraphEnd, nullptr, nullptr, nullptr, napi_default, nullptr},
