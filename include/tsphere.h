@@ -35,7 +35,7 @@ extern "C" {
 #define TSPHERE_ERR_INVALID 1     /* bad argument or configuration            */
 #define TSPHERE_ERR_HIP 2         /* HIP runtime error (message has details)  */
 #define TSPHERE_ERR_COMPILE 3     /* hipRTC compile failure (log in message)  */
-#define TSPHERE_ERR_UNSUPPORTED 4 /* valid request this build cannot serve     */
+#define TSPHERE_ERR_UNSUPPORTED 4 /* valid request the library cannot serve    */
 #define TSPHERE_ERR_NODEVICE 5    /* no GPU visible                            */
 
 /* Library version, as 10000*major + 100*minor + patch. */

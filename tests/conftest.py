@@ -10,7 +10,7 @@ for p in (ROOT, os.path.join(ROOT, "turing-sphere_amd")):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
 
 
 def _has_gpu() -> bool:

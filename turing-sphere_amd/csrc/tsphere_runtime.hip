@@ -386,7 +386,7 @@ static int validate(const tsphere_sht_config* c) {
                 "nphi must be an integer >= 2*mmax+1 to avoid aliasing (got nphi=%d, mmax=%d)",
                 c->nphi, c->mmax);
   if (c->nlat > 4096)
-    return fail(TSPHERE_ERR_UNSUPPORTED, "nlat=%d exceeds this build's limit of 4096", c->nlat);
+    return fail(TSPHERE_ERR_UNSUPPORTED, "nlat=%d exceeds the library's limit of 4096", c->nlat);
   if (c->fourier < TSPHERE_FOURIER_AUTO || c->fourier > TSPHERE_FOURIER_DFT)
     return fail(TSPHERE_ERR_INVALID, "unknown fourier mode %d", c->fourier);
   if (c->legendre != TSPHERE_LEG_FP32 && c->legendre != TSPHERE_LEG_BF16X3 &&

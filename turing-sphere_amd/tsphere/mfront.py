@@ -4,7 +4,7 @@ The reference lowers ``models/*.m`` with numbl (src/mgpu/compile.ts:83-190):
 ``specializeUserFunction`` per function for the host's argument types, then
 ``inlinePass``, which folds single-use temps back so each element-wise source
 line becomes ONE statement whose right-hand side is an expression tree.
-numbl is not available to this build, so this module parses the subset the
+numbl (a separate project) is not part of this library, so this module parses the subset the
 models use and produces that folded IR directly:
 
 * files of ``function [outs] = name(params) ... end`` blocks, ``%`` comments,
