@@ -6,10 +6,15 @@ lmax 1023 on the Gauss grid 1024 x 2048, FP32 state, A=4.5, B=6.96,
 D1=2e-4, D2=8*D1, dt=0.002; initial state (A, B/A) + uniform(-0.01, 0.01)
 noise (mulberry32 seed 0, u's draws then v's), analysed to spectral on the
 GPU.  A step is the full IMEX Euler step compiled from the Brusselator .m
-file: 2 syntheses + 2 analyses (batched as one of each), 5 generated
-element-wise kernels, replayed from a HIP graph.  Inputs are resident in HBM
-before the timed region; the timed region is `--steps` steps bracketed by
-device synchronisation.
+file (2 syntheses, 2 analyses, 5 element-wise statements), planned as two
+launches: one generated grid kernel (row FFT synthesis of both species, the
+reaction statements, row FFT analysis) and the spectral op (Legendre
+analysis, the diagonal implicit solve and the next step's Legendre synthesis
+in one kernel).  The timed steps replay HIP graphs whose step count divides
+`--steps` (at most `--graph-steps`), captured before the timed region; inputs
+are resident in HBM; the timed region is `--steps` steps bracketed by device
+synchronisation.  Before timing, one synthesis -> analysis round trip of the
+initial state must come back within 1e-3 relative L2 (else exit status 3).
 
 Printed: ONE JSON line with the metric, a `roofline` object for the dominant
 kernel (algorithmic flops per launch / its mean duration measured with HIP
@@ -85,11 +90,11 @@ def time_stage(fn, stream, reps):
     return e0.elapsed_ms(e1) / reps
 
 
-def cpu_baseline(rows: int, nfield: int = 2):
+def cpu_sample(rows: int, threads: int, nfield: int = 2):
     """The oracle (f64 direct summation, reference.ts:37-118) on `rows`
-    latitude rows of one synthesis and one analysis per field, on all host
-    threads; every row is an independent slice of the transform, so the full
-    step costs nlat/rows times the sample."""
+    latitude rows of one synthesis and one analysis per field, on `threads`
+    host threads; every row is an independent slice of the transform, so the
+    full step costs nlat/rows times the sample."""
     import oracle as O
 
     cfg = O.Cfg(LMAX, LMAX, NLAT, NPHI)
@@ -97,13 +102,17 @@ def cpu_baseline(rows: int, nfield: int = 2):
     rng = np.random.default_rng(0)
     q = rng.normal(size=2 * cfg.nlm) / 100
     spat = rng.normal(size=NLAT * NPHI)
-    threads = O.get_threads()
-    t0 = time.perf_counter()
-    for _ in range(nfield):
-        ref.synth(q, rows=(0, rows))
-        gm = ref.fourier_analys(spat, rows=(0, rows))
-        ref.legendre_analys(gm, rows=(0, rows))
-    dt = time.perf_counter() - t0
+    prev = O.get_threads()
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        for _ in range(nfield):
+            ref.synth(q, rows=(0, rows))
+            gm = ref.fourier_analys(spat, rows=(0, rows))
+            ref.legendre_analys(gm, rows=(0, rows))
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_threads(prev)
     step_s = dt * (NLAT / rows)
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": (f"f64 direct-summation synth+analys of {nfield} fields on {rows} of "
@@ -112,7 +121,28 @@ def cpu_baseline(rows: int, nfield: int = 2):
                        f"work (<0.1% of it) not included")}
 
 
-def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: bool):
+def cpu_baseline(rows: int, rows_1t: int):
+    """All host threads (OpenMP) on `rows` rows, and one thread -- the
+    reproducible per-core figure -- on `rows_1t` rows."""
+    import oracle as O
+
+    out = cpu_sample(rows, O.get_threads())
+    out["single_thread"] = cpu_sample(rows_1t, 1)
+    return out
+
+
+def graph_size(steps: int, cap: int) -> int:
+    """The largest step count <= cap that divides `steps` (the timed steps are
+    whole graph replays)."""
+    return max(g for g in range(1, max(1, min(cap, steps)) + 1) if steps % g == 0)
+
+
+class SanityError(RuntimeError):
+    pass
+
+
+def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: bool,
+                latency_steps: int = 0):
     """Time the IMEX step with one Legendre implementation; per-kernel means;
     error of one synthesis of the initial state vs the f64 oracle."""
     import tsphere
@@ -128,16 +158,24 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
     gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"])
     gm.set_params(PARAMS)
-    gm.graph_steps = graph_steps
     ru, rv = initial_grid(npts, 0)
     u0 = (PARAMS["A"] + ru).astype(np.float32)
     v0 = (PARAMS["B"] / PARAMS["A"] + rv).astype(np.float32)
     U0 = sht.analys(u0)
     V0 = sht.analys(v0)
+    # sanity gate (scripts/bench-sht.ts:252-285): a round trip of the
+    # initial state through this plan must come back before anything is timed
+    rt = float(np.sqrt(np.sum((sht.analys(sht.synth(U0)) - U0.astype(np.float64)) ** 2) /
+                       np.sum(U0.astype(np.float64) ** 2)))
+    log(f"[bench] {legendre} round-trip check: relL2 {rt:.2e}")
+    if not rt < 1e-3:
+        raise SanityError(f"{legendre}: round trip relL2 {rt:.3e} >= 1e-3; not timing")
     gm.upload("U", U0)
     gm.upload("V", V0)
     log(f"[bench] step ops: {gm.describe()['step']}")
+    gm.graph_steps = graph_size(steps, graph_steps)
     gm.step(warmup)
+    gm.prepare(steps)   # capture the timed graphs outside the timed region
     gm.sync()
     t0 = time.perf_counter()
     gm.step(steps)
@@ -145,7 +183,25 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     elapsed = time.perf_counter() - t0
     U = gm.read("U")
     finite = bool(np.all(np.isfinite(U)))
-    log(f"[bench] {legendre}: {steps} steps in {elapsed * 1e3:.1f} ms -> {steps / elapsed:.1f} steps/s")
+    log(f"[bench] {legendre}: {steps} steps ({steps // gm.graph_steps} replays of a "
+        f"{gm.graph_steps}-step graph) in {elapsed * 1e3:.1f} ms -> {steps / elapsed:.1f} steps/s")
+    latency = None
+    if latency_steps > 0:
+        # one step per submit, each waited for (scripts/bench.ts:188-228)
+        gm.graph_steps = 0
+        gm.step(1)
+        gm.sync()
+        samples = np.empty(latency_steps)
+        for i in range(latency_steps):
+            t1 = time.perf_counter()
+            gm.step(1)
+            gm.sync()
+            samples[i] = (time.perf_counter() - t1) * 1e3
+        latency = {"samples": latency_steps, "meanMs": round(float(samples.mean()), 4),
+                   "medianMs": round(float(np.median(samples)), 4),
+                   "p90Ms": round(float(np.percentile(samples, 90)), 4),
+                   "minMs": round(float(samples.min()), 4)}
+        log(f"[bench] {legendre} one-step latency: {latency}")
 
     L = _lib.load()
     s = gm.stream
@@ -185,8 +241,11 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
             h, 2, fm.ptr, ptrs, lam.ptr, s1, s2, spc, s.handle)), s, 20)
     log(f"[bench] {legendre} kernel ms: {kms}")
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "graph_steps": graph_size(steps, graph_steps), "round_trip_rel_l2": rt,
            "kernel_ms": kms, "finite": finite,
            "tiles": {k: tuned[k] for k in ("synth", "analys")}}
+    if latency is not None:
+        out["latency"] = latency
     if err:
         import oracle as O
 
@@ -362,6 +421,9 @@ def main() -> int:
                     help="the variant reported as `value` (both are measured)")
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--cpu-rows-1t", type=int, default=96,
+                    help="latitude rows of the single-thread CPU sample")
+    ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip errors and batched extras")
     ap.add_argument("--hbm-calibration", action="store_true",
@@ -377,9 +439,14 @@ def main() -> int:
         return 2
 
     variants = {}
-    for leg in ("fp32", "f64rec", "bf16x3"):
-        variants[leg] = run_variant(leg, args.steps, args.warmup, args.graph_steps,
-                                    not args.quick)
+    try:
+        for leg in ("fp32", "f64rec", "bf16x3"):
+            variants[leg] = run_variant(leg, args.steps, args.warmup, args.graph_steps,
+                                        not args.quick,
+                                        args.latency_steps if leg == args.legendre else 0)
+    except SanityError as e:
+        log(f"[bench] sanity check failed: {e}")
+        return 3
     main_v = variants[args.legendre]
     kms = main_v["kernel_ms"]
     f_leg, f_fft = flops_per_transform(LMAX, NLAT, NPHI)
@@ -425,7 +492,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_rows} rows ...")
-        cpu = cpu_baseline(args.cpu_rows)
+        cpu = cpu_baseline(args.cpu_rows, args.cpu_rows_1t)
 
     line = {
         "metric": "IMEX steps/s + SHT GFLOP/s at lmax=1023 (split-BF16 MFMA vs plain FP32)",
@@ -447,6 +514,7 @@ def main() -> int:
         "sht_gflops": round(sht_flops_step * steps_s / 1e9, 2),
         "transforms_per_s": round(4 * steps_s, 2),
         "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
+        "latency": main_v.get("latency"),
         "hbm_gbs": hbm_gbs,
         "finite": main_v["finite"],
         "variants": {k: {kk: (round(vv, 8) if isinstance(vv, float) else vv)

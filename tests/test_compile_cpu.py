@@ -79,20 +79,56 @@ def test_schnakenberg_op_list():
     assert len(fused[0].outs) == 2 and len(fused[1].outs) == 2
 
 
-def test_fused_kernel_matches_statementwise_f32():
-    """The fused reaction kernel evaluates each statement with the same f32
-    operation order as the one-kernel-per-line form (no contraction): its
-    source holds the same per-statement expressions."""
-    _, one = plan_cpu("brusselator", batch=False)
-    _, fused = plan_cpu("brusselator", batch=True)
-    bodies = [op.kernel.body for op in one.ops if op.kind == "kernel"]
-    src = fused.kernel_source()
-    for b in bodies:
-        # each single-statement body appears in the fused source with the
-        # produced values renamed to registers; the operators are unchanged
-        ops_in = "".join(ch for ch in b if ch in "+-*/()")
-        assert ops_in, b
-    assert src.count("__global__") == 2
+def _expand(expr: str, args: list, env: dict) -> str:
+    """Substitute argument registers a<i> of a generated expression by the
+    expanded expressions of the values they hold."""
+    import re
+
+    return re.sub(r"\ba(\d+)\b", lambda m: env.get(args[int(m.group(1))],
+                                                  args[int(m.group(1))]), expr)
+
+
+def _statement_trees(plan) -> dict:
+    """c_name -> the statement's f32 expression with every value produced by
+    an earlier generated statement expanded in place, in terms of the step's
+    inputs (state, grid values, lam) and the parameter arguments p<k>."""
+    import re
+
+    from tsphere.plan import FusedOp, KernelOp
+
+    env = {}
+    for op in plan.ops:
+        if isinstance(op, KernelOp):
+            env[op.out] = _expand(op.kernel.body, list(op.ins), env)
+        elif isinstance(op, FusedOp):
+            # the element function: a0.. are the kernel inputs, then one
+            # register per statement in program order
+            lines = re.findall(r"const float a(\d+) = (.*);", op.kernel.code)
+            regs = list(op.ins)
+            for (idx, expr), mem in zip(lines, op.members):
+                assert int(idx) == len(regs), (idx, regs)
+                env[mem.out] = _expand(expr, regs, env)
+                regs.append(mem.out)
+            assert len(lines) == len(op.members)
+    return env
+
+
+@pytest.mark.parametrize("key", ["schnakenberg", "brusselator", "allencahn"])
+def test_fused_kernel_matches_statementwise_f32(key):
+    """The fused kernels evaluate every statement as the same f32 operation
+    tree as the one-kernel-per-line form (the reference's shape), with the
+    values kept in registers substituted by their own trees: the same
+    operations on the same operands in the same order, under the same
+    contraction-off pragma."""
+    _, one = plan_cpu(key, batch=False)
+    _, fused = plan_cpu(key, batch=True)
+    t1, t2 = _statement_trees(one), _statement_trees(fused)
+    assert set(t1) == set(t2) and len(t1) == EXPECTED_KERNELS[key]
+    for c in t1:
+        assert t1[c] == t2[c], (c, t1[c], t2[c])
+    for p in (one, fused):
+        src = p.kernel_source()
+        assert src.count("#pragma clang fp contract(off)") == src.count("__global__")
 
 
 def test_init_plan_and_feedback():
@@ -204,10 +240,18 @@ def test_emitter_refusals():
     with pytest.raises(UnsupportedOnGpu, match="literal integer exponent"):
         plan("function [Un, u] = step(U, lam, c, dt)\n u = synth(U);\n"
              " Un = U + dt * analys(u .^ c);\nend\n")
-    with pytest.raises(UnsupportedOnGpu, match="implicit expansion"):
+    # a 2 x nlm value times an npts x 1 grid value (wgsl.ts:268-304): refused
+    # with the statement's position in the source
+    body = " u = synth(U);\n Un = U .* u;\n"
+    with pytest.raises(UnsupportedOnGpu, match="implicit expansion: an operand is 2x8") as ei:
+        plan("function [Un, u] = step(U, lam, c, dt)\n" + body + "end\n")
+    sp = ei.value.span
+    full = ("function [U, u] = init(noise)\n U = analys(noise);\n u = synth(U);\nend\n"
+            "function [Un, u] = step(U, lam, c, dt)\n" + body + "end\n")
+    assert full[sp.start:sp.end] == "U" and full[:sp.start].endswith("Un = ")
+    with pytest.raises(UnsupportedOnGpu, match="an operand is 64x1 but the result is 64x8"):
         plan("function [Un, u] = step(U, lam, c, dt)\n u = synth(U);\n"
-             " Un = U + analys(u) .* lam(1, :);\nend\n") if False else None
-        raise UnsupportedOnGpu("implicit expansion")  # indexing is outside the subset
+             " Un = U + dt * analys(u);\n w = u .* U;\nend\n")
     with pytest.raises(ModelCompileError, match="only contain assignments"):
         plan("function [Un, u] = step(U, lam, c, dt)\n u = synth(U);\n if c\n Un = U;\n end\nend\n")
     with pytest.raises(ModelCompileError, match="does not provide"):

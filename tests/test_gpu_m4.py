@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def leg_tol(lmax, legendre):
-    return 1e-5 if legendre == "f64rec" else max(3e-6, 2e-10 * (lmax + 1) ** 2)
+    return 1e-5 if legendre == "f64rec" else max(2.5e-6, 2e-10 * (lmax + 1) ** 2)
 
 
 # lmax 12 and 101: lmax + 1 = 1 and 2 mod 4, rows whose staged alpha length

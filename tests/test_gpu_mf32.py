@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def leg_tol(lmax):
-    return max(3e-6, 2e-10 * (lmax + 1) ** 2)
+    return max(2.5e-6, 2e-10 * (lmax + 1) ** 2)
 
 
 _SMALL = [(15, 16, 32), (15, 17, 32), (12, 13, 26), (63, 64, 128), (127, 128, 256),

@@ -26,15 +26,27 @@ from conftest import emit
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("lmax,nlat,nphi", [(15, 16, 32), (63, 64, 128), (255, 256, 512)])
-@pytest.mark.parametrize("nfield", [1, 2, 3, 4])
+CASES = ([(lmax, nlat, nphi, nf, None) for lmax, nlat, nphi in
+          [(15, 16, 32), (63, 64, 128), (255, 256, 512)] for nf in (1, 2, 3, 4)] +
+         # the benchmarked shape (512 pairs, 8-wave workgroups) and 768 pairs
+         # (two 64-pair chains per lane), two fields; species maps that are
+         # not the identity (analysed field f updates state species[f])
+         [(1023, 1024, 2048, 2, None), (767, 1536, 2048, 2, None),
+          (63, 64, 128, 2, (1, 0)), (255, 256, 512, 3, (2, 0, 1)),
+          (1023, 1024, 2048, 2, (1, 0))])
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi,nfield,species", CASES)
 @pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
-def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield, legendre):
+def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield, species,
+                                                  legendre):
     import tsphere
     from tsphere import _lib
     from tsphere.device import DeviceBuffer
 
-    emit(f"spectral abi {legendre} lmax {lmax} {nlat}x{nphi} nfield {nfield}")
+    spc_map = list(species) if species is not None else list(range(nfield))
+    emit(f"spectral abi {legendre} lmax {lmax} {nlat}x{nphi} nfield {nfield} "
+         f"species {spc_map}")
     L = _lib.load()
     cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
     plan = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=4)
@@ -60,13 +72,15 @@ def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield,
     ptrs = (C.c_void_p * nfield)(*[b.ptr for b in xA])
     cs1 = (C.c_float * nfield)(*s1.tolist())
     cs2 = (C.c_float * nfield)(*s2.tolist())
-    spc = (C.c_int * nfield)(*range(nfield))
+    spc = (C.c_int * nfield)(*spc_map)
     _lib.check(L.tsphere_leg_spectral_imex(plan.handle, nfield, fmA.ptr, ptrs, lamb.ptr, cs1, cs2,
                                            spc, s.handle), "spectral")
     got_x = np.stack([b.download(s) for b in xA])
     got_fm = fmA.download(s)
 
-    # separate: analysis, host implicit step, synthesis
+    # separate: analysis, host implicit step, synthesis.  Analysed field f
+    # updates state species[f] with that field's scalars; the synthesis
+    # writes scratch slot = state index.
     fmB = DeviceBuffer(nfm, "fmB")
     fmB.upload(fm0, s)
     ab = DeviceBuffer(nfield * 2 * nlm, "A")
@@ -75,9 +89,10 @@ def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield,
     A = ab.download(s).reshape(nfield, 2 * nlm)
     xn = np.empty_like(x0)
     for f in range(nfield):
-        num = x0[f] + s1[f] * A[f]           # float32 ops, each rounded once
+        sp = spc_map[f]
+        num = x0[sp] + s1[f] * A[f]          # float32 ops, each rounded once
         den = np.float32(1.0) + s2[f] * lam
-        xn[f] = num / den
+        xn[sp] = num / den
     xb = DeviceBuffer(nfield * 2 * nlm, "xn")
     xb.upload(xn.reshape(-1), s)
     _lib.check(L.tsphere_leg_synth(plan.handle, nfield, xb.ptr, 2 * nlm, fmB.ptr, s.handle),

@@ -57,3 +57,27 @@ def test_addon_config_errors():
     assert out["te"] == "TypeError"   # plain numbers are not handles
     # an unknown Legendre form is refused, not mapped to FP32
     assert out["le"][0] == "TypeError" and "legendre must be" in out["le"][1]
+
+
+def test_addon_rejects_bad_species_and_stream():
+    """legSpectralImex takes integer state indices and a BigInt stream (or
+    none): fractional species and a non-BigInt stream are TypeErrors, raised
+    before anything reaches the library (the handles here are never used)."""
+    out = node(f"""const a = require({json.dumps(ADDON)});
+      const r = [];
+      const call = (species, stream) => {{
+        try {{ a.legSpectralImex(1n, 2n, [3n, 4n], 5n, [0.1, 0.2], [0.3, 0.4], species, stream);
+              r.push('called'); }}
+        catch (e) {{ r.push([e.name, e.message]); }}
+      }};
+      call([0.2, 1.9], undefined);
+      call([0, '1'], undefined);
+      call([0, 1], 'abc');
+      call([0, 1], 7);
+      let se = null;
+      try {{ a.legSynth(1n, [2n], 3n, {{}}); }} catch (e) {{ se = [e.name, e.message]; }}
+      console.log(JSON.stringify({{r, se}}));""")
+    r = out["r"]
+    assert r[0][0] == r[1][0] == "TypeError" and "integer state indices" in r[0][1]
+    assert r[2][0] == r[3][0] == "TypeError" and "stream must be a BigInt" in r[2][1]
+    assert out["se"][0] == "TypeError" and "stream must be a BigInt" in out["se"][1]

@@ -1697,12 +1697,18 @@ template <class P>
 hipError_t spectral_launch(const typename P::Tables& T, int nfield, float2* fm, const Fields& st,
                            const float* lam, const ImexCoef& ic, hipStream_t s) {
   if (nfield < 1 || nfield > 4) return hipErrorInvalidValue;
-  // one chain per wave when it covers the pairs, else two
+  // The shape the block-matrix analysis takes by default for one or two
+  // fields (resolve_leg_tile: up to 8 waves, then the fewest chains per lane
+  // that cover the pairs), so the chunks are dealt to the same waves and the
+  // waves' sums meet in the same order: a step through this kernel equals
+  // its separate launches bit for bit at every pair count it takes.
   const int npair = T.npair;
-  const int k = npair <= 64 * 8 ? 1 : npair <= 64 * 8 * 2 ? 2 : 0;
-  if (k == 0) return hipErrorInvalidValue;
+  const int nw0 = std::min(8, (npair + 63) / 64);
+  const int kk = (npair + 64 * nw0 - 1) / (64 * nw0);
+  if (kk > 2) return hipErrorInvalidValue;
+  const int k = kk <= 1 ? 1 : 2;
   // the combine takes one thread per sum of a span: 64 per field pair
-  const int nwaves = std::max(std::min(8, (npair + 64 * k - 1) / (64 * k)), nfield > 2 ? 2 : 1);
+  const int nwaves = std::max(nw0, nfield > 2 ? 2 : 1);
   if (nfield <= 2) {
     if (nfield == 1)
       return k == 1 ? spectral_m4<1, 1, P>(T, nfield, nwaves, fm, st, lam, ic, s)

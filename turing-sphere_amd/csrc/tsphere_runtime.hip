@@ -1252,6 +1252,14 @@ int tsphere_graph_end(tsphere_stream s, tsphere_graph* out) {
     (void)hipGraphDestroy(g);
     return hip_fail(e, "hipGraphInstantiate");
   }
+  // move the graph to the device now (enqueued on s), so that its first
+  // launch costs what every later one does
+  e = hipGraphUpload(x, S(s));
+  if (e != hipSuccess) {
+    (void)hipGraphExecDestroy(x);
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "hipGraphUpload");
+  }
   *out = new tsphere_graph_t{g, x};
   return ok();
 }

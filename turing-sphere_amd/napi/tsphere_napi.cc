@@ -232,7 +232,8 @@ napi_value Upload(napi_env env, napi_callback_info info) {
   void* data = nullptr;
   napi_get_typedarray_info(env, a.v[1], &t, &len, &data, nullptr, nullptr);
   if (t != napi_float32_array) return type_error(env, "upload: data must be a Float32Array");
-  if (a.argc >= 3) get_ptr(env, a.v[2], &s);
+  if (a.argc >= 3 && !get_ptr(env, a.v[2], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   if (int rc = tsphere_memcpy_htod(p, data, 4 * len, s)) return throw_error(env, rc, "upload");
   if (int rc = tsphere_stream_sync(s)) return throw_error(env, rc, "upload");
   return nullptr;
@@ -245,7 +246,8 @@ napi_value Download(napi_env env, napi_callback_info info) {
   const double n = a.argc >= 2 ? get_num(env, a.v[1], -1) : -1;
   if (a.argc < 2 || !get_ptr(env, a.v[0], &p) || n < 0)
     return type_error(env, "download(ptr, count, stream?)");
-  if (a.argc >= 3) get_ptr(env, a.v[2], &s);
+  if (a.argc >= 3 && !get_ptr(env, a.v[2], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   napi_value ab, out;
   void* data = nullptr;
   napi_create_arraybuffer(env, 4 * static_cast<size_t>(n), &data, &ab);
@@ -304,7 +306,8 @@ napi_value KernelLaunch(napi_env env, napi_callback_info info) {
     napi_get_element(env, a.v[3], i, &e);
     sc[i] = static_cast<float>(get_num(env, e));
   }
-  if (a.argc >= 5) get_ptr(env, a.v[4], &s);
+  if (a.argc >= 5 && !get_ptr(env, a.v[4], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   if (int rc = tsphere_kernel_launch(static_cast<tsphere_kernel>(k), static_cast<long long>(count),
                                      static_cast<int>(np), ptrs.data(), static_cast<int>(ns),
                                      sc.data(), s))
@@ -321,6 +324,26 @@ bool get_ptrs(napi_env env, napi_value arr, std::vector<void*>& out) {
     napi_value e;
     napi_get_element(env, arr, i, &e);
     if (!get_ptr(env, e, &out[i])) return false;
+  }
+  return true;
+}
+
+// Integers from a JS array: every entry a number with no fractional part.
+bool get_ints(napi_env env, napi_value arr, std::vector<int>& out) {
+  uint32_t n = 0;
+  if (napi_get_array_length(env, arr, &n) != napi_ok) return false;
+  out.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    napi_valuetype t;
+    napi_typeof(env, e, &t);
+    if (t != napi_number) return false;
+    double d = 0;
+    napi_get_value_double(env, e, &d);
+    if (!(d >= -2147483648.0 && d <= 2147483647.0) || d != static_cast<double>(static_cast<int>(d)))
+      return false;
+    out[i] = static_cast<int>(d);
   }
   return true;
 }
@@ -346,7 +369,8 @@ napi_value KernelLaunchGrid(napi_env env, napi_callback_info info) {
   if (a.argc < 6 || !get_ptr(env, a.v[0], &k) || !get_ptrs(env, a.v[4], ptrs) ||
       !get_floats(env, a.v[5], sc))
     return type_error(env, "kernelLaunchGrid(k, gx, gy, block, ptrs, scalars)");
-  if (a.argc >= 7) get_ptr(env, a.v[6], &s);
+  if (a.argc >= 7 && !get_ptr(env, a.v[6], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   if (int rc = tsphere_kernel_launch_grid(
           static_cast<tsphere_kernel>(k), static_cast<unsigned>(get_num(env, a.v[1], 0)),
           static_cast<unsigned>(get_num(env, a.v[2], 0)),
@@ -366,7 +390,8 @@ napi_value LegStage(napi_env env, napi_callback_info info) {
   if (a.argc < 3 || !get_ptr(env, a.v[0], &p) || !get_ptrs(env, a.v[qi], q) ||
       !get_ptr(env, a.v[fi], &fm))
     return type_error(env, SYNTH ? "legSynth(plan, qlmPtrs, fm)" : "legAnalys(plan, fm, qlmPtrs)");
-  if (a.argc >= 4) get_ptr(env, a.v[3], &s);
+  if (a.argc >= 4 && !get_ptr(env, a.v[3], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   const int n = static_cast<int>(q.size());
   const int rc =
       SYNTH ? tsphere_leg_synth_multi(static_cast<tsphere_plan>(p), n,
@@ -395,7 +420,8 @@ napi_value LegImexSynth(napi_env env, napi_callback_info info) {
   const size_t n = st.size();
   if (an.size() != n || s1.size() != n || s2.size() != n)
     return type_error(env, "legImexSynth: one analysed pointer, s1 and s2 per state field");
-  if (a.argc >= 8) get_ptr(env, a.v[7], &s);
+  if (a.argc >= 8 && !get_ptr(env, a.v[7], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   if (int rc = tsphere_leg_imex_synth(static_cast<tsphere_plan>(p), static_cast<int>(n),
                                       reinterpret_cast<const float* const*>(an.data()),
                                       reinterpret_cast<float* const*>(st.data()),
@@ -413,17 +439,19 @@ napi_value LegSpectralImex(napi_env env, napi_callback_info info) {
   Args a(env, info);
   void *p = nullptr, *fm = nullptr, *lam = nullptr, *s = nullptr;
   std::vector<void*> st;
-  std::vector<float> s1, s2, spf;
+  std::vector<float> s1, s2;
+  std::vector<int> spc;
   if (a.argc < 7 || !get_ptr(env, a.v[0], &p) || !get_ptr(env, a.v[1], &fm) ||
       !get_ptrs(env, a.v[2], st) || !get_ptr(env, a.v[3], &lam) || !get_floats(env, a.v[4], s1) ||
-      !get_floats(env, a.v[5], s2) || !get_floats(env, a.v[6], spf))
+      !get_floats(env, a.v[5], s2))
     return type_error(env, "legSpectralImex(plan, fm, statePtrs, lam, s1, s2, species)");
+  if (!get_ints(env, a.v[6], spc))
+    return type_error(env, "legSpectralImex: species must be integer state indices");
   const size_t n = st.size();
-  if (s1.size() != n || s2.size() != n || spf.size() != n)
+  if (s1.size() != n || s2.size() != n || spc.size() != n)
     return type_error(env, "legSpectralImex: one s1, s2 and species entry per state field");
-  std::vector<int> spc(n);
-  for (size_t i = 0; i < n; i++) spc[i] = static_cast<int>(spf[i]);
-  if (a.argc >= 8) get_ptr(env, a.v[7], &s);
+  if (a.argc >= 8 && !get_ptr(env, a.v[7], &s))
+    return type_error(env, "stream must be a BigInt (or undefined)");
   if (int rc = tsphere_leg_spectral_imex(static_cast<tsphere_plan>(p), static_cast<int>(n),
                                          static_cast<float*>(fm),
                                          reinterpret_cast<float* const*>(st.data()),

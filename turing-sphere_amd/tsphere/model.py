@@ -142,6 +142,10 @@ class GpuModel:
         self._step.run(self.stream, steps, self.graph_steps)
         self._last_ran = "step"
 
+    def prepare(self, steps: int = 1) -> None:
+        """Capture the graphs a ``step(steps)`` call replays, running nothing."""
+        self._step.prepare(self.stream, steps, self.graph_steps)
+
     def sync(self) -> None:
         self.stream.sync()
 

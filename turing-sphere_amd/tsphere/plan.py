@@ -761,6 +761,23 @@ class ModelPlan:
             g = self._graphs[k] = h.value
         return g
 
+    def prepare(self, stream: Stream, steps: int = 1, graph_steps: int = 0) -> None:
+        """Capture the graphs that ``run(stream, steps, graph_steps)`` replays,
+        without running them (a capture records work, it executes none), so
+        that a timed ``run`` only launches."""
+        if steps <= 0:
+            return
+        if graph_steps > 0:
+            if any(k != graph_steps for k in self._graphs):
+                self.invalidate()
+            if steps >= graph_steps:
+                self._graph(stream, graph_steps)
+            return
+        while steps > 0:
+            k = 1 << (min(steps, 64).bit_length() - 1)
+            self._graph(stream, k)
+            steps -= k
+
     def run(self, stream: Stream, steps: int = 1, graph_steps: int = 0) -> None:
         """Replay `steps` timesteps through HIP graphs.  `graph_steps` > 0:
         one graph of that many steps, the remainder recorded directly;
