@@ -1399,13 +1399,28 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
 // span i - 1 (solved in the previous iteration, published by this barrier).
 // The row's Fourier scratch entries are read at the row's start and its
 // synthesis written at its end, so the scratch is updated in place.
-// two 8-wave workgroups per CU need at most 128 VGPRs: the one-chain,
-// two-field form is held to that (it compiles to 132 with the FP32 recurrence;
-// the FP64 one spills 28 bytes per lane there and still runs its step
-// faster than at one workgroup per CU; the one- and four-field forms would
-// spill more)
+// Two 8-wave workgroups per CU need at most 128 VGPRs: the one-chain,
+// two-field form is held to that (the FP64-recurrence form spills only the
+// row's output addresses, outside the span loop).
+// Measured at lmax 1023 (two fields, ~110 us): the recurrence arithmetic is
+// not on the critical path (removing it leaves the time unchanged), nor are
+// the combine's loads; the matrix products, the tile stores and the span
+// barrier are (one more barrier per span: +15 us).  Spans grouped four to a
+// barrier, with the synthesis regenerating its recurrence values instead of
+// keeping them in registers, measured 130 us: the waves lose the overlap of
+// one span's synthesis with the next span's analysis.
 template <int NF, int K, class P>
 constexpr int m4_spectral_waves_per_eu() { return NF == 2 && K == 1 ? 4 : 1; }
+
+// a[f] for a lane-varying f < NF, as a select over the entries (which stay
+// in scalar registers, where a dynamic index would load from memory)
+template <int NF, class T>
+__device__ __forceinline__ T pick(const T (&a)[4], int f) {
+  T v = a[0];
+#pragma unroll
+  for (int i = 1; i < NF; ++i) v = f == i ? a[i] : v;
+  return v;
+}
 
 template <int NF, int K, class P>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_spectral_waves_per_eu<NF, K, P>())))
@@ -1415,7 +1430,9 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
   constexpr int NO = 64 * FP;  // combined sums per span
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ float red[2][8][NO];
-  __shared__ float qs[2][M4_SPAN][4 * FP];  // c_l X_new, (l, species, re/im)
+  // c_l X_new of a span, (species, re/im) column-major over the span's l:
+  // a lane's B operands of its column are 16 consecutive floats
+  __shared__ __attribute__((aligned(16))) float qs[2][4 * FP][M4_SPAN];
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
@@ -1426,6 +1443,19 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
   const bool odd = lg >> 1;
   float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) +
                 wv * (K * M4_SPAN * M4_TROW);
+  // this thread's coefficient of a span's combine (threads below NO); its
+  // field's scalars are selected from the argument registers with pick (a
+  // dynamic index would re-read them from argument memory after every
+  // barrier: the state stores between may alias it as far as the compiler
+  // knows)
+  const int cfp = tid % FP, cln = (tid / FP) & 63;
+  const int cjj = cln & 3, cbb = cln >> 2;
+  const int clo = m4_off(m4_lgroup(cbb & 3), cbb >> 2);
+  const int cf = 2 * cfp + (cjj >> 1);
+  const bool cthr = tid < NO && cf < NF;
+  const bool cfield = cthr && cf < nfield;
+  float* const cX = cfield ? reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[cf])))
+                           : nullptr;
 
   for (int r = 0; r < 2; ++r) {
     const int m = row_m(r, rows, M, false);
@@ -1503,8 +1533,11 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
       for (int fp = 0; fp < FP; ++fp) accE[k][fp] = accO[k][fp] = f4v{0.f, 0.f, 0.f, 0.f};
 
     // span ls's analysis into this wave's sums red[b][wv] and the chain
-    // values zc; false when every chain of the wave is still 0 there
-    auto analyse = [&](int ls, int b, float (&zc)[K][M4_SPAN]) -> bool {
+    // values zc; false when every chain of the wave is still 0 there.  INJ:
+    // lanes may take their seed in this span (spans past the chains' last
+    // seed run without the test)
+    auto analyse = [&](int ls, int b, float (&zc)[K][M4_SPAN], auto inj) -> bool {
+      constexpr bool INJ = decltype(inj)::value;
       float* rw = &red[b][wv][0];
       if (lmin >= ls + M4_SPAN) {
 #pragma unroll
@@ -1527,7 +1560,7 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
 #pragma unroll
           for (int p = pb; p < pb + AB; p += 2) {
             const int l = ls + p;
-            if (p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
+            if (INJ && p % kTile == 0 && ls <= lzmax && c[k].ls == l) {
               c[k].y0 = c[k].s0;
               c[k].y1 = c[k].s1;
             }
@@ -1548,16 +1581,22 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
-        float4 a4[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) a4[t] = *reinterpret_cast<const float4*>(tk + 4 * t);
+        // each 16-byte operand load issued one group ahead of the products
+        // that use it (the fences keep that order), so a wave waits for one
+        // LDS round trip per span here, not four
+        float4 cur = *reinterpret_cast<const float4*>(tk);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const float a[4] = {a4[t].x, a4[t].y, a4[t].z, a4[t].w};
+          float4 nxt = cur;
+          if (t < 3) nxt = *reinterpret_cast<const float4*>(tk + 4 * (t + 1));
+          __builtin_amdgcn_sched_barrier(0);
+          const float a[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int fp = 0; fp < FP; ++fp) acc[fp] = mf441(a[u], W[k][4 * t + u][fp], acc[fp]);
+          __builtin_amdgcn_sched_barrier(0);
+          cur = nxt;
         }
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -1581,34 +1620,30 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
     auto synthesise = [&](const float (&zp)[K][M4_SPAN], int b) {
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int p = 0; p < M4_SPAN; p += 2) {
-        float bE[FP], bO[FP];
+      for (int fp = 0; fp < FP; ++fp) {
+        // the lane's column of the span as four 16-byte loads, each issued
+        // one group of products ahead
+        const float4* q4 = reinterpret_cast<const float4*>(&qs[b][4 * fp + j][0]);
+        float4 cur = q4[0];
 #pragma unroll
-        for (int fp = 0; fp < FP; ++fp) {
-          const float* qa = &qs[b][p][4 * fp];
-          const float* qb = &qs[b][p + 1][4 * fp];
-          bE[fp] = (NF == 1 && j >= 2) ? 0.f : qa[j];
-          bO[fp] = (NF == 1 && j >= 2) ? 0.f : qb[j];
+        for (int t = 0; t < M4_SPAN / 4; ++t) {
+          float4 nxt = cur;
+          if (t < M4_SPAN / 4 - 1) nxt = q4[t + 1];
+          __builtin_amdgcn_sched_barrier(0);
+          float bq[4] = {cur.x, cur.y, cur.z, cur.w};
+          if (NF == 1 && j >= 2) bq[0] = bq[1] = bq[2] = bq[3] = 0.f;
+#pragma unroll
+          for (int u = 0; u < 4; u += 2)
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              accE[k][fp] = mf441(zp[k][4 * t + u], bq[u], accE[k][fp]);
+              accO[k][fp] = mf441(zp[k][4 * t + u + 1], bq[u + 1], accO[k][fp]);
+            }
+          __builtin_amdgcn_sched_barrier(0);
+          cur = nxt;
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-          for (int fp = 0; fp < FP; ++fp) {
-            accE[k][fp] = mf441(zp[k][p], bE[fp], accE[k][fp]);
-            accO[k][fp] = mf441(zp[k][p + 1], bO[fp], accO[k][fp]);
-          }
       }
-      __builtin_amdgcn_sched_barrier(0);
     };
-    // this thread's coefficient of a span's combine (threads below NO)
-    const int cfp = tid % FP, cln = (tid / FP) & 63;
-    const int cjj = cln & 3, cbb = cln >> 2;
-    const int clo = m4_off(m4_lgroup(cbb & 3), cbb >> 2);
-    const int cf = 2 * cfp + (cjj >> 1);
-    const bool cthr = tid < NO && cf < NF;
-    const bool cfield = cthr && cf < nfield;
-    float* const cX = cfield ? reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[cf])))
-                             : nullptr;
     // one pipeline iteration: span ls's analysis, the barrier, its combine and
     // implicit step beside the synthesis of the span before it (if any)
     auto iteration = [&](int ls, int b, float (&zc)[K][M4_SPAN], const float (&zp)[K][M4_SPAN],
@@ -1623,18 +1658,18 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
         lv = lam[ci];
         cz = T.cz[base + (l - m)];
       }
-      const bool live = analyse(ls, b, zc);
+      const bool live = analyse(ls, b, zc, std::true_type{});
       __syncthreads();  // the waves' sums of span ls; qs[b ^ 1] of the span before
       if (cthr) {
         float qv = 0.f;
         if (has) {
           float s = red[b][0][tid];
           for (int w = 1; w < nwaves; ++w) s += red[b][w][tid];
-          const float xn = imex_step(xo, ic.s1[cf], s * cz, ic.s2[cf], lv);
+          const float xn = imex_step(xo, pick<NF>(ic.s1, cf), s * cz, pick<NF>(ic.s2, cf), lv);
           cX[ci] = xn;
           qv = xn * cz;
         }
-        qs[b][clo][2 * (cf < nfield ? ic.spc[cf] : cf) + (cjj & 1)] = qv;
+        qs[b][2 * (cfield ? pick<NF>(ic.spc, cf) : cf) + (cjj & 1)][clo] = qv;
       }
       if (livep) synthesise(zp, b ^ 1);
       return live;
