@@ -144,6 +144,22 @@ int tsphere_synth_batch(tsphere_plan p, int nfield, const float *qlm, long long 
 int tsphere_analys_batch(tsphere_plan p, int nfield, const float *spat, long long spat_stride,
                          float *qlm, long long qlm_stride, tsphere_stream s);
 
+/* tsphere_synth_batch for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  No counterpart in the reference.  The same launches, with the
+ * Fourier-space intermediate in the library's static device scratch (4 Mi
+ * complex values = 32 MiB per GPU, held until the process ends) instead of
+ * the plan's; the call allocates nothing.  Limit: nfield*(mmax+1)*nlat <=
+ * 4194304 (e.g. 4 fields at lmax 1023 on 1024 latitudes), else
+ * TSPHERE_ERR_INVALID; nfield is not bounded by max_fields.  The plan's
+ * device must be current.  Every call on one GPU shares its scratch: calls
+ * that overlap on one GPU (different streams, or a graph replaying beside a
+ * direct call) race; the scratch is fully overwritten by each call, so
+ * nothing needs clearing between calls. */
+int tsphere_synth_batch_noalloc(tsphere_plan p, int nfield, const float *qlm,
+                                long long qlm_stride, float *spat, long long spat_stride,
+                                tsphere_stream s);
+
 /* Batched over a pointer list: field f reads qlm[f] / spat[f], nfield <= 8
  * (and <= max_fields).  This is how a step that transforms several separate
  * buffers -- synth(U) and synth(V) of a two-species model, whose two

@@ -105,6 +105,7 @@ struct Plan {
   LegTile tile[2] = {};  // [synthesis, analysis]; zeros = built-in heuristic
   float2* d_trig = nullptr;
   float2* d_fm = nullptr;  // Fourier scratch, max_fields * (mmax+1) * nlat
+  int device = 0;          // the GPU the plan's tables live on
   size_t bytes = 0;
   std::vector<double> theta, cos_theta, weights;
   LegTables tables(bool unit_w = false) const {

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -446,6 +447,32 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
 
 extern "C" {
 
+// ------------------------------------- the synthesis that allocates nothing
+// tsphere_synth_batch_noalloc keeps its Fourier-space intermediate here
+// instead of in the plan's scratch: 4 Mi complex values (32 MiB), which every
+// GPU the library runs on holds until the process ends.  Host code never
+// uses the array's name as an address: the address on each device comes from
+// hipGetSymbolAddress, taken once per device (with that device current) when
+// the first plan on it is created.
+constexpr size_t kNoAllocScratch = size_t(4) << 20;  // float2 entries
+static __device__ __align__(256) float2 scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static float2* g_scratch[kMaxDevices];  // per device: scratch's device address
+static std::mutex g_scratch_mu;
+
+// With device dev current: resolve its copy of scratch (once).
+static int resolve_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(TSPHERE_ERR_UNSUPPORTED, "device %d beyond the library's %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return TSPHERE_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch[dev] = static_cast<float2*>(ptr);
+  return TSPHERE_OK;
+}
+
 int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   if (!out) return fail(TSPHERE_ERR_INVALID, "out is NULL");
   *out = nullptr;
@@ -460,9 +487,17 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     return fail(TSPHERE_ERR_NODEVICE, "tsphere_plan_create: no GPU visible");
   }
 
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(TSPHERE_ERR_NODEVICE, "tsphere_plan_create: no current GPU");
+  }
+  if (int r = resolve_scratch(dev)) return r;
+
   auto* p = new (std::nothrow) tsphere_plan_t();
   if (!p) return fail(TSPHERE_ERR_INVALID, "out of host memory");
   p->cfg = *cfg;
+  p->device = dev;
   p->fourier = (cfg->fourier == TSPHERE_FOURIER_DFT || !fits) ? TSPHERE_FOURIER_DFT
                                                                : TSPHERE_FOURIER_FFT;
   p->npair = (cfg->nlat + 1) / 2;
@@ -697,6 +732,39 @@ int tsphere_analys_batch(tsphere_plan p, int nfield, const float* spat, long lon
     return r;
   return run_analys(p, nfield, strided(spat, 4 * spat_stride), strided(qlm, 4 * qlm_stride),
                     S(s));
+}
+
+// The synthesis of tsphere_synth_batch with its Fourier-space intermediate in
+// the static scratch above (no allocation, no plan scratch).
+int tsphere_synth_batch_noalloc(tsphere_plan p, int nfield, const float* qlm,
+                                long long qlm_stride, float* spat, long long spat_stride,
+                                tsphere_stream s) {
+  if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
+  if (nfield < 1) return fail(TSPHERE_ERR_INVALID, "nfield=%d must be >= 1", nfield);
+  if (!qlm || !spat) return fail(TSPHERE_ERR_INVALID, "NULL buffer");
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, nfield, "qlm_stride")) return r;
+  if (int r = check_stride(spat_stride, (long long)p->cfg.nlat * p->cfg.nphi, nfield,
+                           "spat_stride"))
+    return r;
+  const size_t need = (size_t)nfield * (p->cfg.mmax + 1) * p->cfg.nlat;
+  if (need > kNoAllocScratch)
+    return fail(TSPHERE_ERR_INVALID,
+                "synth_batch_noalloc: %d field(s) at mmax=%d, nlat=%d need %zu complex values "
+                "of Fourier scratch, more than its %zu (32 MiB)",
+                nfield, p->cfg.mmax, p->cfg.nlat, need, kNoAllocScratch);
+  int dev = -1;
+  TS_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != p->device)
+    return fail(TSPHERE_ERR_INVALID, "synth_batch_noalloc: the plan lives on device %d, "
+                "device %d is current", p->device, dev);
+  float2* fm = g_scratch[dev];  // resolved when the plan was created
+  if (!fm) return fail(TSPHERE_ERR_INVALID, "synth_batch_noalloc: no scratch on device %d", dev);
+  const Fields q = strided(qlm, 4 * qlm_stride), out = strided(spat, 4 * spat_stride);
+  TS_HIP(leg_synth(p, nfield, q, fm, S(s)), "leg_synth");
+  TS_HIP(launch_fourier_synth(p->fourier, p->cfg.mmax, p->cfg.nlat, p->cfg.nphi, p->d_trig,
+                              nfield, fm, out, S(s)),
+         "fourier_synth");
+  return ok();
 }
 
 int tsphere_synth_multi(tsphere_plan p, int nfield, const float* const* qlm, float* const* spat,

@@ -82,6 +82,7 @@ SIGNATURES = {
     "tsphere_synth": [_vp, _vp, _vp, _vp],
     "tsphere_analys": [_vp, _vp, _vp, _vp],
     "tsphere_synth_batch": [_vp, _i, _vp, _ll, _vp, _ll, _vp],
+    "tsphere_synth_batch_noalloc": [_vp, _i, _vp, _ll, _vp, _ll, _vp],
     "tsphere_analys_batch": [_vp, _i, _vp, _ll, _vp, _ll, _vp],
     "tsphere_synth_multi": [_vp, _i, C.POINTER(_vp), C.POINTER(_vp), _vp],
     "tsphere_analys_multi": [_vp, _i, C.POINTER(_vp), C.POINTER(_vp), _vp],
