@@ -129,10 +129,12 @@ def _f64_reference(x0):
     return _F64
 
 
-# GPU gradient against the f64 adjoint and the central differences, relative:
-# FP32 transforms measured 2.8e-5 (U0, V0), 2.5e-5 (a, b), 1.3e-4 (D1);
-# the FP64-recurrence form removes the transforms' share of it
-TOL_GRAD = {"fp32": 3e-4, "f64rec": 1e-4}
+# GPU gradient against the f64 adjoint and the central differences, relative.
+# Measured with FP32 transforms: 2.8e-5 (U0, V0), 2.5e-5 (a, b), 1.3e-4 (D1),
+# 1.0e-4 (D2); with the FP64 recurrence 2.4e-5 / 2.5e-5 / 1.5e-4 / 1.3e-4: the
+# error is the FP32 trajectory's (state and element-wise arithmetic), not the
+# transforms', and D1, D2 weight the damped high-degree modes by l(l+1)
+TOL_GRAD = {"fp32": 3e-4, "f64rec": 3e-4}
 
 
 @pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
