@@ -426,12 +426,18 @@ def main() -> int:
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip errors and batched extras")
+    ap.add_argument("--ensemble-only", type=int, default=0, metavar="STEPS",
+                    help="only run the 256-run mixed-lmax Allen-Cahn ensemble for STEPS steps "
+                         "(a short workload to trace its graph's kernels) and print its figures")
     ap.add_argument("--hbm-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (the known-size "
                          "workload that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
     args = ap.parse_args()
     if args.hbm_calibration:
         return hbm_calibration()
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench(args.ensemble_only), default=float), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the step runs on a "
               f"single MI355X; across GPUs it would shard by spectral rows m and latitude rows.",

@@ -233,6 +233,37 @@ __device__ __forceinline__ float2 analys_coef(const float2* z, int k,
   return cadd(xe, cmul(xo, w));
 }
 
+// The staging of stage_twiddles split in two, for an NT-thread workgroup:
+// load() reads this thread's entries into registers, store() writes them to
+// W (no barrier: the caller's next barrier publishes them).  A kernel issues
+// load() before its first pass, whose own row loads then overlap it.
+template <int LOG2H, int NT>
+struct TwiddleStage {
+  using S = FftShape<LOG2H>;
+  static constexpr int PER = (S::NTW + NT - 1) / NT;
+  float2 v[PER];
+  __device__ __forceinline__ void load(const float2* __restrict__ trig) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i >= S::NTW) break;
+      int Ns = S::NS0;
+      while (i >= 4 * Ns - S::NS0) Ns <<= 2;  // pass of entry i: offsets Ns - NS0
+      const int loc = i - (Ns - S::NS0);
+      const int k = loc / Ns + 1, jm = loc - (k - 1) * Ns;
+      v[p] = trig[k * jm * (S::N / (4 * Ns))];
+    }
+  }
+  __device__ __forceinline__ void store(float2* W) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i >= S::NTW) break;
+      W[i] = v[p];
+    }
+  }
+};
+
 // Stage the pass twiddle tables (FftShape::NTW entries) from the plan's
 // (cos, sin) table into the workgroup's LDS table W, then a barrier.  The
 // passes read their twiddles from W: a table read from global memory inside

@@ -50,7 +50,10 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat, int tiles
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
   __shared__ float2 W[S::NTW];
-  tsfft::stage_twiddles<LOG2H>(trig, W);
+  // pass twiddles read into registers now, stored to W after the first pass
+  // (which needs none); the first barrier of the remaining passes publishes them
+  tsfft::TwiddleStage<LOG2H, 256> tws;
+  tws.load(trig);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
@@ -59,7 +62,9 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat, int tiles
   const float2* F = fm + ((long long)f * nlat + (row ? ilat : nlat - 1)) * (mmax + 1);
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
-  const float2* res = tsfft::synth_row<LOG2H>(F, mmax, sc.mscale, trig, W, a, b, t);
+  tsfft::first_pass<+1, LOG2H>(tsfft::SynthInput<LOG2H>{F, trig, mmax, sc.mscale}, a, t);
+  tws.store(W);
+  const float2* res = tsfft::rest_passes<+1, LOG2H>(a, b, W, t);
   if (row) {
     float2* out = reinterpret_cast<float2*>(const_cast<char*>(Fs.at(f))) + (long long)ilat * S::H;
     const float w = sc.rowscale ? sc.rowscale[ilat] : 1.f;
@@ -80,7 +85,8 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
   __shared__ float2 W[S::NTW];
-  tsfft::stage_twiddles<LOG2H>(trig, W);
+  tsfft::TwiddleStage<LOG2H, 256> tws;  // as in k_fft_synth
+  tws.load(trig);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
@@ -90,7 +96,9 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
       reinterpret_cast<const float2*>(Fs.at(f)) + (long long)(row ? ilat : nlat - 1) * S::H;
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
-  const float2* z = tsfft::analys_row<LOG2H>([&](int j) { return in[j]; }, W, a, b, t);
+  tsfft::first_pass<-1, LOG2H>([&](int j) { return in[j]; }, a, t);
+  tws.store(W);
+  const float2* z = tsfft::rest_passes<-1, LOG2H>(a, b, W, t);
   if (row) {
     float2* G = fm + ((long long)f * nlat + ilat) * (mmax + 1);
     for (int k = t; k <= mmax; k += S::TPR) {

@@ -1583,12 +1583,15 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
         const float* tk = tile + k * (M4_SPAN * M4_TROW) + arow;
         // each 16-byte operand load issued one group ahead of the products
         // that use it (the fences keep that order), so a wave waits for one
-        // LDS round trip per span here, not four
+        // LDS round trip per span here, not four; the FP64-recurrence form,
+        // whose registers are full, loads each group where it is used
+        constexpr bool AHEAD = sizeof(typename P::ab_t) == 4;
         float4 cur = *reinterpret_cast<const float4*>(tk);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           float4 nxt = cur;
-          if (t < 3) nxt = *reinterpret_cast<const float4*>(tk + 4 * (t + 1));
+          if (AHEAD && t < 3) nxt = *reinterpret_cast<const float4*>(tk + 4 * (t + 1));
+          if (!AHEAD && t > 0) cur = *reinterpret_cast<const float4*>(tk + 4 * t);
           __builtin_amdgcn_sched_barrier(0);
           const float a[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll

@@ -718,7 +718,11 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
     ap("  constexpr int NPT = S::RB * S::H / (256 * G);  // complex points per thread")
     ap("  __shared__ float2 lds[G][2][S::RB * S::H];")
     ap("  __shared__ float2 W[S::NTW];")
-    ap(f"  tsfft::stage_twiddles<{log2h}>(trig, W);")
+    # the pass twiddles are read into registers first and stored after the
+    # first pass (which needs none): the two loads overlap, and the first
+    # barrier of the remaining passes publishes both
+    ap(f"  tsfft::TwiddleStage<{log2h}, 256 * G> tws;")
+    ap("  tws.load(trig);")
     ap("  const int run = blockIdx.y;")
     ap("  const long long grun = (long long)run * NLAT * S::H;  // a run's grid, float2 units")
     if ensemble:
@@ -737,6 +741,7 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
        f"(MMAX + 1), trig, MMAX, 1.0f}};")
     ap(f"    tsfft::first_pass<+1, {log2h}>(in, a, t);")
     ap("  }")
+    ap("  tws.store(W);")
     ap(f"  tsfft::rest_passes_masked<+1, {log2h}>(sact, a, b, W, t);")
     # element-wise over the block's points
 
