@@ -1400,8 +1400,9 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
 // The row's Fourier scratch entries are read at the row's start and its
 // synthesis written at its end, so the scratch is updated in place.
 // Two 8-wave workgroups per CU need at most 128 VGPRs: the one-chain,
-// two-field form is held to that (the FP64-recurrence form spills only the
-// row's output addresses, outside the span loop).
+// two-field form is held to that, without spills in either recurrence form
+// (the FP64 form reads its alpha four at a time and its tile operands where
+// used).
 // Measured at lmax 1023 (two fields, ~110 us): the recurrence arithmetic is
 // not on the critical path (removing it leaves the time unchanged), nor are
 // the combine's loads; the matrix products, the tile stores and the span
@@ -1697,7 +1698,12 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
     }
 
     // the synthesis of the row into the Fourier scratch (north = E + O,
-    // south = E - O), species s in field slot s
+    // south = E - O), species s in field slot s.  The offsets are formed
+    // here from an opaque copy of the row stride: hoisted to the kernel's
+    // start (as the compiler otherwise does) they would hold 16 VGPRs
+    // through the span loop
+    int M1 = M + 1;
+    asm volatile("" : "+s"(M1));
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int pb = pair_of(k) - lane + 4 * blk;
@@ -1705,14 +1711,14 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
       for (int fp = 0; fp < FP; ++fp) {
         const int s = 2 * fp + (j >> 1);
         if (s >= NF || s >= nfield) continue;
-        float* out = reinterpret_cast<float*>(fm + (long long)s * nlat * (M + 1) + m) + (j & 1);
+        float* out = reinterpret_cast<float*>(fm + (long long)s * nlat * M1 + m) + (j & 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int jp = pb + q;
           if (jp >= T.npair) continue;
           const float e = accE[k][fp][q], o = accO[k][fp][q];
-          out[2LL * jp * (M + 1)] = e + o;
-          out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+          out[2LL * jp * M1] = e + o;
+          out[2LL * (nlat - 1 - jp) * M1] = e - o;
         }
       }
     }
