@@ -31,6 +31,19 @@ def gpu():
     return True
 
 
+_MEASURED: list[str] = []
+
+
 def emit(msg: str) -> None:
-    """Print a progress line immediately (names the case before GPU work)."""
+    """Print a progress line immediately (names the case before GPU work) and
+    keep it for the run's summary, where a captured -q run still shows the
+    measured errors."""
     print(msg, flush=True)
+    _MEASURED.append(msg)
+
+
+def pytest_terminal_summary(terminalreporter):
+    if _MEASURED:
+        terminalreporter.section("measured")
+        for m in _MEASURED:
+            terminalreporter.write_line(m)

@@ -643,6 +643,30 @@ __device__ __forceinline__ f4v mf4(float a, float b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// The waves' partial sums of one output, from a buffer with a slot for each
+// of 8 waves (at(w): slot w), added in wave order -- the order of a loop over
+// the first nwaves slots -- with the reads issued four at a time (slots past
+// nwaves exist and are ignored): two LDS round trips instead of one per wave,
+// for four registers.
+template <class At>
+__device__ __forceinline__ float sum_waves(At at, int nwaves) {
+  float v[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) v[w] = at(w);
+  float s = v[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w)
+    if (w < nwaves) s += v[w];
+  if (nwaves > 4) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) v[w] = at(w + 4);
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+      if (w + 4 < nwaves) s += v[w];
+  }
+  return s;
+}
+
 // Synthesis: D[pair][col] = sum_l P[pair][l] B[l][col], B = Q masked by parity.
 // A row tile is 16 pairs; K-block kb of a 16-l tile holds l = lt + 4 (L/16) + kb
 // for lane L, so each lane reads its 4 K values as one 16-byte LDS load.
@@ -1367,8 +1391,7 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
         const int l = ls0 + h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
         const int f = 2 * fp + (jj >> 1);
         if (l <= L && f < NF && f0 + f < nfield) {
-          float s = red[buf][0][o];
-          for (int w = 1; w < nwaves; ++w) s += red[buf][w][o];
+          const float s = sum_waves([&](int w) { return red[buf][w][o]; }, nwaves);
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
           dst[jj & 1] = s * T.cz[base + (l - m)];
@@ -1667,6 +1690,8 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
       if (cthr) {
         float qv = 0.f;
         if (has) {
+          // (the analysis kernel's batched sum_waves measured slower here:
+          // this combine is not on the span's critical path)
           float s = red[b][0][tid];
           for (int w = 1; w < nwaves; ++w) s += red[b][w][tid];
           const float xn = imex_step(xo, pick<NF>(ic.s1, cf), s * cz, pick<NF>(ic.s2, cf), lv);
