@@ -4,9 +4,10 @@ grid 1024 x 2048, A=4.5, B=6.96, D1=2e-4, D2=8 D1, dt=0.002, initial state
 v's) analysed to spectral by the f64 oracle.
 
 * The default plan (grid kernel + spectral op: k_leg_spectral_m4 at 512
-  latitude pairs in 8-wave workgroups, the generated grid kernel at
-  nphi 2048) against the plan with the implicit solve folded into the next
-  synthesis and against the plan of separate launches, bit for bit, over
+  latitude pairs in 8-wave workgroups, k_leg_spectral_b4 in the split-BF16
+  form, the generated grid kernel at nphi 2048) against the plan with the
+  implicit solve folded into the next synthesis and against the plan of
+  separate launches, bit for bit, over
   graph replays, directly encoded remainders and a state upload between
   calls (the step order of src/mgpu/plan.ts:436-478).
 * The same step against the f64 trajectory (oracle, fast transforms): the
@@ -41,6 +42,12 @@ LMAX, NLAT, NPHI = 1023, 1024, 2048
 # size are checked in tests/test_gpu_m4.py and tests/test_gpu_f64rec.py.
 STATE_TOL = 1e-6
 INCR_TOL = 3e-4
+# The split-BF16 form carries ~5e-6 per transform (its precision target is
+# 1e-5 per transform; tests/test_gpu_mfma.py): the synthesised grid values
+# carry that, and the reaction's cancellation (terms of ~31 summing to ~0.05)
+# multiplies it by ~600 in the increment.  Measured after 10 steps: state
+# 2.5e-6 (U) and 8.0e-6 (V), increment 4.5e-3 (U) and 3.9e-3 (V).
+TOLS = {"fp32": (STATE_TOL, INCR_TOL), "f64rec": (STATE_TOL, INCR_TOL), "bf16x3": (2e-5, 1e-2)}
 
 _INIT = {}
 
@@ -103,7 +110,7 @@ def _run_plans(lmax, nlat, nphi, legendre, U0, V0, src="brusselator", params=C2,
     return outs[0]
 
 
-@pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
 def test_headline_step_plans_bit_exact(gpu, legendre):
     emit(f"headline step plans, Brusselator lmax {LMAX} {NLAT}x{NPHI} {legendre}")
     U0, V0, _ = _c2_initial()
@@ -111,7 +118,7 @@ def test_headline_step_plans_bit_exact(gpu, legendre):
     _run_plans(LMAX, NLAT, NPHI, legendre, U0, V0, expect=expect)
 
 
-@pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
 def test_headline_step_vs_f64_trajectory(gpu, legendre):
     steps = 10
     emit(f"headline step vs f64 trajectory, {steps} steps, {legendre}")
@@ -131,8 +138,9 @@ def test_headline_step_vs_f64_trajectory(gpu, legendre):
     ev = O.rel_l2(V.astype(np.float64) - V0, d64[1])
     su, sv = O.rel_l2(U, Ur), O.rel_l2(V, Vr)
     emit(f"  increment relL2 U {eu:.2e} V {ev:.2e}; state relL2 U {su:.2e} V {sv:.2e}")
-    assert su < STATE_TOL and sv < STATE_TOL
-    assert eu < INCR_TOL and ev < INCR_TOL
+    stol, itol = TOLS[legendre]
+    assert su < stol and sv < stol
+    assert eu < itol and ev < itol
 
 
 @pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
@@ -154,4 +162,16 @@ def test_beyond_spectral_limit_takes_fold(gpu):
     U0, V0, _ = _c2_initial(lmax, nlat, nphi)
     expect = {(True, False): "fold", (False, False): "separate"}
     _run_plans(lmax, nlat, nphi, "fp32", U0, V0, forms=((True, False), (False, False)),
+               expect=expect)
+
+
+def test_split_bf16_beyond_block_limit_separate(gpu):
+    """768 latitude pairs in the split-BF16 form: above the 4x4x4 block
+    kernels' 512 pairs the step keeps separate launches on the 16x16x32
+    column-tile kernels, and a spectral request plans exactly that."""
+    lmax, nlat, nphi = 767, 1536, 2048
+    emit(f"split-BF16 beyond the block kernels' limit: lmax {lmax} {nlat}x{nphi}")
+    U0, V0, _ = _c2_initial(lmax, nlat, nphi)
+    expect = {(True, False): "separate", (False, False): "separate"}
+    _run_plans(lmax, nlat, nphi, "bf16x3", U0, V0, forms=((True, False), (False, False)),
                expect=expect)

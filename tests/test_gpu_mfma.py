@@ -1,4 +1,6 @@
-"""The split-BF16 x3 MFMA Legendre stage vs the f64 oracle.
+"""The split-BF16 x3 MFMA Legendre stage vs the f64 oracle: the 4x4x4 block
+kernels (the built-in choice up to 512 latitude pairs) and the 16x16x32
+column-tile kernels.
 
 Stated tolerance: 1e-5 relative spectral (or grid) L2 against FP64 for a
 single transform, at every lmax up to 1023 -- the precision target of the
@@ -23,12 +25,22 @@ def _plan(cfg, **kw):
                                   legendre="bf16x3", **kw)
 
 
+# tile None: the built-in 4x4x4 block kernels; ctc: the 16x16x32 column-tile
+# kernels (the form's choice above 512 latitude pairs, and a tuner candidate)
+@pytest.mark.parametrize("tile", [None, 1])
 @pytest.mark.parametrize("lmax,nlat,nphi", [(15, 16, 32), (15, 17, 32), (63, 64, 128),
                                             (127, 192, 256), (255, 256, 512)])
-def test_stages_vs_oracle_bf16x3(gpu, lmax, nlat, nphi):
+def test_stages_vs_oracle_bf16x3(gpu, lmax, nlat, nphi, tile):
     cfg = O.Cfg(lmax, lmax, nlat, nphi)
-    emit(f"bf16x3 stages lmax{lmax} {nlat}x{nphi}")
+    emit(f"bf16x3 stages lmax{lmax} {nlat}x{nphi} tile {'blocks' if tile is None else 'ctc 1'}")
     plan = _plan(cfg)
+    if tile is not None:
+        plan.set_tile("synth", ctc=tile)
+        plan.set_tile("analys", ctc=tile)
+    want = {"impl": 4} if tile is None else {"ctc": tile}
+    for stage in ("synth", "analys"):
+        got = plan.get_tile(stage, 1)
+        assert all(got[k] == v for k, v in want.items()), (stage, got)
     ref = O.ShtReference(cfg)
     q = _random_field_spectrum(cfg, 11)
     s = plan.stream
@@ -78,11 +90,11 @@ def test_batched_many_fields_bf16x3(gpu):
     plan.destroy()
 
 
-@pytest.mark.parametrize("lmax", [511, 1023])
+@pytest.mark.parametrize("lmax", [511, 767, 1023])
 def test_precision_form_large(gpu, lmax):
     """lmax up to 1023 (nlat = lmax+1, nphi = 2(lmax+1)): synthesis vs the
     f64 oracle and round trip within 1e-5 -- where the plain FP32 recurrence
-    is ~1e-4 off."""
+    is ~1e-4 off (lmax 767: 384 pairs in 6-wave block workgroups)."""
     cfg = O.Cfg(lmax, lmax, lmax + 1, 2 * (lmax + 1))
     emit(f"bf16x3 precision lmax{lmax}")
     plan = _plan(cfg, max_fields=2)

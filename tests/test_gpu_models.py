@@ -297,7 +297,9 @@ def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nl
 @pytest.mark.parametrize("key,lmax,nlat,nphi,legendre", [
     ("brusselator", 31, 32, 64, "fp32"), ("schnakenberg", 15, 17, 32, "fp32"),
     ("allencahn", 63, 64, 128, "fp32"), ("brusselator", 127, 128, 256, "f64rec"),
-    ("schnakenberg", 255, 256, 512, "fp32"), ("brusselator", 255, 257, 512, "f64rec")])
+    ("schnakenberg", 255, 256, 512, "fp32"), ("brusselator", 255, 257, 512, "f64rec"),
+    ("brusselator", 31, 32, 64, "bf16x3"), ("schnakenberg", 15, 17, 32, "bf16x3"),
+    ("allencahn", 127, 128, 256, "bf16x3"), ("schnakenberg", 255, 257, 512, "bf16x3")])
 def test_spectral_op_matches_grid_plan(gpu, key, lmax, nlat, nphi, legendre):
     """The spectral op (Legendre analysis, the implicit solve and the next
     step's Legendre synthesis in one kernel per step) gives the state and the

@@ -36,10 +36,16 @@ CASES = ([(lmax, nlat, nphi, nf, None) for lmax, nlat, nphi in
           (1023, 1024, 2048, 2, (1, 0))])
 
 
-@pytest.mark.parametrize("lmax,nlat,nphi,nfield,species", CASES)
-@pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
-def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield, species,
-                                                  legendre):
+# every case in the FP32 and FP64-recurrence forms; the split-BF16 form's
+# spectral op covers up to 512 latitude pairs (its refusal above that is
+# tested below)
+PARAMS = [(leg,) + c for leg in ("fp32", "f64rec", "bf16x3") for c in CASES
+          if leg != "bf16x3" or (c[1] + 1) // 2 <= 512]
+
+
+@pytest.mark.parametrize("legendre,lmax,nlat,nphi,nfield,species", PARAMS)
+def test_spectral_entry_matches_separate_launches(gpu, legendre, lmax, nlat, nphi, nfield,
+                                                  species):
     import tsphere
     from tsphere import _lib
     from tsphere.device import DeviceBuffer
@@ -103,6 +109,36 @@ def test_spectral_entry_matches_separate_launches(gpu, lmax, nlat, nphi, nfield,
     ef = O.rel_l2(got_fm, want_fm)
     emit(f"  state {ex:.2e}  scratch {ef:.2e}")
     assert np.all(np.isfinite(got_fm)) and ex < 2e-6 and ef < 2e-6
+    plan.destroy()
+
+
+def test_split_bf16_spectral_entry_refuses_beyond_block_limit(gpu):
+    """A split-BF16 plan with 768 latitude pairs: the spectral entry point
+    and the fused implicit step return TSPHERE_ERR_INVALID with a message
+    (no launch), as the planner expects."""
+    import tsphere
+    from tsphere import _lib
+    from tsphere.device import DeviceBuffer
+
+    lmax, nlat, nphi = 767, 1536, 2048
+    emit(f"split-BF16 spectral refusal lmax {lmax} {nlat}x{nphi}")
+    L = _lib.load()
+    plan = tsphere.ShtPlan.create(tsphere.ShtConfig(lmax, lmax, nlat, nphi), legendre="bf16x3",
+                                  max_fields=2)
+    nlm = plan.nlm
+    fm = DeviceBuffer(2 * 2 * (lmax + 1) * nlat, "fm")
+    x = [DeviceBuffer(2 * nlm, f"x{f}") for f in range(2)]
+    lam = DeviceBuffer(2 * nlm, "lam")
+    ptrs = (C.c_void_p * 2)(*[b.ptr for b in x])
+    s1 = (C.c_float * 2)(0.002, 0.002)
+    s2 = (C.c_float * 2)(1e-6, 1e-6)
+    spc = (C.c_int * 2)(0, 1)
+    rc = L.tsphere_leg_spectral_imex(plan.handle, 2, fm.ptr, ptrs, lam.ptr, s1, s2, spc,
+                                     plan.stream.handle)
+    assert rc == _lib.ERR_INVALID and "512 latitude pairs" in _lib.last_error()
+    rc = L.tsphere_leg_imex_synth(plan.handle, 2, ptrs, ptrs, lam.ptr, s1, s2, fm.ptr,
+                                  plan.stream.handle)
+    assert rc == _lib.ERR_INVALID and "512 latitude pairs" in _lib.last_error()
     plan.destroy()
 
 

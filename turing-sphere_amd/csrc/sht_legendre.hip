@@ -1853,6 +1853,630 @@ hipError_t analys_m4_nf(const typename P::Tables& T, int nfield, const LegTile& 
   }
 }
 
+// ------------------------------ split-BF16 x3 on 4x4x4 matrix blocks (b4)
+// v_mfma_f32_4x4x4_16b_bf16 keeps the block and output layout of the 4x4x1
+// FP32 product above (A_b[r][k] = the k-th bf16 of lane 4b + r, B_b[k][j] =
+// the k-th bf16 of lane 4b + j, D_b[r][j] in register r of lane 4b + j;
+// checked on gfx950) and its issue cost (~4 ns per instruction per SIMD,
+// measured) for four times the multiply-adds.  K runs over four l of one
+// parity of l - m in the synthesis (a lane's own pair, so no operand moves
+// between lanes) and over four latitude pairs in the analysis.  Every operand
+// is split x = hi + lo into bf16 values rounded to nearest even, and a
+// product is lo hi + hi lo + hi hi accumulated in FP32 (lo lo, below 2^-16
+// relative, is dropped).  The recurrence is the FP64 one (RecD on the
+// LegTablesZ64 tables); each value is rounded to f32 once, then split.
+// Groups of four l of one parity are taken from 8-l blocks aligned to
+// l - m = 0 mod 8 in every kernel (chains are exactly 0 before their seeds,
+// so starting earlier adds exact zeros): the synthesis, the analysis and
+// the spectral op then sum the same groups in the same order, and a step
+// through the spectral op equals its separate launches bit for bit.
+// One chain per lane (up to 512 latitude pairs with 8 waves).
+typedef __attribute__((ext_vector_type(4))) short s4v;
+
+__device__ __forceinline__ short bf16_rn(float v) { return __builtin_bit_cast(short, (__bf16)v); }
+__device__ __forceinline__ float bf16_f32(short h) {
+  return __uint_as_float((unsigned)(unsigned short)h << 16);
+}
+struct HL {
+  short hi, lo;
+};
+// The split of the f32 value v itself, in every kernel alike: v is made
+// opaque first, since for v = (float)d of a double the compiler may fold the
+// two roundings into one correctly rounded double -> bf16 conversion (a
+// round-to-odd sequence of ~10 VALU ops per value, whose hi differs from
+// bf16(float(d)) near ties), and lo = v - hi is rounded as written (a
+// product v must not be fused into the subtraction).  Either would make the
+// kernels disagree bit for bit.
+__device__ __forceinline__ HL split_bf16(float v) {
+#pragma clang fp contract(off)
+  asm("" : "+v"(v));
+  const short h = bf16_rn(v);
+  return HL{h, bf16_rn(v - bf16_f32(h))};
+}
+// four values as the hi and lo operands of one K = 4 product
+struct __attribute__((aligned(16))) Split4 {
+  s4v hi, lo;
+};
+__device__ __forceinline__ Split4 split4(float a, float b, float c, float d) {
+  const HL x = split_bf16(a), y = split_bf16(b), z = split_bf16(c), w = split_bf16(d);
+  return Split4{s4v{x.hi, y.hi, z.hi, w.hi}, s4v{x.lo, y.lo, z.lo, w.lo}};
+}
+__device__ __forceinline__ f4v mb3(const Split4& a, const Split4& b, f4v c) {
+  c = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a.lo, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a.hi, b.lo, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a.hi, b.hi, c, 0, 0, 0);
+}
+// group (E0, O0, E1, O1) and position k of offset p of a 16-l span whose
+// start has l - m = 0 mod 8
+__device__ __forceinline__ constexpr int b4_group(int p) { return ((p >> 3) << 1) | (p & 1); }
+__device__ __forceinline__ constexpr int b4_pos(int p) { return (p & 7) >> 1; }
+
+// Stage a row's alpha (f64) and its split synthesis coefficients c_l Q_l
+// (IMEX: of X_new = (X + s1 A) / (1 + s2 lam), written back to the state as
+// stage_row_imex does): 8-l block t, parity par, column col (4 fp + j) at
+// shorts ((t 2 + par) NC + col) 8 + {k: hi, 4 + k: lo}, l - m = 8 t + 2 k + par.
+template <int NF, bool IMEX>
+__device__ __forceinline__ void stage_row_b4(const LegTablesZ64& T, int m, int f0, int nfield,
+                                             const Fields& Fq, const Fields& Fa, const float* lam,
+                                             const ImexCoef& ic, double* sab, short* sqb, int nt) {
+  constexpr int NC = 4 * ((NF + 1) / 2);
+  const int n = T.lmax - m + 1;
+  const long long base = lm_base(T.lmax, m);
+  for (int i = threadIdx.x; i < 8 * nt + kPadQ; i += blockDim.x)
+    sab[i] = i < n ? T.ab[base + i] : 0.0;
+  for (int i = threadIdx.x; i < 8 * nt; i += blockDim.x) {
+    const float cz = i < n ? T.cz[base + i] : 0.f;
+    short* dst = sqb + ((i >> 3) * 2 + (i & 1)) * (NC * 8) + ((i & 7) >> 1);
+#pragma unroll
+    for (int f = 0; f < NC / 2; ++f) {
+      const int fld = f0 + f;
+      float2 v = make_float2(0.f, 0.f);
+      if (f < NF && i < n && fld < nfield) {
+        if constexpr (IMEX) {
+          float2* X = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(fld))) + base + i;
+          const float2 x = *X;
+          const float2 a = reinterpret_cast<const float2*>(Fa.at(fld))[base + i];
+          const float2 lm2 = reinterpret_cast<const float2*>(lam)[base + i];
+          float2 xn;
+          xn.x = imex_step(x.x, ic.s1[fld], a.x, ic.s2[fld], lm2.x);
+          xn.y = imex_step(x.y, ic.s1[fld], a.y, ic.s2[fld], lm2.y);
+          *X = xn;
+          v = make_float2(xn.x * cz, xn.y * cz);
+        } else {
+          const float2 q = reinterpret_cast<const float2*>(Fq.at(fld))[base + i];
+          v = make_float2(q.x * cz, q.y * cz);
+        }
+      }
+      const HL re = split_bf16(v.x), im = split_bf16(v.y);
+      dst[(2 * f) * 8] = re.hi;
+      dst[(2 * f) * 8 + 4] = re.lo;
+      dst[(2 * f + 1) * 8] = im.hi;
+      dst[(2 * f + 1) * 8 + 4] = im.lo;
+    }
+  }
+}
+
+// One 8-l block of the synthesis from l - m = i (8-aligned): the recurrence
+// of the 8 values, their split, then the products as one group (the matrix
+// products and the VALU do not overlap on gfx950).
+template <bool INJ, int NF>
+__device__ __forceinline__ void synth_tile_b4(const double* sab, const short* sqb, int m, int i,
+                                              int j, ChainT<RecD>& c, f4v (&accE)[(NF + 1) / 2],
+                                              f4v (&accO)[(NF + 1) / 2]) {
+  constexpr int FP = (NF + 1) / 2, NC = 4 * FP;
+  const short* tq = sqb + (i >> 3) * (2 * NC * 8);
+  Split4 bE[FP], bO[FP];
+#pragma unroll
+  for (int fp = 0; fp < FP; ++fp) {
+    bE[fp] = *reinterpret_cast<const Split4*>(tq + (4 * fp + j) * 8);
+    bO[fp] = *reinterpret_cast<const Split4*>(tq + (NC + 4 * fp + j) * 8);
+  }
+  double al[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) al[p] = sab[i + p + 2];
+  float z[8];
+  const int l = m + i;
+#pragma unroll
+  for (int p = 0; p < 8; p += 2) {
+    if (INJ && (p & 3) == 0 && c.ls == l + p) {
+      c.y0 = c.s0;
+      c.y1 = c.s1;
+    }
+    z[p] = (float)c.y0;
+    z[p + 1] = (float)c.y1;
+    RecD::advance(al[p], al[p + 1], c.ct, c.y0, c.y1);
+  }
+  const Split4 aE = split4(z[0], z[2], z[4], z[6]);
+  const Split4 aO = split4(z[1], z[3], z[5], z[7]);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int fp = 0; fp < FP; ++fp) {
+    accE[fp] = mb3(aE, bE[fp], accE[fp]);
+    accO[fp] = mb3(aO, bO[fp], accO[fp]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+__host__ __device__ constexpr int b4_tiles(int lmax) { return (lmax + 8) / 8; }
+
+// Synthesis (IMEX: the implicit step of every field in front, as
+// k_leg_synth_m4<..., true>): the 8-l blocks of the row from the wave's first
+// seed (rounded down to 8 in l - m), with the seed test up to its last seed.
+template <int NF, bool IMEX = false>
+__global__ __launch_bounds__(512) void k_leg_synth_b4(LegTablesZ64 T, int nfield, Fields Fq,
+                                                      float2* __restrict__ fm, int rows,
+                                                      Fields Fa, const float* lam, ImexCoef ic) {
+  using P = RecD;
+  constexpr int FP = (NF + 1) / 2;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  double* sab = reinterpret_cast<double*>(lds_raw);
+  short* sqb = reinterpret_cast<short*>(sab + 8 * b4_tiles(L) + kPadQ);
+  const int f0 = blockIdx.y * NF;
+  const int lane = threadIdx.x & 63, j = lane & 3, blk = lane >> 2;
+
+  for (int r = 0; r < 2; ++r) {
+    const int m = row_m(r, rows, M);
+    if (m < 0) break;
+    const int n = L - m + 1;
+    __syncthreads();  // the previous row's readers are done
+    stage_row_b4<NF, IMEX>(T, m, f0, nfield, Fq, Fa, lam, ic, sab, sqb, (n + 7) / 8);
+    __syncthreads();
+
+    ChainT<P> c;
+    load_chain_t<P>(T, m, pair_of(0), c);
+    const int lk = wave_min(c.ls), lz = last_seed(c.ls, L);
+    f4v accE[FP], accO[FP];
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) accE[fp] = accO[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+    if (lk <= L) {
+      int i = (lk - m) & ~7;
+      for (; i < n && m + i <= lz; i += 8) synth_tile_b4<true, NF>(sab, sqb, m, i, j, c, accE, accO);
+      for (; i < n; i += 8) synth_tile_b4<false, NF>(sab, sqb, m, i, j, c, accE, accO);
+    }
+
+    const int pb = pair_of(0) - lane + 4 * blk;  // pair of block row 0
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) {
+      const int f = 2 * fp + (j >> 1);
+      if (f >= NF || f0 + f >= nfield) continue;
+      float* out = reinterpret_cast<float*>(fm + (long long)(f0 + f) * nlat * (M + 1) + m) + (j & 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int jp = pb + q;
+        if (jp >= T.npair) continue;
+        const float e = accE[fp][q], o = accO[fp][q];
+        out[2LL * jp * (M + 1)] = e + o;
+        out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+      }
+    }
+  }
+}
+
+// Analysis weights of a row for the b4 kernels: each lane computes W =
+// (G_N +/- G_S) w of its own pair (every column, both parities) into its
+// wave's tile, then gathers and splits the 16 pairs of its block's
+// pair-group for its column and parity, four pairs per K group.
+template <int NF>
+__device__ __forceinline__ void b4_weights(const LegTablesZ64& T, const float2* fm, int m, int f0,
+                                           int nfield, float* tile, int lane, int j, int pg,
+                                           bool odd, Split4 (&W)[4][(NF + 1) / 2]) {
+  constexpr int FP = (NF + 1) / 2;
+  const int M = T.mmax, nlat = T.nlat;
+  const int jp = pair_of(0);
+  const int jc = min(jp, T.npair - 1);
+  const int js = nlat - 1 - jc;
+  const float w = jp < T.npair ? T.w[jc] : 0.f;
+  float* tw = tile + lane;  // [parity][fp][col][pair]
+#pragma unroll
+  for (int f = 0; f < 2 * FP; ++f) {
+    const bool live = f < NF && f0 + f < nfield;
+    const int fld = min(f0 + min(f, NF - 1), nfield - 1);
+    const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
+    const float2 gN = g[(long long)jc * (M + 1)], gS = g[(long long)js * (M + 1)];
+    float4 we{0.f, 0.f, 0.f, 0.f};
+    if (live) {
+      if (js == jc)  // equator of an odd grid pairs with itself
+        we = make_float4(gN.x * w, gN.y * w, 0.f, 0.f);
+      else
+        we = make_float4((gN.x + gS.x) * w, (gN.y + gS.y) * w, (gN.x - gS.x) * w,
+                         (gN.y - gS.y) * w);
+    }
+    const int fp = f >> 1, c0 = 2 * (f & 1);
+    tw[((0 * FP + fp) * 4 + c0) * M4_WROW] = we.x;
+    tw[((0 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.y;
+    tw[((1 * FP + fp) * 4 + c0) * M4_WROW] = we.z;
+    tw[((1 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.w;
+  }
+  wave_sync_lds();
+#pragma unroll
+  for (int fp = 0; fp < FP; ++fp) {
+    const float* src = tile + (((odd ? FP : 0) + fp) * 4 + j) * M4_WROW + 16 * pg;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      W[t][fp] = split4(src[4 * t], src[4 * t + 1], src[4 * t + 2], src[4 * t + 3]);
+  }
+  wave_sync_lds();  // the tile is rewritten by the spans
+}
+
+// A span's tile as bf16: physical row m4_row(p) of span offset p, the pairs
+// in quads of [hi x 4, lo x 4] (16 bytes), so that the 16-byte operand load
+// of lane 16 pg + 4 q + r (row 4 q + r, pairs 16 pg + 4 t ..) is the A
+// operand of K group t.  A row is 64 pairs x 4 bytes, as the FP32 tile's.
+__device__ __forceinline__ void b4_tile_store(short* tks, int p, int lane, short hi, short lo) {
+  short* d = tks + m4_row(p) * (2 * M4_TROW) + (lane >> 2) * 8 + (lane & 3);
+  d[0] = hi;
+  d[4] = lo;
+}
+
+// The analysis products of one span: block (l-group, pair-group), K groups
+// t = 0..3 of four pairs each.
+template <int FP>
+__device__ __forceinline__ void b4_span_products(const short* tks, int blk, int j, int pg,
+                                                 const Split4 (&W)[4][FP], f4v (&acc)[FP]) {
+  const short* ta = tks + (4 * (blk & 3) + j) * (2 * M4_TROW) + 32 * pg;
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const Split4 a = *reinterpret_cast<const Split4*>(ta + 8 * t);
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) acc[fp] = mb3(a, W[t][fp], acc[fp]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Sum of the block rows over the 4 pair-groups (as in the FP32 kernels):
+// lane 4 b + j ends with the sum for row pg of its block.
+__device__ __forceinline__ float b4_pair_group_sum(const f4v& a) {
+  float v[4] = {a[0], a[1], a[2], a[3]};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[i + 2]),
+                                                     false, false);
+    v[i] = __uint_as_float(s2[0]) + __uint_as_float(s2[1]);
+  }
+  const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]), __float_as_uint(v[1]),
+                                                   false, false);
+  return __uint_as_float(s1[0]) + __uint_as_float(s1[1]);
+}
+
+// The first span of a row for the b4 kernels: the workgroup's first seed,
+// rounded down to l - m = 0 mod 8 (L + 1: the row is 0).
+__device__ __forceinline__ int b4_row_start(const int* smin, int nwaves, int m, int L) {
+  int lblk = smin[0];
+  for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+  return lblk <= L ? m + ((lblk - m) & ~7) : L + 1;
+}
+
+// Analysis: k_leg_analys_m4's spans (one chain per lane) with the tile in
+// bf16 and the products as split 4x4x4 blocks.
+template <int NF>
+__global__ __launch_bounds__(512) void k_leg_analys_b4(LegTablesZ64 T, int nfield,
+                                                       const float2* __restrict__ fm, Fields Fq,
+                                                       int rows) {
+  using P = RecD;
+  constexpr int FP = (NF + 1) / 2;
+  constexpr int NH = 2;  // spans per workgroup barrier
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ float red[2][8][NH * 64 * FP];
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax;
+  double* sab = reinterpret_cast<double*>(lds_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
+  const int j = lane & 3, blk = lane >> 2;
+  const int pg = blk >> 2;
+  const bool odd = m4_lgroup(blk & 3) >> 1;
+  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) + wv * (M4_SPAN * M4_TROW);
+  short* tks = reinterpret_cast<short*>(tile);
+  const int f0 = blockIdx.y * NF;
+
+  for (int r = 0; r < 2; ++r) {
+    const int m = row_m(r, rows, M, false);
+    if (m < 0) break;
+    const long long base = lm_base(L, m);
+    __syncthreads();
+    stage_row_t<NF, P>(T, m, f0, nfield, Fq, false, sab, nullptr, kPadAnalys);
+    ChainT<P> c;
+    load_chain_t<P>(T, m, pair_of(0), c);
+    const int lmin = wave_min(c.ls), lzmax = last_seed(c.ls, L);
+    Split4 W[4][FP];
+    b4_weights<NF>(T, fm, m, f0, nfield, tile, lane, j, pg, odd, W);
+    if (lane == 0) smin[wv] = lmin;
+    __syncthreads();  // also publishes the staged row
+    const int lblk = b4_row_start(smin, nwaves, m, L);
+
+    for (int i = m + tid; i < lblk; i += nthr)
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        if (f0 + f < nfield)
+          reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
+              make_float2(0.f, 0.f);
+
+    int buf = 0;
+    for (int ls0 = lblk; ls0 <= L; ls0 += NH * M4_SPAN) {
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int ls = ls0 + h * M4_SPAN;
+        float* rw = &red[buf][wv][h * 64 * FP];
+        if (lmin >= ls + M4_SPAN || ls > L) {
+#pragma unroll
+          for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
+          continue;
+        }
+        // the span's values: recurrence (alpha four at a time), split, tile
+#pragma unroll
+        for (int pb = 0; pb < M4_SPAN; pb += 4) {
+          double al[4];
+#pragma unroll
+          for (int p = 0; p < 4; ++p) al[p] = sab[ls - m + pb + p + 2];
+#pragma unroll
+          for (int p = pb; p < pb + 4; p += 2) {
+            const int l = ls + p;
+            if (p % kTile == 0 && ls <= lzmax && c.ls == l) {
+              c.y0 = c.s0;
+              c.y1 = c.s1;
+            }
+            const HL v0 = split_bf16((float)c.y0), v1 = split_bf16((float)c.y1);
+            b4_tile_store(tks, p, lane, v0.hi, v0.lo);
+            b4_tile_store(tks, p + 1, lane, v1.hi, v1.lo);
+            P::advance(al[p - pb], al[p - pb + 1], c.ct, c.y0, c.y1);
+          }
+        }
+        wave_sync_lds();
+        f4v acc[FP];
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+        b4_span_products<FP>(tks, blk, j, pg, W, acc);
+        wave_sync_lds();  // the tile is rewritten in the next span
+#pragma unroll
+        for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = b4_pair_group_sum(acc[fp]);
+      }
+      __syncthreads();
+      for (int o = tid; o < NH * 64 * FP; o += nthr) {
+        const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
+        const int jj = ln & 3, bb = ln >> 2;
+        const int l = ls0 + h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
+        const int f = 2 * fp + (jj >> 1);
+        if (l <= L && f < NF && f0 + f < nfield) {
+          const float s = sum_waves([&](int w) { return red[buf][w][o]; }, nwaves);
+          float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
+                       2 * (base + (l - m));
+          dst[jj & 1] = s * T.cz[base + (l - m)];
+        }
+      }
+      buf ^= 1;
+    }
+  }
+}
+
+// The spectral op in the split-BF16 form: k_leg_spectral_m4's pipeline (one
+// chain per lane) with the b4 analysis of each span, the solved span's
+// coefficients staged split in LDS, and the synthesis products of the span's
+// four parity groups from the split values the wave kept in registers.
+template <int NF>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : 1)))
+void k_leg_spectral_b4(LegTablesZ64 T, int nfield, float2* fm, Fields St,
+                       const float* __restrict__ lam, ImexCoef ic) {
+  using P = RecD;
+  constexpr int FP = (NF + 1) / 2;
+  constexpr int NO = 64 * FP;  // combined sums per span
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ float red[2][8][NO];
+  // c_l X_new of a span, split: [column][group (E0, O0, E1, O1)][hi x 4, lo x 4]
+  __shared__ __attribute__((aligned(16))) short qsb[2][4 * FP][4][8];
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  double* sab = reinterpret_cast<double*>(lds_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
+  const int j = lane & 3, blk = lane >> 2;
+  const int pg = blk >> 2;
+  const bool odd = m4_lgroup(blk & 3) >> 1;
+  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) + wv * (M4_SPAN * M4_TROW);
+  short* tks = reinterpret_cast<short*>(tile);
+  // this thread's coefficient of a span's combine (as k_leg_spectral_m4)
+  const int cfp = tid % FP, cln = (tid / FP) & 63;
+  const int cjj = cln & 3, cbb = cln >> 2;
+  const int clo = m4_off(m4_lgroup(cbb & 3), cbb >> 2);
+  const int cf = 2 * cfp + (cjj >> 1);
+  const bool cthr = tid < NO && cf < NF;
+  const bool cfield = cthr && cf < nfield;
+  float* const cX = cfield ? reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[cf])))
+                           : nullptr;
+  const int m = row_m(0, 1, M, false);  // rows dealt as in k_leg_spectral_m4
+  const long long base = lm_base(L, m);
+  stage_row_t<NF, P>(T, m, 0, nfield, Fields{}, false, sab, nullptr, kPadAnalys);
+
+  ChainT<P> c;
+  load_chain_t<P>(T, m, pair_of(0), c);
+  const int lmin = wave_min(c.ls), lzmax = last_seed(c.ls, L);
+  Split4 W[4][FP];
+  b4_weights<NF>(T, fm, m, 0, nfield, tile, lane, j, pg, odd, W);
+  if (lane == 0) smin[wv] = lmin;
+  __syncthreads();  // also publishes the staged row; every W is read
+  const int lblk = b4_row_start(smin, nwaves, m, L);
+
+  // below the row start the analysed values are exactly 0: the implicit
+  // step alone (and no synthesis contribution)
+  for (int o = tid; o < (lblk - m) * 2 * NF; o += nthr) {
+    const int f = o % NF, q = o / NF;
+    if (f >= nfield) continue;
+    const long long i = 2 * base + q;
+    float* X = reinterpret_cast<float*>(const_cast<char*>(St.at(ic.spc[f])));
+    X[i] = imex_step(X[i], ic.s1[f], 0.0f, ic.s2[f], lam[i]);
+  }
+
+  f4v accE[FP], accO[FP];
+#pragma unroll
+  for (int fp = 0; fp < FP; ++fp) accE[fp] = accO[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  // span ls's analysis into this wave's sums red[b][wv] and its split values
+  // zs (groups E0, O0, E1, O1); false when the wave's chain is still 0 there
+  auto analyse = [&](int ls, int b, Split4 (&zs)[4]) -> bool {
+    float* rw = &red[b][wv][0];
+    if (lmin >= ls + M4_SPAN) {
+#pragma unroll
+      for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
+      return false;
+    }
+    float z[8];
+#pragma unroll
+    for (int pb = 0; pb < M4_SPAN; pb += 4) {
+      double al[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) al[p] = sab[ls - m + pb + p + 2];
+#pragma unroll
+      for (int p = pb; p < pb + 4; p += 2) {
+        const int l = ls + p;
+        if (p % kTile == 0 && ls <= lzmax && c.ls == l) {
+          c.y0 = c.s0;
+          c.y1 = c.s1;
+        }
+        z[p & 7] = (float)c.y0;
+        z[(p & 7) + 1] = (float)c.y1;
+        P::advance(al[p - pb], al[p - pb + 1], c.ct, c.y0, c.y1);
+      }
+      if ((pb & 7) == 4) {  // an 8-l block complete: its two parity groups
+        const int g = (pb >> 3) << 1;
+        zs[g] = split4(z[0], z[2], z[4], z[6]);
+        zs[g + 1] = split4(z[1], z[3], z[5], z[7]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          b4_tile_store(tks, (pb & 8) + 2 * k, lane, zs[g].hi[k], zs[g].lo[k]);
+          b4_tile_store(tks, (pb & 8) + 2 * k + 1, lane, zs[g + 1].hi[k], zs[g + 1].lo[k]);
+        }
+      }
+    }
+    wave_sync_lds();
+    f4v acc[FP];
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) acc[fp] = f4v{0.f, 0.f, 0.f, 0.f};
+    b4_span_products<FP>(tks, blk, j, pg, W, acc);
+    wave_sync_lds();  // the tile is rewritten by the next span
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = b4_pair_group_sum(acc[fp]);
+    return true;
+  };
+  // synthesis products of one span's four groups, E0 and E1 into accE, O0
+  // and O1 into accO (the order of the synthesis kernel's 8-l blocks)
+  auto synthesise = [&](const Split4 (&zp)[4], int b) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int fp = 0; fp < FP; ++fp) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        Split4 bq = *reinterpret_cast<const Split4*>(&qsb[b][4 * fp + j][g][0]);
+        if (NF == 1 && j >= 2) bq = Split4{};  // the idle columns of one field
+        if (g & 1)
+          accO[fp] = mb3(zp[g], bq, accO[fp]);
+        else
+          accE[fp] = mb3(zp[g], bq, accE[fp]);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto iteration = [&](int ls, int b, Split4 (&zc)[4], const Split4 (&zp)[4], bool livep) -> bool {
+    const int l = ls + clo;
+    const bool has = cfield && l <= L;
+    long long ci = 0;
+    float xo = 0.f, lv = 0.f, cz = 0.f;
+    if (has) {  // issued before the analysis, used after the barrier
+      ci = 2 * (base + (l - m)) + (cjj & 1);
+      xo = cX[ci];
+      lv = lam[ci];
+      cz = T.cz[base + (l - m)];
+    }
+    const bool live = analyse(ls, b, zc);
+    __syncthreads();  // the waves' sums of span ls; qsb[b ^ 1] of the span before
+    if (cthr) {
+      float qv = 0.f;
+      if (has) {
+        float s = red[b][0][tid];
+        for (int w = 1; w < nwaves; ++w) s += red[b][w][tid];
+        const float xn = imex_step(xo, pick<NF>(ic.s1, cf), s * cz, pick<NF>(ic.s2, cf), lv);
+        cX[ci] = xn;
+        qv = xn * cz;
+      }
+      const HL h = split_bf16(qv);
+      short* d = &qsb[b][2 * (cfield ? pick<NF>(ic.spc, cf) : cf) + (cjj & 1)][b4_group(clo)]
+                     [b4_pos(clo)];
+      d[0] = h.hi;
+      d[4] = h.lo;
+    }
+    if (livep) synthesise(zp, b ^ 1);
+    return live;
+  };
+  Split4 zA[4], zB[4];
+  bool liveA = false, liveB = false, lastA = false;
+  for (int ls = lblk; ls <= L; ls += 2 * M4_SPAN) {
+    liveA = iteration(ls, 0, zA, zB, liveB);
+    lastA = true;
+    if (ls + M4_SPAN > L) break;
+    liveB = iteration(ls + M4_SPAN, 1, zB, zA, liveA);
+    lastA = false;
+  }
+  if (lblk <= L) {
+    __syncthreads();  // qsb of the last span
+    if (lastA) {
+      if (liveA) synthesise(zA, 0);
+    } else if (liveB) {
+      synthesise(zB, 1);
+    }
+  }
+
+  int M1 = M + 1;
+  asm volatile("" : "+s"(M1));
+  const int pb = pair_of(0) - lane + 4 * blk;
+#pragma unroll
+  for (int fp = 0; fp < FP; ++fp) {
+    const int s = 2 * fp + (j >> 1);
+    if (s >= NF || s >= nfield) continue;
+    float* out = reinterpret_cast<float*>(fm + (long long)s * nlat * M1 + m) + (j & 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int jp = pb + q;
+      if (jp >= T.npair) continue;
+      const float e = accE[fp][q], o = accO[fp][q];
+      out[2LL * jp * M1] = e + o;
+      out[2LL * (nlat - 1 - jp) * M1] = e - o;
+    }
+  }
+}
+
+template <int NF, bool IMEX = false>
+hipError_t synth_b4(const LegTablesZ64& T, int nfield, int nwaves, int rows, const Fields& q,
+                    float2* fm, hipStream_t s, const Fields& a = Fields{},
+                    const float* lam = nullptr, const ImexCoef& ic = ImexCoef{}) {
+  const int nt = b4_tiles(T.lmax);
+  const size_t lds = sizeof(double) * (8 * nt + kPadQ) +
+                     sizeof(short) * (size_t)nt * 2 * (4 * ((NF + 1) / 2)) * 8;
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  hipLaunchKernelGGL((k_leg_synth_b4<NF, IMEX>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm,
+                     rows, a, lam, ic);
+  return hipGetLastError();
+}
+
+template <int NF>
+hipError_t analys_b4(const LegTablesZ64& T, int nfield, int nwaves, int rows, const float2* fm,
+                     const Fields& q, hipStream_t s) {
+  dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  const size_t lds = sizeof(double) * analys_ab_len(T.lmax) +
+                     sizeof(float) * (size_t)nwaves * M4_SPAN * M4_TROW;
+  hipLaunchKernelGGL((k_leg_analys_b4<NF>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm, q, rows);
+  return hipGetLastError();
+}
+
+template <int NF>
+hipError_t spectral_b4(const LegTablesZ64& T, int nfield, int nwaves, float2* fm,
+                       const Fields& st, const float* lam, const ImexCoef& ic, hipStream_t s) {
+  const size_t lds = sizeof(double) * analys_ab_len(T.lmax) +
+                     sizeof(float) * (size_t)nwaves * M4_SPAN * M4_TROW;
+  hipLaunchKernelGGL((k_leg_spectral_b4<NF>), dim3(T.mmax + 1), dim3(64 * nwaves), lds, s, T,
+                     nfield, fm, st, lam, ic);
+  return hipGetLastError();
+}
+
 // Built-in choice of the contraction unit (1 VALU, 2 16x16x4 matrix tiles,
 // 3 4x4 matrix blocks).
 constexpr int kDefaultImpl = 3;
@@ -1865,6 +2489,19 @@ int mfma_col_tiles(int nfield);
 
 LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile t) {
   if (legendre == TSPHERE_LEG_BF16X3) {
+    // built-in: the 4x4x4 block kernels (impl 4) where one chain per lane
+    // covers the pairs; the 16x16x32 column-tile kernels (ctc) beyond that
+    // or when a column-tile count is asked for
+    if (t.impl == 0 && t.ctc == 0 && npair <= kB4MaxPairs) t.impl = 4;
+    if (t.impl == 4 && npair <= kB4MaxPairs) {
+      if (t.nf == 0) t.nf = nfield == 1 ? 1 : nfield == 2 ? 2 : 4;
+      if (t.rows == 0) t.rows = nfield <= 2 ? 1 : 2;
+      if (t.nwaves == 0) t.nwaves = (npair + 63) / 64;
+      if (t.k == 0) t.k = 1;
+      t.ctc = 0;
+      return t;
+    }
+    t.impl = 0;
     const int ct = mfma_col_tiles(nfield);
     if (t.ctc == 0)
       t.ctc = stage == 0 ? (ct >= 4 ? 4 : ct >= 2 ? 2 : 1) : (ct >= 2 ? 2 : 1);
@@ -2016,6 +2653,68 @@ hipError_t launch_leg_spectral_f64r(const LegTablesZ64& T, int nfield, float2* f
                                     const Fields& st, const float* lam, const ImexCoef& ic,
                                     hipStream_t s) {
   return spectral_launch<RecD>(T, nfield, fm, st, lam, ic, s);
+}
+
+}  // namespace tsphere
+
+namespace tsphere {
+
+namespace {
+
+bool b4_tile_ok(const LegTile& t, int npair) {
+  return t.impl == 4 && (t.nf == 1 || t.nf == 2 || t.nf == 4) && (t.rows == 1 || t.rows == 2) &&
+         t.k == 1 && t.nwaves >= 1 && t.nwaves <= 8 && 64 * t.nwaves >= npair;
+}
+
+}  // namespace
+
+hipError_t launch_leg_synth_b4(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
+                               LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_BF16X3, 0, T.npair, nfield, tile);
+  if (!b4_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return synth_b4<1>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 2: return synth_b4<2>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    default: return synth_b4<4>(T, nfield, t.nwaves, t.rows, q, fm, s);
+  }
+}
+
+hipError_t launch_leg_analys_b4(const LegTablesZ64& T, int nfield, const float2* fm,
+                                const Fields& q, LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_BF16X3, 1, T.npair, nfield, tile);
+  if (!b4_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return analys_b4<1>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 2: return analys_b4<2>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    default: return analys_b4<4>(T, nfield, t.nwaves, t.rows, fm, q, s);
+  }
+}
+
+hipError_t launch_leg_imex_synth_b4(const LegTablesZ64& T, int nfield, const Fields& q,
+                                    const Fields& a, const float* lam, const ImexCoef& ic,
+                                    float2* fm, LegTile tile, hipStream_t s) {
+  LegTile t = resolve_leg_tile(TSPHERE_LEG_BF16X3, 0, T.npair, nfield, tile);
+  if (t.impl != 4)
+    t = resolve_leg_tile(TSPHERE_LEG_BF16X3, 0, T.npair, nfield, LegTile{0, 0, 0, 0, 4, 0});
+  if (!b4_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  switch (t.nf) {
+    case 1: return synth_b4<1, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic);
+    case 2: return synth_b4<2, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic);
+    default: return synth_b4<4, true>(T, nfield, t.nwaves, t.rows, q, fm, s, a, lam, ic);
+  }
+}
+
+hipError_t launch_leg_spectral_b4(const LegTablesZ64& T, int nfield, float2* fm, const Fields& st,
+                                  const float* lam, const ImexCoef& ic, hipStream_t s) {
+  if (nfield < 1 || nfield > 4 || T.npair > kB4MaxPairs) return hipErrorInvalidValue;
+  // the analysis kernel's default waves (the same chunk-to-wave mapping, so
+  // the same combine order), and 64 combine threads per field pair
+  const int nwaves = std::max((T.npair + 63) / 64, nfield > 2 ? 2 : 1);
+  if (nfield == 1) return spectral_b4<1>(T, nfield, nwaves, fm, st, lam, ic, s);
+  if (nfield == 2) return spectral_b4<2>(T, nfield, nwaves, fm, st, lam, ic, s);
+  return spectral_b4<4>(T, nfield, nwaves, fm, st, lam, ic, s);
 }
 
 }  // namespace tsphere

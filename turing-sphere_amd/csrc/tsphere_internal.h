@@ -81,7 +81,9 @@ struct LegTablesZ64 {
 //  (1, 2, 4, 8 on the VALU, 1, 2, 4 on the matrix cores; nwaves * 64 * k must
 //  cover the pairs).
 //  rows (VALU kernels): m-rows per workgroup, 2 = the pair (m, M - m), 1 = one.
-//  Split-BF16 form: ctc column tiles (16 columns each) per workgroup.
+//  Split-BF16 form: impl 4 = 4x4x4 matrix blocks (nf, nwaves, rows as the
+//  FP32 block kernels, k = 1), else ctc column tiles (16 columns each) of the
+//  16x16x32 kernels per workgroup.
 struct LegTile {
   int nf, nwaves, k, ctc, impl, rows;
 };
@@ -211,6 +213,20 @@ hipError_t launch_leg_spectral(const LegTables& T, int nfield, float2* fm, const
 hipError_t launch_leg_spectral_f64r(const LegTablesZ64& T, int nfield, float2* fm,
                                     const Fields& st, const float* lam, const ImexCoef& ic,
                                     hipStream_t s);
+// The split-BF16 form on 4x4x4 matrix blocks (tile impl 4, the built-in
+// choice up to kB4MaxPairs latitude pairs: one chain per lane, 8 waves), on
+// the FP64-recurrence tables; the spectral op and the fused implicit step as
+// in the FP32 forms.
+constexpr int kB4MaxPairs = 512;
+hipError_t launch_leg_synth_b4(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
+                               LegTile tile, hipStream_t s);
+hipError_t launch_leg_analys_b4(const LegTablesZ64& T, int nfield, const float2* fm,
+                                const Fields& q, LegTile tile, hipStream_t s);
+hipError_t launch_leg_imex_synth_b4(const LegTablesZ64& T, int nfield, const Fields& q,
+                                    const Fields& a, const float* lam, const ImexCoef& ic,
+                                    float2* fm, LegTile tile, hipStream_t s);
+hipError_t launch_leg_spectral_b4(const LegTablesZ64& T, int nfield, float2* fm, const Fields& st,
+                                  const float* lam, const ImexCoef& ic, hipStream_t s);
 // The FP64-recurrence form: the VALU kernels with an f64 recurrence.
 hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields& q, float2* fm,
                                  LegTile tile, hipStream_t s);

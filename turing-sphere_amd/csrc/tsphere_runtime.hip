@@ -519,7 +519,7 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
   const std::vector<float> ones(cfg->nlat, 1.0f);
   if (e == hipSuccess) e = upload(&p->d_ones, ones.data(), (size_t)cfg->nlat);
   size_t bytes_tab = 0;
-  if (cfg->legendre != TSPHERE_LEG_BF16X3) {  // c_l of the rescaled recurrence
+  {  // c_l of the rescaled recurrence (every form: split-BF16 runs it in f64)
     std::vector<float> cz(ntab);
     for (size_t i = 0; i < ntab; i++) cz[i] = (float)t.cz64[i];
     if (e == hipSuccess) e = upload(&p->d_cz, cz.data(), ntab);
@@ -539,7 +539,9 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     if (e == hipSuccess) e = upload(&p->d_ct64, t.x.data(), (size_t)p->npair);
     bytes_tab += sizeof(double) * p->npair;
   }
-  if (cfg->legendre == TSPHERE_LEG_F64REC) {
+  if (cfg->legendre == TSPHERE_LEG_F64REC || cfg->legendre == TSPHERE_LEG_BF16X3) {
+    // the rescaled f64 recurrence: the FP64-recurrence form and the split-BF16
+    // 4x4x4 block kernels
     if (e == hipSuccess) e = upload(&p->d_alz64, t.alz64.data(), ntab);
     if (e == hipSuccess) e = upload(&p->d_seedz64, t.seedz64.data(), t.seedz64.size());
     bytes_tab += sizeof(double) * ntab + sizeof(SeedD) * t.seedz64.size();
@@ -633,9 +635,17 @@ static Fields listed(const void* const* v, int n) {
   return F;
 }
 
+// split-BF16: the 4x4x4 block kernels where the tile resolves to them
+static bool b4(const Plan* p, int stage, int nfield, const LegTile& t) {
+  return p->cfg.legendre == TSPHERE_LEG_BF16X3 &&
+         resolve_leg_tile(TSPHERE_LEG_BF16X3, stage, p->npair, nfield, t).impl == 4;
+}
+
 static hipError_t leg_synth(Plan* p, int nfield, const Fields& q, float2* fm, hipStream_t s) {
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
     return launch_leg_synth_f64r(p->tablesz64(), nfield, q, fm, p->tile[0], s);
+  if (b4(p, 0, nfield, p->tile[0]))
+    return launch_leg_synth_b4(p->tablesz64(), nfield, q, fm, p->tile[0], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_synth_mfma(p->tables64(), nfield, q, fm, p->tile[0], s);
   return launch_leg_synth(p->tables(), nfield, q, fm, p->tile[0], s);
@@ -645,6 +655,8 @@ static hipError_t leg_analys(Plan* p, int nfield, const float2* fm, const Fields
                              hipStream_t s, bool unit_w = false) {
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
     return launch_leg_analys_f64r(p->tablesz64(unit_w), nfield, fm, q, p->tile[1], s);
+  if (b4(p, 1, nfield, p->tile[1]))
+    return launch_leg_analys_b4(p->tablesz64(unit_w), nfield, fm, q, p->tile[1], s);
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
     return launch_leg_analys_mfma(p->tables64(unit_w), nfield, fm, q, p->tile[1], s);
   return launch_leg_analys(p->tables(unit_w), nfield, fm, q, p->tile[1], s);
@@ -808,9 +820,9 @@ int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float* const* analy
   if (nfield < 1 || nfield > 4 || nfield > p->cfg.max_fields)
     return fail(TSPHERE_ERR_INVALID, "nfield=%d: need 1..4 and <= max_fields=%d", nfield,
                 p->cfg.max_fields);
-  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
-    return fail(TSPHERE_ERR_INVALID, "the fused implicit step runs the FP32 and FP64-recurrence "
-                                     "forms");
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3 && p->npair > kB4MaxPairs)
+    return fail(TSPHERE_ERR_INVALID, "the fused implicit step of the split-BF16 form covers up "
+                                     "to %d latitude pairs", kB4MaxPairs);
   if (!analysed || !state || !lam || !s1 || !s2 || !fm)
     return fail(TSPHERE_ERR_INVALID, "NULL argument");
   ImexCoef ic{};
@@ -826,6 +838,9 @@ int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float* const* analy
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
     TS_HIP(launch_leg_imex_synth_f64r(p->tablesz64(), nfield, q, a, lam, ic, f2, p->tile[0], S(s)),
            "leg_imex_synth");
+  else if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    TS_HIP(launch_leg_imex_synth_b4(p->tablesz64(), nfield, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
   else
     TS_HIP(launch_leg_imex_synth(p->tables(), nfield, q, a, lam, ic, f2, p->tile[0], S(s)),
            "leg_imex_synth");
@@ -839,8 +854,9 @@ int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float* fm, float* cons
   if (nfield < 1 || nfield > 4 || nfield > p->cfg.max_fields)
     return fail(TSPHERE_ERR_INVALID, "nfield=%d: need 1..4 and <= max_fields=%d", nfield,
                 p->cfg.max_fields);
-  if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
-    return fail(TSPHERE_ERR_INVALID, "the spectral op runs the FP32 and FP64-recurrence forms");
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3 && p->npair > kB4MaxPairs)
+    return fail(TSPHERE_ERR_INVALID, "the split-BF16 spectral op covers up to %d latitude pairs",
+                kB4MaxPairs);
   if (!fm || !state || !lam || !s1 || !s2 || !species)
     return fail(TSPHERE_ERR_INVALID, "NULL argument");
   ImexCoef ic{};
@@ -860,6 +876,8 @@ int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float* fm, float* cons
   float2* f2 = reinterpret_cast<float2*>(fm);
   if (p->cfg.legendre == TSPHERE_LEG_F64REC)
     TS_HIP(launch_leg_spectral_f64r(p->tablesz64(), nfield, f2, st, lam, ic, S(s)), "leg_spectral");
+  else if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    TS_HIP(launch_leg_spectral_b4(p->tablesz64(), nfield, f2, st, lam, ic, S(s)), "leg_spectral");
   else
     TS_HIP(launch_leg_spectral(p->tables(), nfield, f2, st, lam, ic, S(s)), "leg_spectral");
   return ok();
@@ -1011,13 +1029,21 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
   if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
   const LegTile t = tile ? to_tile(*tile) : LegTile{};
-  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 3 ||
-      t.rows < 0 || t.rows > 2)
-    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0..3; rows 0, 1 or 2)");
+  const int impl_max = p->cfg.legendre == TSPHERE_LEG_BF16X3 ? 4 : 3;
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > impl_max ||
+      t.rows < 0 || t.rows > 2 || (p->cfg.legendre == TSPHERE_LEG_BF16X3 && t.impl != 0 && t.impl != 4))
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0..3, split-BF16 0 or 4; "
+                                     "rows 0, 1 or 2)");
   // validate against a one-field and a max-fields launch
   for (int nf : {1, p->cfg.max_fields}) {
     const LegTile r = resolve_leg_tile(p->cfg.legendre, stage, p->npair, nf, t);
-    if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
+    if (p->cfg.legendre == TSPHERE_LEG_BF16X3 && r.impl == 4) {
+      if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.k != 1 || r.nwaves < 1 || r.nwaves > 8 ||
+          64LL * r.nwaves < p->npair)
+        return fail(TSPHERE_ERR_INVALID,
+                    "split-BF16 block tile nf=%d nwaves=%d k=%d: need nf in {1,2,4}, k 1, "
+                    "64*nwaves >= %d latitude pairs (<= 8 waves)", r.nf, r.nwaves, r.k, p->npair);
+    } else if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
       if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
         return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
     } else {
@@ -1053,6 +1079,10 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
     const int ct = (2 * nfield + 15) / 16;
     for (int ctc : {1, 2, 4})
       if ((stage == 0 || ctc <= 2) && (ctc == 1 || ctc <= ct)) c.push_back(LegTile{0, 0, 0, ctc, 0, 0});
+    if (p->npair <= kB4MaxPairs)  // 4x4x4 blocks: fields per workgroup x rows
+      for (int nf : {1, 2, 4})
+        if (nf == 1 || nf <= nfield || (nf == 4 && nfield == 3))
+          for (int rows : {1, 2}) c.push_back(LegTile{nf, (p->npair + 63) / 64, 1, 0, 4, rows});
     return c;
   }
   for (int impl = 1; impl <= 3; impl++)
@@ -1095,6 +1125,9 @@ int tsphere_plan_tune(tsphere_plan p, int nfield, int reps, tsphere_stream strea
     float chosen_ms = 0.f;
     for (const LegTile& t : tile_candidates(p, stage, nfield)) {
       auto launch = [&]() -> hipError_t {
+        if (p->cfg.legendre == TSPHERE_LEG_BF16X3 && t.impl == 4)
+          return stage == 0 ? launch_leg_synth_b4(p->tablesz64(), nfield, F, p->d_fm, t, s)
+                            : launch_leg_analys_b4(p->tablesz64(), nfield, p->d_fm, F, t, s);
         if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
           return stage == 0 ? launch_leg_synth_mfma(p->tables64(), nfield, F, p->d_fm, t, s)
                             : launch_leg_analys_mfma(p->tables64(), nfield, p->d_fm, F, t, s);

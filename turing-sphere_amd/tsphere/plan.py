@@ -530,7 +530,10 @@ class ModelPlan:
         sht = self._sht
         if mode == "spectral" and sht is not None and (sht.cfg.nlat + 1) // 2 > 1024:
             mode = "fold"  # the spectral kernel covers up to 1024 latitude pairs
-        if (sht is None or getattr(sht, "legendre", None) not in ("fp32", "f64rec") or
+        if (sht is not None and getattr(sht, "legendre", None) == "bf16x3" and
+                (sht.cfg.nlat + 1) // 2 > 512):
+            return ops  # the split-BF16 block kernels cover up to 512 latitude pairs
+        if (sht is None or getattr(sht, "legendre", None) not in ("fp32", "f64rec", "bf16x3") or
                 len(ops) != 2 or not isinstance(ops[0], GridOp) or
                 not isinstance(ops[1], (KernelOp, FusedOp))):
             return ops
