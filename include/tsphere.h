@@ -209,7 +209,8 @@ int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float *fm
  * (A_f = analysed[f], rounded as written, as tsphere_leg_spectral_imex), and
  * the new state synthesised into fm (as tsphere_leg_synth_multi): the
  * implicit-solve dispatch and the next step's encodeSynthInto (plan.ts:
- * 436-478) as one launch.  FP32 and FP64-recurrence plans, nfield <= 4. */
+ * 436-478) as one launch.  nfield <= 4; split-BF16 plans up to 512
+ * latitude pairs (TSPHERE_ERR_INVALID above). */
 int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float *const *analysed,
                            float *const *state, const float *lam, const float *s1,
                            const float *s2, float *fm, tsphere_stream s);
@@ -223,7 +224,8 @@ int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float *const *analy
  * state index), ready for the next step's grid kernel: the analysis of step
  * n and the synthesis of step n+1 share one generation of the recurrence.
  * A(l,m) below every lane's significance point is 0, as in the analysis.
- * FP32 and FP64-recurrence plans, nfield <= 4, up to 1024 latitude pairs.
+ * FP32 and FP64-recurrence plans up to 1024 latitude pairs, split-BF16
+ * plans up to 512 (TSPHERE_ERR_INVALID above); nfield <= 4.
  * No reference counterpart of its own (it fuses encodeAnalysInto, the
  * implicit-solve dispatch and encodeSynthInto, plan.ts:436-478). */
 int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float *fm, float *const *state,
@@ -305,7 +307,12 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  *               64*nwaves*k >= pairs; rows = m-rows per workgroup for impl 1
  *               and 3 (2: the pair (m, mmax - m), balanced; 1: one row, twice
  *               the workgroups).
- *   MFMA form:  ctc 16-column tiles per workgroup (1, 2; 4 for synthesis). */
+ *   Split-BF16 form:  impl 4 = 16 4x4x4 blocks per wave
+ *               (v_mfma_f32_4x4x4_16b_bf16: two fields' re/im fill the 4
+ *               columns; the built-in choice up to 512 latitude pairs) with
+ *               nf, nwaves and rows as impl 3 and k = 1; impl 0 with ctc =
+ *               16-column tiles of v_mfma_f32_16x16x32_bf16 per workgroup (1,
+ *               2; 4 for synthesis; the built-in choice above 512 pairs). */
 typedef struct {
   int nf, nwaves, k, ctc, impl, rows;
 } tsphere_leg_tile;
