@@ -3,9 +3,10 @@ products through checkpointed IMEX trajectories, against the f64 oracle's
 adjoint (itself pinned by finite differences in tests/test_oracle.py).
 
 Stated tolerances: transposes 1e-5 relative to the f64 transpose (FP32
-transforms); trajectory gradients 2e-3 relative (the FP32 forward trajectory
-the gradient is taken along differs from the f64 one by ~1e-6 per step, and
-the adjoint accumulates FP32 rounding over every step)."""
+transforms); trajectory gradients 1e-5 relative at these sizes (measured
+1.0-1.8e-6: the FP32 forward trajectory the gradient is taken along differs
+from the f64 one by ~1e-7 per step, and the adjoint accumulates FP32 rounding
+over every step; at lmax 255 and 200 steps see tests/test_gpu_configs.py)."""
 import os
 
 import numpy as np
@@ -85,6 +86,11 @@ def _initial(cfg, nlm, state, seed):
 @pytest.mark.parametrize("key,lmax,steps,every", [("schnakenberg", 31, 30, 7),
                                                    ("brusselator", 31, 24, 10),
                                                    ("allencahn", 63, 40, 20)])
+# GPU vjp against the f64 adjoint along the f64 trajectory, relative; measured
+# 1.0-1.8e-6 for the initial states and 2e-8 - 1.2e-6 for the parameters
+TOL_VJP = 1e-5
+
+
 def test_vjp_vs_f64_adjoint(gpu, key, lmax, steps, every):
     cfg, sht, am, model, state, params = _adjoint(key, lmax, every)
     emit(f"vjp {key} lmax{lmax} steps {steps} checkpoints every {every}")
@@ -97,15 +103,15 @@ def test_vjp_vs_f64_adjoint(gpu, key, lmax, steps, every):
                             cot.get("V"), O.ShtReference(oc))
     eu = O.rel_l2(grads["U"], Ub)
     msg = f"  U0bar {eu:.2e}"
-    assert eu < 2e-3
+    assert eu < TOL_VJP
     if "V" in grads:
         ev = O.rel_l2(grads["V"], Vb)
         msg += f"  V0bar {ev:.2e}"
-        assert ev < 2e-3
+        assert ev < TOL_VJP
     for k, v in og.items():
         e = abs(pg[k] - v) / max(abs(v), 1e-30)
         msg += f"  {k} {pg[k]:.4e}/{v:.4e} ({e:.1e})"
-        assert e < 2e-3, (k, pg[k], v)
+        assert e < TOL_VJP, (k, pg[k], v)
     emit(msg)
     am.destroy()
     sht.destroy()
