@@ -83,14 +83,14 @@ def _initial(cfg, nlm, state, seed):
     return out
 
 
-@pytest.mark.parametrize("key,lmax,steps,every", [("schnakenberg", 31, 30, 7),
-                                                   ("brusselator", 31, 24, 10),
-                                                   ("allencahn", 63, 40, 20)])
 # GPU vjp against the f64 adjoint along the f64 trajectory, relative; measured
 # 1.0-1.8e-6 for the initial states and 2e-8 - 1.2e-6 for the parameters
 TOL_VJP = 1e-5
 
 
+@pytest.mark.parametrize("key,lmax,steps,every", [("schnakenberg", 31, 30, 7),
+                                                   ("brusselator", 31, 24, 10),
+                                                   ("allencahn", 63, 40, 20)])
 def test_vjp_vs_f64_adjoint(gpu, key, lmax, steps, every):
     cfg, sht, am, model, state, params = _adjoint(key, lmax, every)
     emit(f"vjp {key} lmax{lmax} steps {steps} checkpoints every {every}")
