@@ -170,6 +170,40 @@ def test_batched_fields(gpu):
     plan.destroy()
 
 
+@pytest.mark.parametrize("lmax,nlat,nphi", [
+    (255, 256, 512), (127, 129, 512),      # 256-point rows; band limit below the row
+    (511, 512, 1024), (300, 301, 1024),    # 512-point rows
+    (1023, 1024, 2048), (600, 601, 2048),  # 1024-point rows (the benchmarked grid)
+    (1365, 1366, 4096),                    # 2048-point rows (radix-4 kernels)
+])
+def test_fourier_stage_rows(gpu, lmax, nlat, nphi):
+    """Both Fourier stages alone on random rows against numpy's real FFT (the
+    oracle's fast stages), at the row lengths of the one-wave-per-row kernels
+    (nphi 512 .. 2048) and beyond, with band limits below the row length and
+    odd latitude counts: a few ulps."""
+    cfg = O.Cfg(lmax, lmax, nlat, nphi)
+    emit(f"fourier stages lmax{lmax} {nlat}x{nphi}")
+    plan = _plan(cfg)
+    ref = O.ShtReference(cfg)
+    rng = np.random.default_rng(lmax + nphi)
+    fm_dev = rng.normal(size=(nlat, lmax + 1, 2)).astype(np.float32)
+    fm_dev[:, 0, 1] = 0.0  # X[0] = Re F_0
+    s = plan.stream
+    plan.fm_buf.upload(fm_dev.reshape(-1), s)
+    plan.encode_stage(s, "fourSynth")
+    spat = plan.spat_buf.download(s)
+    fm_ref = np.ascontiguousarray(fm_dev.transpose(1, 0, 2)).reshape(-1)  # [m][ilat]
+    e_s = O.rel_l2(spat, ref.fourier_synth_fast(fm_ref.astype(np.float64)))
+    g = rng.normal(size=nlat * nphi).astype(np.float32)
+    plan.spat_buf.upload(g, s)
+    plan.encode_stage(s, "fourAnalys")
+    gm = plan.read_fm()
+    e_a = O.rel_l2(gm * (2 * np.pi / nphi), ref.fourier_analys_fast(g.astype(np.float64)))
+    emit(f"  fourSynth {e_s:.2e} fourAnalys {e_a:.2e}")
+    assert e_s < 2e-7 and e_a < 2e-7  # measured 1.0-1.2e-7
+    plan.destroy()
+
+
 @pytest.mark.parametrize("lmax", [255, 511, 1023])
 def test_round_trip_large(gpu, lmax):
     """Full-size grids (nlat = lmax+1, nphi = 2(lmax+1)): round trip identity,
