@@ -231,9 +231,11 @@ int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float *const *analy
 int tsphere_leg_spectral_imex(tsphere_plan p, int nfield, float *fm, float *const *state,
                               const float *lam, const float *s1, const float *s2,
                               const int *species, tsphere_stream s);
-/* The plan's device (cos, sin)(2 pi k / nphi) table, 2*nphi floats, which
- * the row FFT of a generated grid kernel must read to round as the plan's
- * Fourier stages do. */
+/* The plan's device twiddle table, which the row FFT of a generated grid
+ * kernel must read to round as the plan's Fourier stages do: the nphi
+ * entries (cos, sin)(2 pi k / nphi) (2*nphi floats), followed, for FFT
+ * plans, by the row FFT's radix-4 pass tables (copies of those entries,
+ * fft_row.h pass_table_size(log2(nphi/2)) float pairs). */
 int tsphere_plan_fourier_table(tsphere_plan p, const void **trig);
 
 /* ------------------------------------------- generated element-wise kernels

@@ -258,7 +258,8 @@ def test_fused_plan_matches_statementwise(gpu, key):
 @pytest.mark.parametrize("groups", [True, False])
 @pytest.mark.parametrize("key,lmax,nlat,nphi", [
     ("brusselator", 31, 32, 64), ("schnakenberg", 15, 17, 32), ("allencahn", 63, 64, 128),
-    ("brusselator", 127, 128, 256), ("schnakenberg", 255, 256, 512)])
+    ("brusselator", 127, 128, 256), ("schnakenberg", 255, 256, 512),
+    ("brusselator", 700, 701, 4096)])
 def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nlat, nphi,
                                                  groups):
     """Grid fusion (Legendre synthesis, one generated kernel running the row
@@ -268,7 +269,8 @@ def test_grid_fused_step_matches_separate_stages(gpu, monkeypatch, key, lmax, nl
     the row FFT is the library's own code (shared header), the statements
     the same f32 operations.  Both kernel forms (one thread group per field's
     row FFT, or one group transforming the rows in turn); several rows per
-    workgroup (small nphi) and an odd latitude count."""
+    workgroup (small nphi), an odd latitude count, and 2048-point rows with
+    mmax well below the row's Nyquist index."""
     import tsphere.hipgen
 
     monkeypatch.setattr(tsphere.hipgen, "GRID_GROUPS", groups)

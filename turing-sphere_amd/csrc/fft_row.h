@@ -18,8 +18,9 @@
 //   analysis   z[m] = x[2m] + i x[2m+1], Z = FFT_H^{-}(z),
 //              X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2pi i k/N} (-i)(Z[k] - conj Z[H-k])/2
 // with X[0] = Re F_0, X[m] = F_m (m <= mmax), 0 above.  Twiddles come from the
-// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N); those of the
-// radix-4 passes are staged per workgroup in LDS, in per-pass tables.
+// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N); each thread reads
+// the few its radix-4 butterflies need into registers when its kernel starts,
+// from per-pass copies of trig's entries that follow the table on the device.
 #pragma once
 
 namespace tsfft {
@@ -64,39 +65,123 @@ struct FftShape {
   static constexpr int RB = 256 / TPR;
   static constexpr bool ODD = LOG2H & 1;
   static constexpr int NS0 = ODD ? 2 : 4;  // Ns of the first pass after the first
-  static constexpr int NPASS = (LOG2H - (ODD ? 1 : 2)) / 2;
-  // Pass twiddle tables, k-major per pass: pass Ns holds W[k - 1][jm] =
-  // e^{+2 pi i k jm / (4 Ns)}, k = 1..3, jm < Ns, at offset Ns - NS0.
-  static constexpr int NTW = NPASS > 0 ? NS0 * ((1 << (2 * NPASS)) - 1) : 1;
+  static constexpr int NPASS = (LOG2H - (ODD ? 1 : 2)) / 2;  // radix-4 passes after the first
+};
+
+// The pass tables that follow the N entries of a plan's trig table on the
+// device: pass Ns (Ns = NS0 4^p) holds W[k - 1][jm] = trig[k jm N / (4 Ns)],
+// k = 1..3, jm < Ns, k-major at offset Ns - NS0, so a thread's entries for
+// consecutive jm are consecutive.  Entries in all: NS0 (4^NPASS - 1).
+__host__ __device__ constexpr int pass_table_size(int log2h) {
+  const int ns0 = (log2h & 1) ? 2 : 4;
+  const int npass = (log2h - ((log2h & 1) ? 1 : 2)) / 2;
+  return npass > 0 ? ns0 * ((1 << (2 * npass)) - 1) : 0;
+}
+
+// Twiddles of the radix-4 passes after the first: pass P (Ns = NS0 4^P)
+// multiplies input k = 1..3 of butterfly j = t + TPR b by
+// e^{+2 pi i k jm / (4 Ns)} = W[k - 1][jm], jm = j mod Ns, an entry of the
+// pass tables after the plan's trig table.  Two holders with the same
+// values, so the same arithmetic: at<P>(b, k, jm) returns the entry.
+//
+// PassTwiddles keeps one thread's entries in registers (3 per pass and
+// distinct jm), read once when a kernel starts (the reads overlap the first
+// pass's row loads; a wave's read of one table row covers at most 512
+// contiguous bytes), so the passes read no twiddles from memory; for
+// Ns <= TPR all of a thread's butterflies share jm and the entries of b = 0.
+// For kernels whose LDS, not registers, limits the waves a CU holds.
+template <int LOG2H>
+struct PassTwiddles {
+  using S = FftShape<LOG2H>;
+  static constexpr int NP = S::NPASS > 0 ? S::NPASS : 1;
+  float2 w[NP][S::BPT][3];
+  __device__ __forceinline__ void load(const float2* __restrict__ trig, int t) {
+    const float2* __restrict__ W = trig + S::N;
+#pragma unroll
+    for (int p = 0; p < S::NPASS; ++p) {
+      const int Ns = S::NS0 << (2 * p);
+#pragma unroll
+      for (int b = 0; b < S::BPT; ++b) {
+#pragma unroll
+        for (int k = 1; k <= 3; ++k) {
+          if (b > 0 && Ns <= S::TPR) {
+            w[p][b][k - 1] = w[p][0][k - 1];
+          } else {
+            const int jm = (t + S::TPR * b) & (Ns - 1);
+            w[p][b][k - 1] = W[(Ns - S::NS0) + (k - 1) * Ns + jm];
+          }
+        }
+      }
+    }
+  }
+  template <int P>
+  __device__ __forceinline__ float2 at(int b, int k, int) const {
+    return w[P][b][k - 1];
+  }
+};
+
+// PassTwiddlesLds keeps one copy of the pass tables per workgroup of NT
+// threads in LDS (`W`, pass_table_size entries): each thread copies a few
+// entries, load() into registers when the kernel starts and publish() to W
+// after the first pass (which needs none), whose row loads overlap the
+// table's; the first barrier of the remaining passes makes W visible.  A
+// workgroup then reads the tables from memory once, where per-thread
+// registers read each entry once per thread: for kernels that stream many
+// rows, whose memory pipeline the per-thread reads would load.
+template <int LOG2H, int NT>
+struct PassTwiddlesLds {
+  using S = FftShape<LOG2H>;
+  static constexpr int NTW = pass_table_size(LOG2H);
+  static constexpr int PER = NTW > 0 ? (NTW + NT - 1) / NT : 1;
+  float2* W;
+  float2 v[PER];
+  __device__ __forceinline__ void load(const float2* __restrict__ trig) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i < NTW) v[p] = trig[S::N + i];
+    }
+  }
+  __device__ __forceinline__ void publish() const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i < NTW) W[i] = v[p];
+    }
+  }
+  template <int P>
+  __device__ __forceinline__ float2 at(int, int k, int jm) const {
+    constexpr int Ns = S::NS0 << (2 * P);
+    return W[(Ns - S::NS0) + (k - 1) * Ns + jm];
+  }
 };
 
 // One radix-4 Stockham pass, natural order, from src to dst (both the
-// thread's row in LDS).  Butterfly j reads src[j + k Q], k = 0..3, and writes
+// thread's row in LDS), pass P of those after the first (Ns = NS0 4^P).
+// Butterfly j reads src[j + k Q], k = 0..3, and writes
 // dst[4(j - jm) + jm + k Ns], jm = j mod Ns, with twiddles
-// e^{SIGN 2 pi i jm k / (4 Ns)} = trig[jm k N / (4 Ns)], read from the pass's
-// staged table W (lanes of consecutive jm read consecutive entries, so the
-// reads do not conflict in LDS banks, as strided reads of one table would).
-template <int SIGN, int LOG2H>
-__device__ __forceinline__ void r4_pass(const float2* src, float2* dst, const float2* W, int t,
-                                        int Ns) {
+// e^{SIGN 2 pi i jm k / (4 Ns)} from tw.
+template <int SIGN, int LOG2H, int P, class TW>
+__device__ __forceinline__ void r4_pass(const float2* src, float2* dst, const TW& tw, int t) {
 #pragma clang fp contract(off)
   using S = FftShape<LOG2H>;
+  constexpr int NS = S::NS0 << (2 * P);
 #pragma unroll
   for (int b = 0; b < S::BPT; ++b) {
     const int j = t + S::TPR * b;
-    const int jm = j & (Ns - 1);
+    const int jm = j & (NS - 1);
     const float2 x0 = src[j];
-    const float2 x1 = cmul(src[j + S::Q], twc<SIGN>(W[jm]));
-    const float2 x2 = cmul(src[j + 2 * S::Q], twc<SIGN>(W[Ns + jm]));
-    const float2 x3 = cmul(src[j + 3 * S::Q], twc<SIGN>(W[2 * Ns + jm]));
+    const float2 x1 = cmul(src[j + S::Q], twc<SIGN>(tw.template at<P>(b, 1, jm)));
+    const float2 x2 = cmul(src[j + 2 * S::Q], twc<SIGN>(tw.template at<P>(b, 2, jm)));
+    const float2 x3 = cmul(src[j + 3 * S::Q], twc<SIGN>(tw.template at<P>(b, 3, jm)));
     const float2 a = cadd(x0, x2), bb = csub(x0, x2);
     const float2 c = cadd(x1, x3), e = csub(x1, x3);
     const float2 d = SIGN > 0 ? make_float2(-e.y, e.x) : make_float2(e.y, -e.x);
     const int o = 4 * (j - jm) + jm;
     dst[o] = cadd(a, c);
-    dst[o + Ns] = cadd(bb, d);
-    dst[o + 2 * Ns] = csub(a, c);
-    dst[o + 3 * Ns] = csub(bb, d);
+    dst[o + NS] = cadd(bb, d);
+    dst[o + 2 * NS] = csub(a, c);
+    dst[o + 3 * NS] = csub(bb, d);
   }
 }
 
@@ -132,42 +217,37 @@ __device__ __forceinline__ void first_pass(In in, float2* dst, int t) {
   }
 }
 
-// Remaining passes, ping-ponging between the two LDS rows (one barrier per
-// pass: a pass only writes the buffer the previous pass read).  Returns the
-// buffer that holds the result, after a final barrier.
-template <int SIGN, int LOG2H>
-__device__ __forceinline__ float2* rest_passes(float2* a, float2* b, const float2* W, int t) {
+// Passes P.. of the radix-4 passes after the first, ping-ponging between the
+// two LDS rows (one barrier per pass: a pass only writes the buffer the
+// previous pass read).  `active` (uniform per 256-thread group) masks a
+// group's arithmetic but never its barriers, so every thread of the workgroup
+// meets the same barriers.  Returns the buffer that holds the result, after a
+// final barrier.
+template <int SIGN, int LOG2H, int P, class TW>
+__device__ __forceinline__ float2* passes_from(bool active, float2* a, float2* b, const TW& tw,
+                                               int t) {
   using S = FftShape<LOG2H>;
-#pragma unroll
-  for (int Ns = S::NS0; Ns < S::H; Ns <<= 2) {
-    __syncthreads();
-    r4_pass<SIGN, LOG2H>(a, b, W + (Ns - S::NS0), t, Ns);
-    float2* tmp = a;
-    a = b;
-    b = tmp;
-  }
   __syncthreads();
-  return a;
+  if constexpr (P >= S::NPASS) {
+    return a;
+  } else {
+    if (active) r4_pass<SIGN, LOG2H, P>(a, b, tw, t);
+    return passes_from<SIGN, LOG2H, P + 1>(active, b, a, tw, t);
+  }
 }
 
-// The passes after the first for workgroups in which several groups of 256
-// threads each transform their own row: `active` (uniform per group) masks a
-// group's arithmetic but never its barriers, so every thread of the
-// workgroup meets the same barriers.  Same result buffer as rest_passes.
-template <int SIGN, int LOG2H>
+// Remaining passes of one row per 256-thread group (after first_pass).
+template <int SIGN, int LOG2H, class TW>
+__device__ __forceinline__ float2* rest_passes(float2* a, float2* b, const TW& tw, int t) {
+  return passes_from<SIGN, LOG2H, 0>(true, a, b, tw, t);
+}
+
+// The same for workgroups in which several groups of 256 threads each
+// transform their own row, some of them idle (`active` false).
+template <int SIGN, int LOG2H, class TW>
 __device__ __forceinline__ float2* rest_passes_masked(bool active, float2* a, float2* b,
-                                                      const float2* W, int t) {
-  using S = FftShape<LOG2H>;
-#pragma unroll
-  for (int Ns = S::NS0; Ns < S::H; Ns <<= 2) {
-    __syncthreads();
-    if (active) r4_pass<SIGN, LOG2H>(a, b, W + (Ns - S::NS0), t, Ns);
-    float2* tmp = a;
-    a = b;
-    b = tmp;
-  }
-  __syncthreads();
-  return a;
+                                                      const TW& tw, int t) {
+  return passes_from<SIGN, LOG2H, 0>(active, a, b, tw, t);
 }
 
 // Synthesis pre-processing, fourier.ts:309-350: Z[k] from the row's
@@ -179,17 +259,21 @@ struct SynthInput {
   const float2* __restrict__ trig;
   int mmax;
   float mscale;
+  // The load is unconditional, at an index clamped into the row, and the
+  // entries above mmax are zeroed by a select: a load under a branch waits
+  // for its data before the next one is issued.
   __device__ __forceinline__ float2 X(int k) const {
 #pragma clang fp contract(off)
-    if (k > mmax) return make_float2(0.f, 0.f);
-    const float2 v = F[k];
-    return k == 0 ? v : make_float2(mscale * v.x, mscale * v.y);
+    const float2 v = F[k <= mmax ? k : mmax];
+    const float2 s = k == 0 ? v : make_float2(mscale * v.x, mscale * v.y);
+    return k <= mmax ? s : make_float2(0.f, 0.f);
   }
   __device__ __forceinline__ float2 operator()(int k) const {
 #pragma clang fp contract(off)
     float2 xk = X(k);
     if (k == 0) xk.y = 0.f;
-    const float2 xh = k == 0 ? make_float2(0.f, 0.f) : X(FftShape<LOG2H>::H - k);
+    const float2 xhk = X(FftShape<LOG2H>::H - k);  // X(H) is 0 (mmax < H): no branch
+    const float2 xh = k == 0 ? make_float2(0.f, 0.f) : xhk;
     const float2 cj = make_float2(xh.x, -xh.y);
     const float2 bq = cmul(csub(xk, cj), trig[k]);
     const float2 s = cadd(xk, cj);
@@ -200,21 +284,21 @@ struct SynthInput {
 // One row's synthesis into the ping-pong rows a, b; returns the row holding
 // x[2m] = Re z[m], x[2m+1] = Im z[m] (thread t reads back its own entries
 // t + TPR b, b < 4 BPT, after the final barrier).
-template <int LOG2H>
+template <int LOG2H, class TW>
 __device__ __forceinline__ float2* synth_row(const float2* __restrict__ F, int mmax, float mscale,
-                                            const float2* __restrict__ trig, const float2* W,
+                                            const float2* __restrict__ trig, const TW& tw,
                                             float2* a, float2* b, int t) {
   const SynthInput<LOG2H> in{F, trig, mmax, mscale};
   first_pass<+1, LOG2H>(in, a, t);
-  return rest_passes<+1, LOG2H>(a, b, W, t);
+  return rest_passes<+1, LOG2H>(a, b, tw, t);
 }
 
 // One row's analysis FFT from in(j) = x[2j] + i x[2j+1]; returns z.
-template <int LOG2H, class In>
-__device__ __forceinline__ const float2* analys_row(In in, const float2* W, float2* a, float2* b,
+template <int LOG2H, class In, class TW>
+__device__ __forceinline__ const float2* analys_row(In in, const TW& tw, float2* a, float2* b,
                                                    int t) {
   first_pass<-1, LOG2H>(in, a, t);
-  return rest_passes<-1, LOG2H>(a, b, W, t);
+  return rest_passes<-1, LOG2H>(a, b, tw, t);
 }
 
 // Analysis post-processing, fourier.ts:352-386: X[k] from z, 0 <= k < H.
@@ -231,54 +315,6 @@ __device__ __forceinline__ float2 analys_coef(const float2* z, int k,
   const float2 xo = make_float2(d.y, -d.x);
   const float2 w = make_float2(trig[k].x, -trig[k].y);
   return cadd(xe, cmul(xo, w));
-}
-
-// The staging of stage_twiddles split in two, for an NT-thread workgroup:
-// load() reads this thread's entries into registers, store() writes them to
-// W (no barrier: the caller's next barrier publishes them).  A kernel issues
-// load() before its first pass, whose own row loads then overlap it.
-template <int LOG2H, int NT>
-struct TwiddleStage {
-  using S = FftShape<LOG2H>;
-  static constexpr int PER = (S::NTW + NT - 1) / NT;
-  float2 v[PER];
-  __device__ __forceinline__ void load(const float2* __restrict__ trig) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int i = threadIdx.x + p * NT;
-      if (i >= S::NTW) break;
-      int Ns = S::NS0;
-      while (i >= 4 * Ns - S::NS0) Ns <<= 2;  // pass of entry i: offsets Ns - NS0
-      const int loc = i - (Ns - S::NS0);
-      const int k = loc / Ns + 1, jm = loc - (k - 1) * Ns;
-      v[p] = trig[k * jm * (S::N / (4 * Ns))];
-    }
-  }
-  __device__ __forceinline__ void store(float2* W) const {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int i = threadIdx.x + p * NT;
-      if (i >= S::NTW) break;
-      W[i] = v[p];
-    }
-  }
-};
-
-// Stage the pass twiddle tables (FftShape::NTW entries) from the plan's
-// (cos, sin) table into the workgroup's LDS table W, then a barrier.  The
-// passes read their twiddles from W: a table read from global memory inside
-// each pass put one L2 round trip between every pair of barriers.
-template <int LOG2H>
-__device__ __forceinline__ void stage_twiddles(const float2* __restrict__ trig, float2* W) {
-  using S = FftShape<LOG2H>;
-  for (int i = threadIdx.x; i < S::NTW; i += blockDim.x) {
-    int Ns = S::NS0;
-    while (i >= 4 * Ns - S::NS0) Ns <<= 2;  // pass of entry i: offsets Ns - NS0
-    const int loc = i - (Ns - S::NS0);
-    const int k = loc / Ns + 1, jm = loc - (k - 1) * Ns;
-    W[i] = trig[k * jm * (S::N / (4 * Ns))];
-  }
-  __syncthreads();
 }
 
 }  // namespace tsfft

@@ -49,22 +49,26 @@ __global__ __launch_bounds__(256) void k_fft_synth(int mmax, int nlat, int tiles
                                                    Fields Fs, FourierScale sc) {
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
-  __shared__ float2 W[S::NTW];
-  // pass twiddles read into registers now, stored to W after the first pass
-  // (which needs none); the first barrier of the remaining passes publishes them
-  tsfft::TwiddleStage<LOG2H, 256> tws;
-  tws.load(trig);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
+  // the pass twiddle tables staged once per workgroup in LDS: these kernels
+  // stream many rows, and per-thread copies would triple the bytes each
+  // wave reads through its memory pipeline (64 fields of 512 x 1024:
+  // synthesis 47 us with the LDS copy, 61 us with per-thread registers)
+  using TW = tsfft::PassTwiddlesLds<LOG2H, 256>;
+  __shared__ float2 Wl[TW::NTW > 0 ? TW::NTW : 1];
+  TW tw;
+  tw.W = Wl;
+  tw.load(trig);
   const int ilat = tile * S::RB + r;
   const bool row = ilat < nlat;
   const float2* F = fm + ((long long)f * nlat + (row ? ilat : nlat - 1)) * (mmax + 1);
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
   tsfft::first_pass<+1, LOG2H>(tsfft::SynthInput<LOG2H>{F, trig, mmax, sc.mscale}, a, t);
-  tws.store(W);
-  const float2* res = tsfft::rest_passes<+1, LOG2H>(a, b, W, t);
+  tw.publish();
+  const float2* res = tsfft::rest_passes<+1, LOG2H>(a, b, tw, t);
   if (row) {
     float2* out = reinterpret_cast<float2*>(const_cast<char*>(Fs.at(f))) + (long long)ilat * S::H;
     const float w = sc.rowscale ? sc.rowscale[ilat] : 1.f;
@@ -84,12 +88,14 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
                                                     FourierScale sc) {
   using S = FftShape<LOG2H>;
   __shared__ float2 lds[2][S::RB * S::H];
-  __shared__ float2 W[S::NTW];
-  tsfft::TwiddleStage<LOG2H, 256> tws;  // as in k_fft_synth
-  tws.load(trig);
   const int L = blockIdx.x;
   const int f = L / tiles, tile = L - f * tiles;
   const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;
+  using TW = tsfft::PassTwiddlesLds<LOG2H, 256>;  // as in k_fft_synth
+  __shared__ float2 Wl[TW::NTW > 0 ? TW::NTW : 1];
+  TW tw;
+  tw.W = Wl;
+  tw.load(trig);
   const int ilat = tile * S::RB + r;
   const bool row = ilat < nlat;
   const float2* in =
@@ -97,8 +103,8 @@ __global__ __launch_bounds__(256) void k_fft_analys(int mmax, int nlat, int tile
   float2* a = lds[0] + r * S::H;
   float2* b = lds[1] + r * S::H;
   tsfft::first_pass<-1, LOG2H>([&](int j) { return in[j]; }, a, t);
-  tws.store(W);
-  const float2* z = tsfft::rest_passes<-1, LOG2H>(a, b, W, t);
+  tw.publish();
+  const float2* z = tsfft::rest_passes<-1, LOG2H>(a, b, tw, t);
   if (row) {
     float2* G = fm + ((long long)f * nlat + ilat) * (mmax + 1);
     for (int k = t; k <= mmax; k += S::TPR) {

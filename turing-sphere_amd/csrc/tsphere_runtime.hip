@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "fft_row.h"
 #include "tsphere_internal.h"
 
 using namespace tsphere;
@@ -552,15 +553,31 @@ int tsphere_plan_create(const tsphere_sht_config* cfg, tsphere_plan* out) {
     if (e == hipSuccess) e = upload(&p->d_seed64, t.seed64.data(), t.seed64.size());
     bytes_tab += sizeof(double2) * ntab + sizeof(SeedD) * t.seed64.size();
   }
-  if (e == hipSuccess)
-    e = upload(&p->d_trig, reinterpret_cast<const float2*>(t.trig.data()), (size_t)cfg->nphi);
+  // FFT plans: the row FFT's pass tables follow the nphi (cos, sin) entries
+  // (entries of the f32 table, so the row FFT rounds as with trig itself)
+  std::vector<float2> dtrig(cfg->nphi);
+  for (int k = 0; k < cfg->nphi; ++k) dtrig[k] = make_float2(t.trig[2 * k], t.trig[2 * k + 1]);
+  if (p->fourier == TSPHERE_FOURIER_FFT) {
+    const int N = cfg->nphi;
+    int log2h = 0;
+    while ((2 << log2h) < N) ++log2h;  // N = 2 H, H = 2^log2h
+    const int ns0 = (log2h & 1) ? 2 : 4;
+    for (int Ns = ns0; Ns < N / 2; Ns <<= 2)
+      for (int k = 1; k <= 3; ++k)
+        for (int jm = 0; jm < Ns; ++jm) {
+          const float2 w = dtrig[(size_t)k * jm * (N / (4 * Ns))];
+          dtrig.push_back(w);
+        }
+    if (dtrig.size() != (size_t)N + tsfft::pass_table_size(log2h)) e = hipErrorInvalidValue;
+  }
+  if (e == hipSuccess) e = upload(&p->d_trig, dtrig.data(), dtrig.size());
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_fm, sizeof(float2) * nfm);
   if (e != hipSuccess) {
     plan_free(p);
     delete p;
     return hip_fail(e, "tsphere_plan_create");
   }
-  p->bytes = bytes_tab + 2 * sizeof(float) * cfg->nlat + sizeof(float2) * cfg->nphi +
+  p->bytes = bytes_tab + 2 * sizeof(float) * cfg->nlat + sizeof(float2) * dtrig.size() +
              sizeof(float2) * nfm;
   *out = p;
   return ok();

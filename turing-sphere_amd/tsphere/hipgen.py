@@ -512,11 +512,12 @@ def fft_row_source() -> str:
 
 def grid_lds_bytes(log2h: int, n_rows: int) -> int:
     """LDS of a grid kernel: two ping-pong rows plus `n_rows` analysis rows,
-    each RB rows of H complex values per workgroup."""
+    each RB rows of H complex values per workgroup (the row FFT's twiddles
+    are held in registers)."""
     h = 1 << log2h
     tpr = min(h // 4, 256)
     rb = 256 // tpr
-    return (2 + n_rows) * rb * h * 8 + _ntw(log2h) * 8
+    return (2 + n_rows) * rb * h * 8
 
 
 def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, synth, stmts,
@@ -586,9 +587,9 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
             ap(f"  const float p{i} = prm[run * {npar} + {i}];")
     ap("  __shared__ float2 lds[2][S::RB * S::H];")
     ap(f"  __shared__ float2 arow[{max(1, len(analys))}][S::RB * S::H];")
-    ap("  __shared__ float2 W[S::NTW];")
-    ap(f"  tsfft::stage_twiddles<{log2h}>(trig, W);")
     ap("  const int r = threadIdx.x / S::TPR, t = threadIdx.x % S::TPR;")
+    ap(f"  tsfft::PassTwiddles<{log2h}> tw;")
+    ap("  tw.load(trig, t);")
     ap("  const int ilat = blockIdx.x * S::RB + r;")
     ap("  const bool row = ilat < NLAT;")
     ap("  const int il = row ? ilat : NLAT - 1;")
@@ -599,7 +600,7 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
         ap(f"  float2 sv{si}[NP];")
         ap("  {")
         ap(f"    const float2* res = tsfft::synth_row<{log2h}>(fm + ((long long)({si} * R + run) "
-           f"* NLAT + il) * (MMAX + 1), MMAX, 1.0f, trig, W, a, b, t);")
+           f"* NLAT + il) * (MMAX + 1), MMAX, 1.0f, trig, tw, a, b, t);")
         ap("#pragma unroll")
         ap(f"    for (int i = 0; i < NP; ++i) sv{si}[i] = res[t + S::TPR * i];")
         ap("    __syncthreads();  // the next row's first pass overwrites the ping-pong rows")
@@ -645,7 +646,7 @@ def build_grid_kernel(name: str, label: str, log2h: int, nlat: int, mmax: int, s
         ap("  {")
         ap(f"    const float2* src = arow[{q}] + r * S::H;")
         ap(f"    const float2* z = tsfft::analys_row<{log2h}>([&](int j) {{ return src[j]; }}, "
-           f"W, a, b, t);")
+           f"tw, a, b, t);")
         ap("    if (row) {")
         ap(f"      float2* G = fm + ((long long)({q} * R + run) * NLAT + ilat) * (MMAX + 1);")
         ap("      for (int k = t; k <= MMAX; k += S::TPR)")
@@ -679,15 +680,7 @@ def grid_groups(log2h: int, ns: int, na: int) -> int:
         return 0
     h = 1 << log2h
     rb = 256 // min(h // 4, 256)
-    return g if g * 2 * rb * h * 8 + _ntw(log2h) * 8 <= 65536 else 0
-
-
-def _ntw(log2h: int) -> int:
-    """Entries of a row FFT's staged pass twiddle tables (FftShape::NTW)."""
-    odd = log2h & 1
-    ns0 = 2 if odd else 4
-    npass = (log2h - (1 if odd else 2)) // 2
-    return ns0 * ((1 << (2 * npass)) - 1) if npass > 0 else 1
+    return g if g * 2 * rb * h * 8 <= 65536 else 0
 
 
 def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts, npar, stores,
@@ -712,17 +705,16 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
            [f"const float* gin{k}" for k in range(len(gins))] +
            ([f"float p{i}" for i in range(npar)] if not ensemble else
             ["const float* __restrict__ prm"]))
-    ap(f'extern "C" __global__ __launch_bounds__({256 * G}) void {name}({", ".join(sig)}) {{')
+    # rows of up to 1024 complex points take G x 16 KiB of LDS, which leaves
+    # room for 8 waves per SIMD: registers are capped to match (64 VGPRs;
+    # without the cap the pass twiddles held through the element-wise part
+    # take the C2 kernel to 70 and a CU holds 3 workgroups instead of 4)
+    occ = " __attribute__((amdgpu_waves_per_eu(8)))" if log2h <= 10 else ""
+    ap(f'extern "C" __global__ __launch_bounds__({256 * G}){occ} void {name}({", ".join(sig)}) {{')
     ap(f"  using S = tsfft::FftShape<{log2h}>;")
     ap(f"  constexpr int NLAT = {nlat}, MMAX = {mmax}, R = {R}, G = {G};")
     ap("  constexpr int NPT = S::RB * S::H / (256 * G);  // complex points per thread")
     ap("  __shared__ float2 lds[G][2][S::RB * S::H];")
-    ap("  __shared__ float2 W[S::NTW];")
-    # the pass twiddles are read into registers first and stored after the
-    # first pass (which needs none): the two loads overlap, and the first
-    # barrier of the remaining passes publishes both
-    ap(f"  tsfft::TwiddleStage<{log2h}, 256 * G> tws;")
-    ap("  tws.load(trig);")
     ap("  const int run = blockIdx.y;")
     ap("  const long long grun = (long long)run * NLAT * S::H;  // a run's grid, float2 units")
     if ensemble:
@@ -730,6 +722,8 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
             ap(f"  const float p{i} = prm[run * {npar} + {i}];")
     ap("  const int g = threadIdx.x >> 8, lt = threadIdx.x & 255;")
     ap("  const int r = lt / S::TPR, t = lt % S::TPR;")
+    ap(f"  tsfft::PassTwiddles<{log2h}> tw;")
+    ap("  tw.load(trig, t);")
     ap("  const int ilat = blockIdx.x * S::RB + r;")
     ap("  const int il = ilat < NLAT ? ilat : NLAT - 1;")
     ap("  float2* a = lds[g][0] + r * S::H;")
@@ -741,8 +735,7 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
        f"(MMAX + 1), trig, MMAX, 1.0f}};")
     ap(f"    tsfft::first_pass<+1, {log2h}>(in, a, t);")
     ap("  }")
-    ap("  tws.store(W);")
-    ap(f"  tsfft::rest_passes_masked<+1, {log2h}>(sact, a, b, W, t);")
+    ap(f"  tsfft::rest_passes_masked<+1, {log2h}>(sact, a, b, tw, t);")
     # element-wise over the block's points
 
     def src_of(cn):  # LDS row of a synthesised value
@@ -799,7 +792,7 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
     ap("  if (aact) {")
     ap(f"    tsfft::first_pass<-1, {log2h}>([&](int j) {{ return ain[j]; }}, aout, t);")
     ap("  }")
-    ap(f"  const float2* z = tsfft::rest_passes_masked<-1, {log2h}>(aact, aout, ain, W, t);")
+    ap(f"  const float2* z = tsfft::rest_passes_masked<-1, {log2h}>(aact, aout, ain, tw, t);")
     ap("  if (aact && ilat < NLAT) {")
     ap("    float2* G2 = fm + ((long long)(g * R + run) * NLAT + ilat) * (MMAX + 1);")
     ap("    for (int k = t; k <= MMAX; k += S::TPR)")
