@@ -15,6 +15,10 @@ in one kernel).  The timed steps replay HIP graphs whose step count divides
 are resident in HBM; the timed region is `--steps` steps bracketed by device
 synchronisation.  Before timing, one synthesis -> analysis round trip of the
 initial state must come back within 1e-3 relative L2 (else exit status 3).
+The same step is also timed with the FP64-recurrence and split-BF16 Legendre
+forms and as the plain FP32 step (Legendre sums on the vector ALU, separate
+launches, no matrix cores); `split_bf16_speedup` compares the split-BF16 step
+with both FP32 steps.
 
 Printed: ONE JSON line with the metric, a `roofline` object for the dominant
 kernel (algorithmic flops per launch / its mean duration measured with HIP
@@ -142,9 +146,12 @@ class SanityError(RuntimeError):
 
 
 def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: bool,
-                latency_steps: int = 0):
+                latency_steps: int = 0, plain: bool = False):
     """Time the IMEX step with one Legendre implementation; per-kernel means;
-    error of one synthesis of the initial state vs the f64 oracle."""
+    error of one synthesis of the initial state vs the f64 oracle.  With
+    `plain` (FP32 only), the plain FP32 step: Legendre sums on the vector ALU
+    (tile impl 1, its built-in shape) as separate synthesis and analysis
+    launches, the implicit solve as its generated kernel, no matrix cores."""
     import tsphere
     from tsphere import ShtConfig, ShtPlan, _lib
     from tsphere.model import GpuModel
@@ -153,10 +160,16 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     npts, nlm = cfg.npts, cfg.nlm
     log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={legendre}")
     sht = ShtPlan.create(cfg, legendre=legendre, max_fields=2)
-    tuned = sht.tune(2, reps=5)
+    if plain:
+        for st in ("synth", "analys"):
+            sht.set_tile(st, impl=1)
+        tuned = {st: sht.get_tile(st, 2) for st in ("synth", "analys")}
+    else:
+        tuned = sht.tune(2, reps=5)
     log(f"[bench] {legendre} tuned tiles: {tuned}")
     src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
-    gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"])
+    gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"],
+                         spectral=not plain, fold=not plain)
     gm.set_params(PARAMS)
     ru, rv = initial_grid(npts, 0)
     u0 = (PARAMS["A"] + ru).astype(np.float32)
@@ -450,6 +463,8 @@ def main() -> int:
             variants[leg] = run_variant(leg, args.steps, args.warmup, args.graph_steps,
                                         not args.quick,
                                         args.latency_steps if leg == args.legendre else 0)
+        variants["fp32_valu"] = run_variant("fp32", args.steps, args.warmup, args.graph_steps,
+                                            False, 0, plain=True)
     except SanityError as e:
         log(f"[bench] sanity check failed: {e}")
         return 3
@@ -528,6 +543,14 @@ def main() -> int:
                      | {"kernel_ms": {kk: round(vv, 5) for kk, vv in v["kernel_ms"].items()}}
                      for k, v in variants.items()},
         "batched": extras,
+        # the precision form's step against the plain FP32 step (vector-ALU
+        # Legendre sums) and against the FP32 step with its sums on 4x4
+        # matrix blocks (the default)
+        "split_bf16_speedup": {
+            "vs_plain_fp32_valu": round(variants["bf16x3"]["steps_per_s"] /
+                                        variants["fp32_valu"]["steps_per_s"], 3),
+            "vs_fp32_blocks": round(variants["bf16x3"]["steps_per_s"] /
+                                    variants["fp32"]["steps_per_s"], 3)},
         "roofline": {"bound": "mfma", "kernel": dom,
                      "pipe": ("bf16 MFMA (split x3)" if args.legendre == "bf16x3"
                               else "fp32: recurrence on the vector ALU, sums on the fp32 matrix "
