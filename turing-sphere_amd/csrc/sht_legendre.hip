@@ -1184,6 +1184,10 @@ __global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int 
 // 16-byte operand load under this placement and the earlier one alike: the
 // passes of a full-wave 16-byte load, not avoidable conflicts.)
 constexpr int M4_SPAN = 16;
+// spans of the block analysis per workgroup barrier: two keep the
+// workgroup's LDS (tiles, alpha, the waves' sums) within a third of the CU's
+// 160 KB at lmax 1023 with FP64 alpha
+constexpr int M4_NH = 2;
 constexpr int M4_TROW = 64 + 4;  // floats per l row of the tile
 constexpr int M4_WROW = 64 + 2;  // floats per column row of the row's staged weights
 static_assert(8 * 2 * M4_WROW <= M4_SPAN * M4_TROW, "two field pairs' weights fit a tile");
@@ -1204,7 +1208,9 @@ static_assert(m4_row(0) == 0 && m4_row(2) == 1 && m4_row(1) == 4 && m4_row(7) ==
 // forms would spill under it.
 template <int NF, int K, class P>
 constexpr int m4_analys_waves_per_eu() {
-  return std::is_same<typename P::ab_t, float>::value && (NF + 1) / 2 == 1 && K == 1 ? 6 : 1;
+  return std::is_same<typename P::ab_t, float>::value && K == 1
+             ? ((NF + 1) / 2 == 1 ? 6 : (NF + 1) / 2 == 2 ? 5 : 1)
+             : 1;
 }
 
 template <int NF, int K, class P>
@@ -1212,11 +1218,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_analys_w
 void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict__ fm, Fields Fq,
                      int rows) {
   constexpr int FP = (NF + 1) / 2;
-  // spans per workgroup barrier: two keep the workgroup's LDS (tiles, alpha,
-  // red) within a third of the CU's 160 KB at lmax 1023 with FP64 alpha
-  constexpr int M4_NH = 2;
+  constexpr int NRED = M4_NH * 64 * FP;  // a wave's sums between two barriers
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  __shared__ float red[2][8][M4_NH * 64 * FP];
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
   auto* sab = reinterpret_cast<typename P::ab_t*>(lds_raw);
@@ -1225,8 +1228,11 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
   const int j = lane & 3, blk = lane >> 2;
   const int lg = m4_lgroup(blk & 3), pg = blk >> 2;
   const bool odd = lg >> 1;  // this block's parity of l - m
-  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) +
-               wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
+  float* const tiles = reinterpret_cast<float*>(sab + analys_ab_len(L));
+  float* tile = tiles + wv * (K * M4_SPAN * M4_TROW);  // one tile per chain
+  // the waves' sums, [buffer][wave][NRED], after the tiles: sized by the
+  // launch's wave count, so a 4-wave workgroup takes half of 8 waves' space
+  float* const red = tiles + nwaves * (K * M4_SPAN * M4_TROW);
   const int f0 = blockIdx.y * NF;
 
   for (int r = 0; r < 2; ++r) {
@@ -1309,7 +1315,7 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
 #pragma unroll
       for (int h = 0; h < M4_NH; ++h) {
         const int ls = ls0 + h * M4_SPAN;
-        float* rw = &red[buf][wv][h * 64 * FP];
+        float* rw = red + (buf * nwaves + wv) * NRED + h * 64 * FP;
         if (lmin >= ls + M4_SPAN || ls > L) {
           // every chain of this wave is still exactly 0 in this span (or the
           // span is past lmax)
@@ -1385,13 +1391,14 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
       }
       __syncthreads();
       // 4) the waves' sums, in wave order, one output per thread
-      for (int o = tid; o < M4_NH * 64 * FP; o += nthr) {
+      for (int o = tid; o < NRED; o += nthr) {
         const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
         const int jj = ln & 3, bb = ln >> 2;
         const int l = ls0 + h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
         const int f = 2 * fp + (jj >> 1);
         if (l <= L && f < NF && f0 + f < nfield) {
-          const float s = sum_waves([&](int w) { return red[buf][w][o]; }, nwaves);
+          const float s = sum_waves([&](int w) { return red[(buf * nwaves + w) * NRED + o]; },
+                                    nwaves);
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
           dst[jj & 1] = s * T.cz[base + (l - m)];
@@ -1824,8 +1831,11 @@ template <int NF, int K, class P>
 hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int rows,
                      const float2* fm, const Fields& q, hipStream_t s) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
+  // alpha, one tile per wave and chain, the waves' sums (two buffers of
+  // 2 spans x 64 lanes x field pairs per wave)
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
-                     sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW;
+                     sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW +
+                     sizeof(float) * (size_t)2 * nwaves * M4_NH * 64 * ((NF + 1) / 2);
   hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
                      q, rows);
   return hipGetLastError();
