@@ -1097,10 +1097,17 @@ __device__ __forceinline__ void synth_phases_m4(int l, int L, int m, int j, cons
 // Synthesis: per chain and field pair, D_E / D_O (E: even l - m, O: odd)
 // accumulate P[pair][l] * Q[l][col] over l with the row's 4 pairs of a block
 // as rows; the lane of column j ends with the sums of the 4 pairs of its block.
+// The 4-field, one-chain FP32 form compiles to 100 VGPRs (4 waves per SIMD);
+// held to 96 it takes 5, which its LDS (alpha and 4 fields' Q per row) allows.
+template <int NF, int K, class P>
+constexpr int m4_synth_waves_per_eu() {
+  return std::is_same<typename P::ab_t, float>::value && NF == 4 && K == 1 ? 5 : 1;
+}
+
 template <int NF, int K, class P, bool IMEX = false>
-__global__ __launch_bounds__(512) void k_leg_synth_m4(typename P::Tables T, int nfield, Fields Fq,
-                                                      float2* __restrict__ fm, int rows,
-                                                      Fields Fa, const float* lam, ImexCoef ic) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_synth_waves_per_eu<NF, K, P>())))
+void k_leg_synth_m4(typename P::Tables T, int nfield, Fields Fq, float2* __restrict__ fm, int rows,
+                    Fields Fa, const float* lam, ImexCoef ic) {
   constexpr int FP = (NF + 1) / 2;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
