@@ -2173,9 +2173,9 @@ __global__ __launch_bounds__(512) void k_leg_analys_b4(LegTablesZ64 T, int nfiel
                                                        int rows) {
   using P = RecD;
   constexpr int FP = (NF + 1) / 2;
-  constexpr int NH = 2;  // spans per workgroup barrier
+  constexpr int NH = M4_NH;  // spans per workgroup barrier
+  constexpr int NRED = NH * 64 * FP;  // a wave's sums between two barriers
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  __shared__ float red[2][8][NH * 64 * FP];
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax;
   double* sab = reinterpret_cast<double*>(lds_raw);
@@ -2184,7 +2184,10 @@ __global__ __launch_bounds__(512) void k_leg_analys_b4(LegTablesZ64 T, int nfiel
   const int j = lane & 3, blk = lane >> 2;
   const int pg = blk >> 2;
   const bool odd = m4_lgroup(blk & 3) >> 1;
-  float* tile = reinterpret_cast<float*>(sab + analys_ab_len(L)) + wv * (M4_SPAN * M4_TROW);
+  float* const tiles = reinterpret_cast<float*>(sab + analys_ab_len(L));
+  float* tile = tiles + wv * (M4_SPAN * M4_TROW);
+  // the waves' sums, [buffer][wave][NRED], after the tiles (as k_leg_analys_m4)
+  float* const red = tiles + nwaves * (M4_SPAN * M4_TROW);
   short* tks = reinterpret_cast<short*>(tile);
   const int f0 = blockIdx.y * NF;
 
@@ -2215,7 +2218,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_b4(LegTablesZ64 T, int nfiel
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
         const int ls = ls0 + h * M4_SPAN;
-        float* rw = &red[buf][wv][h * 64 * FP];
+        float* rw = red + (buf * nwaves + wv) * NRED + h * 64 * FP;
         if (lmin >= ls + M4_SPAN || ls > L) {
 #pragma unroll
           for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = 0.f;
@@ -2250,13 +2253,14 @@ __global__ __launch_bounds__(512) void k_leg_analys_b4(LegTablesZ64 T, int nfiel
         for (int fp = 0; fp < FP; ++fp) rw[lane * FP + fp] = b4_pair_group_sum(acc[fp]);
       }
       __syncthreads();
-      for (int o = tid; o < NH * 64 * FP; o += nthr) {
+      for (int o = tid; o < NRED; o += nthr) {
         const int fp = o % FP, ln = (o / FP) & 63, h = o / (64 * FP);
         const int jj = ln & 3, bb = ln >> 2;
         const int l = ls0 + h * M4_SPAN + m4_off(m4_lgroup(bb & 3), bb >> 2);
         const int f = 2 * fp + (jj >> 1);
         if (l <= L && f < NF && f0 + f < nfield) {
-          const float s = sum_waves([&](int w) { return red[buf][w][o]; }, nwaves);
+          const float s = sum_waves([&](int w) { return red[(buf * nwaves + w) * NRED + o]; },
+                                    nwaves);
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
           dst[jj & 1] = s * T.cz[base + (l - m)];
@@ -2479,7 +2483,8 @@ hipError_t analys_b4(const LegTablesZ64& T, int nfield, int nwaves, int rows, co
                      const Fields& q, hipStream_t s) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   const size_t lds = sizeof(double) * analys_ab_len(T.lmax) +
-                     sizeof(float) * (size_t)nwaves * M4_SPAN * M4_TROW;
+                     sizeof(float) * (size_t)nwaves * M4_SPAN * M4_TROW +
+                     sizeof(float) * (size_t)2 * nwaves * M4_NH * 64 * ((NF + 1) / 2);
   hipLaunchKernelGGL((k_leg_analys_b4<NF>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm, q, rows);
   return hipGetLastError();
 }
