@@ -303,8 +303,8 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  *               (v_mfma_f32_16x16x4_f32), 3 = on the FP32 matrix cores as 16
  *               4x4 blocks per wave (v_mfma_f32_4x4x1_16b_f32: two fields'
  *               re/im fill the 4 columns); the recurrence stays on the VALU;
- *               nf fields per workgroup (1, 2, 4), nwaves waves (1..4;
- *               1..8 for impl 3),
+ *               nf fields per workgroup (1, 2, 4; 8 for impl 3 with k 1 or
+ *               2: batched transforms), nwaves waves (1..4; 1..8 for impl 3),
  *               k latitude pairs per lane (1, 2, 4, 8; matrix cores 1, 2, 4),
  *               64*nwaves*k >= pairs; rows = m-rows per workgroup for impl 1
  *               and 3 (2: the pair (m, mmax - m), balanced; 1: one row, twice

@@ -88,3 +88,32 @@ def test_block_matrix_tiles(gpu, nwaves, k, rows, nfb):
     s3, b3 = run(impl=3, nwaves=nwaves, k=k, rows=rows, nf=nfb)
     assert O.rel_l2(s3, s1) < 2e-6 and O.rel_l2(b3, b1) < 2e-6
     plan.destroy()
+
+
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec"])
+@pytest.mark.parametrize("nwaves,k,rows", [(2, 1, 1), (1, 2, 2), (4, 1, 2)])
+def test_block_matrix_eight_fields(gpu, legendre, nwaves, k, rows):
+    """Eight fields per workgroup (a batched-transform tile: the recurrence
+    shared by four field pairs) gives the four-field result bit for bit: each
+    field's sums run over the same l and pairs in the same order.  Eleven
+    fields, so the second workgroup of eight has five idle fields; the
+    weights of four field pairs go through the analysis tile in two passes."""
+    emit(f"m4 eight fields {legendre} nwaves {nwaves} k {k} rows {rows}")
+    plan, qs, run = _plan_and_fields(127, 128, 256, 11, legendre, 3)
+    s4, b4 = run(impl=3, nwaves=nwaves, k=k, rows=rows, nf=4)
+    s8, b8 = run(impl=3, nwaves=nwaves, k=k, rows=rows, nf=8)
+    assert np.array_equal(s4, s8) and np.array_equal(b4, b8)
+    s1, b1 = run(impl=1)
+    assert O.rel_l2(s8, s1) < 2e-6 and O.rel_l2(b8, b1) < 2e-6
+    plan.destroy()
+
+
+def test_eight_field_tiles_refused_outside_block_kernels(gpu):
+    """Eight fields per workgroup exist for the block-matrix kernels with one
+    or two chains per lane only; other shapes are refused when set."""
+    plan, _, _ = _plan_and_fields(127, 128, 256, 8, "fp32", 1)
+    for bad in (dict(impl=1, nf=8, nwaves=2, k=1), dict(impl=3, nf=8, nwaves=1, k=4),
+                dict(impl=2, nf=8, nwaves=2, k=1)):
+        with pytest.raises(Exception, match="nf in"):
+            plan.set_tile("analys", **bad)
+    plan.destroy()

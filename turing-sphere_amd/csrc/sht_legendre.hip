@@ -1255,9 +1255,11 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
     // B operands for the whole row: each lane computes W of its own pair
     // (every column, both parities) into the chain's tile, then gathers the
     // 16 of its block: pair 16 pg + t, column j, the block's parity.  The
-    // tile holds them as rows q = (parity FP + fp) 4 + column of M4_WROW
+    // tile holds them as rows q = (parity FPS + fp) 4 + column of M4_WROW
     // floats, one per pair: the writes are consecutive, and the gather's
     // 32 distinct addresses (q of the block, 16 pg + t) fall in 32 banks.
+    // A tile holds two field pairs' weights: more go through it in passes.
+    constexpr int FPS = FP < 2 ? FP : 2;
     float W[K][16][FP];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1265,42 +1267,48 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
       lk[k] = wave_min(c[k].ls);
       lzmax = max(lzmax, last_seed(c[k].ls, L));
       lmin = min(lmin, lk[k]);
-      const int jp = pair_of(k);
-      const int jc = min(jp, T.npair - 1);
-      const int js = nlat - 1 - jc;
-      const float w = jp < T.npair ? T.w[jc] : 0.f;
-      float* tw = tile + k * (M4_SPAN * M4_TROW) + lane;  // [parity][fp][col][pair]
-#pragma unroll
-      for (int f = 0; f < 2 * FP; ++f) {
-        const bool live = f < NF && f0 + f < nfield;
-        const int fld = min(f0 + min(f, NF - 1), nfield - 1);
-        const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
-        const float2 gN = g[(long long)jc * (M + 1)], gS = g[(long long)js * (M + 1)];
-        float4 we{0.f, 0.f, 0.f, 0.f};
-        if (live) {
-          if (js == jc)  // equator of an odd grid pairs with itself
-            we = make_float4(gN.x * w, gN.y * w, 0.f, 0.f);
-          else
-            we = make_float4((gN.x + gS.x) * w, (gN.y + gS.y) * w, (gN.x - gS.x) * w,
-                             (gN.y - gS.y) * w);
-        }
-        const int fp = f >> 1, c0 = 2 * (f & 1);
-        tw[((0 * FP + fp) * 4 + c0) * M4_WROW] = we.x;
-        tw[((0 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.y;
-        tw[((1 * FP + fp) * 4 + c0) * M4_WROW] = we.z;
-        tw[((1 * FP + fp) * 4 + c0 + 1) * M4_WROW] = we.w;
-      }
     }
-    wave_sync_lds();
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int fh = 0; fh < FP; fh += FPS) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
+      for (int k = 0; k < K; ++k) {
+        const int jp = pair_of(k);
+        const int jc = min(jp, T.npair - 1);
+        const int js = nlat - 1 - jc;
+        const float w = jp < T.npair ? T.w[jc] : 0.f;
+        float* tw = tile + k * (M4_SPAN * M4_TROW) + lane;  // [parity][fp][col][pair]
 #pragma unroll
-        for (int fp = 0; fp < FP; ++fp)
-          W[k][t][fp] = tile[k * (M4_SPAN * M4_TROW) +
-                             (((odd ? FP : 0) + fp) * 4 + j) * M4_WROW + 16 * pg + t];
-    wave_sync_lds();  // the tiles are rewritten by the spans
+        for (int f = 2 * fh; f < 2 * (fh + FPS); ++f) {
+          const bool live = f < NF && f0 + f < nfield;
+          const int fld = min(f0 + min(f, NF - 1), nfield - 1);
+          const float2* g = fm + (long long)fld * nlat * (M + 1) + m;
+          const float2 gN = g[(long long)jc * (M + 1)], gS = g[(long long)js * (M + 1)];
+          float4 we{0.f, 0.f, 0.f, 0.f};
+          if (live) {
+            if (js == jc)  // equator of an odd grid pairs with itself
+              we = make_float4(gN.x * w, gN.y * w, 0.f, 0.f);
+            else
+              we = make_float4((gN.x + gS.x) * w, (gN.y + gS.y) * w, (gN.x - gS.x) * w,
+                               (gN.y - gS.y) * w);
+          }
+          const int fq = (f >> 1) - fh, c0 = 2 * (f & 1);
+          tw[((0 * FPS + fq) * 4 + c0) * M4_WROW] = we.x;
+          tw[((0 * FPS + fq) * 4 + c0 + 1) * M4_WROW] = we.y;
+          tw[((1 * FPS + fq) * 4 + c0) * M4_WROW] = we.z;
+          tw[((1 * FPS + fq) * 4 + c0 + 1) * M4_WROW] = we.w;
+        }
+      }
+      wave_sync_lds();
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+          for (int fq = 0; fq < FPS; ++fq)
+            W[k][t][fh + fq] = tile[k * (M4_SPAN * M4_TROW) +
+                                    (((odd ? FPS : 0) + fq) * 4 + j) * M4_WROW + 16 * pg + t];
+      wave_sync_lds();  // the tiles are rewritten by the next pass or the spans
+    }
     if (lane == 0) smin[wv] = lmin;
     __syncthreads();  // also publishes the staged row
     int lblk = smin[0];
@@ -1866,6 +1874,28 @@ hipError_t analys_m4_nf(const typename P::Tables& T, int nfield, const LegTile& 
     case 1: return analys_m4<NF, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
     case 2: return analys_m4<NF, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
     case 4: return analys_m4<NF, 4, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Eight fields per workgroup (batched transforms: the recurrence shared by
+// four field pairs), with one or two chains per lane.
+template <class P>
+hipError_t synth_m4_nf8(const typename P::Tables& T, int nfield, const LegTile& t,
+                        const Fields& q, float2* fm, hipStream_t s) {
+  switch (t.k) {
+    case 1: return synth_m4<8, 1, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    case 2: return synth_m4<8, 2, P>(T, nfield, t.nwaves, t.rows, q, fm, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class P>
+hipError_t analys_m4_nf8(const typename P::Tables& T, int nfield, const LegTile& t,
+                         const float2* fm, const Fields& q, hipStream_t s) {
+  switch (t.k) {
+    case 1: return analys_m4<8, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 2: return analys_m4<8, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2552,8 +2582,9 @@ namespace {
 bool fp32_tile_ok(const LegTile& t, int npair) {
   const bool kok = t.impl >= 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
   const int wmax = t.impl == 3 ? 8 : 4;
-  return (t.impl >= 1 && t.impl <= 3) && (t.rows == 1 || t.rows == 2) &&
-         (t.nf == 1 || t.nf == 2 || t.nf == 4) &&
+  // eight fields per workgroup: block-matrix kernels with one or two chains
+  const bool nfok = t.nf == 1 || t.nf == 2 || t.nf == 4 || (t.nf == 8 && t.impl == 3 && t.k <= 2);
+  return (t.impl >= 1 && t.impl <= 3) && (t.rows == 1 || t.rows == 2) && nfok &&
          t.nwaves >= 1 && t.nwaves <= wmax && kok && 64 * t.nwaves * t.k >= npair;
 }
 
@@ -2572,6 +2603,7 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
   if (t.impl == 3) switch (t.nf) {
       case 1: return synth_m4_nf<1, RecF>(T, nfield, t, q, fm, s);
       case 2: return synth_m4_nf<2, RecF>(T, nfield, t, q, fm, s);
+      case 8: return synth_m4_nf8<RecF>(T, nfield, t, q, fm, s);
       default: return synth_m4_nf<4, RecF>(T, nfield, t, q, fm, s);
     }
   switch (t.nf) {
@@ -2594,6 +2626,7 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
   if (t.impl == 3) switch (t.nf) {
       case 1: return analys_m4_nf<1, RecF>(T, nfield, t, fm, q, s);
       case 2: return analys_m4_nf<2, RecF>(T, nfield, t, fm, q, s);
+      case 8: return analys_m4_nf8<RecF>(T, nfield, t, fm, q, s);
       default: return analys_m4_nf<4, RecF>(T, nfield, t, fm, q, s);
     }
   switch (t.nf) {
@@ -2616,6 +2649,7 @@ hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields
   if (t.impl == 3) switch (t.nf) {
       case 1: return synth_m4_nf<1, RecD>(T, nfield, t, q, fm, s);
       case 2: return synth_m4_nf<2, RecD>(T, nfield, t, q, fm, s);
+      case 8: return synth_m4_nf8<RecD>(T, nfield, t, q, fm, s);
       default: return synth_m4_nf<4, RecD>(T, nfield, t, q, fm, s);
     }
   switch (t.nf) {
@@ -2638,6 +2672,7 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
   if (t.impl == 3) switch (t.nf) {
       case 1: return analys_m4_nf<1, RecD>(T, nfield, t, fm, q, s);
       case 2: return analys_m4_nf<2, RecD>(T, nfield, t, fm, q, s);
+      case 8: return analys_m4_nf8<RecD>(T, nfield, t, fm, q, s);
       default: return analys_m4_nf<4, RecD>(T, nfield, t, fm, q, s);
     }
   switch (t.nf) {

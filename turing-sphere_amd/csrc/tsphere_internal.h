@@ -77,7 +77,8 @@ struct LegTablesZ64 {
 
 // Tile shape of a Legendre launch (0 = the built-in choice for that entry).
 //  FP32 form: impl 1 = contraction on the VALU, 2 = on the FP32 matrix cores;
-//  nf fields per workgroup (1, 2, 4), nwaves waves, k latitude pairs per lane
+//  nf fields per workgroup (1, 2, 4; 8 for the block kernels with k <= 2),
+//  nwaves waves, k latitude pairs per lane
 //  (1, 2, 4, 8 on the VALU, 1, 2, 4 on the matrix cores; nwaves * 64 * k must
 //  cover the pairs).
 //  rows (VALU kernels): m-rows per workgroup, 2 = the pair (m, M - m), 1 = one.

@@ -1066,12 +1066,13 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
     } else {
       const bool kok = r.impl >= 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
                                    : (r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8);
-      if (!(r.nf == 1 || r.nf == 2 || r.nf == 4) || r.nwaves < 1 ||
-          r.nwaves > (r.impl == 3 ? 8 : 4) || !kok || 64LL * r.nwaves * r.k < p->npair)
+      const bool nfok = r.nf == 1 || r.nf == 2 || r.nf == 4 || (r.nf == 8 && r.impl == 3 && r.k <= 2);
+      if (!nfok || r.nwaves < 1 || r.nwaves > (r.impl == 3 ? 8 : 4) || !kok ||
+          64LL * r.nwaves * r.k < p->npair)
         return fail(TSPHERE_ERR_INVALID,
-                    "FP32 tile impl=%d nf=%d nwaves=%d k=%d: need nf in {1,2,4}, nwaves in "
-                    "1..4 (1..8 for impl 3), k in {1,2,4,8} (VALU) or {1,2,4} (matrix cores), "
-                    "64*nwaves*k >= %d latitude pairs",
+                    "FP32 tile impl=%d nf=%d nwaves=%d k=%d: need nf in {1,2,4} (or 8 for impl 3 "
+                    "with k <= 2), nwaves in 1..4 (1..8 for impl 3), k in {1,2,4,8} (VALU) or "
+                    "{1,2,4} (matrix cores), 64*nwaves*k >= %d latitude pairs",
                     r.impl, r.nf, r.nwaves, r.k, p->npair);
     }
   }
@@ -1087,7 +1088,8 @@ int tsphere_plan_get_tile(tsphere_plan p, int stage, int nfield, tsphere_leg_til
   return ok();
 }
 
-// Candidate tiles of one stage: FP32 -- every fields-per-block choice with
+// Candidate tiles of one stage: FP32 -- every fields-per-block choice (eight
+// only for the block-matrix kernels, with at most two chains per lane) with
 // every wave count at the smallest k that covers the pairs; MFMA -- the
 // column tiles per block.
 static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield) {
@@ -1103,10 +1105,11 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
     return c;
   }
   for (int impl = 1; impl <= 3; impl++)
-    for (int nf : {1, 2, 4}) {
+    for (int nf : {1, 2, 4, 8}) {
       if (nf > 1 && nf > nfield && !(nf == 4 && nfield == 3)) continue;
+      if (nf == 8 && impl != 3) continue;  // eight fields: block-matrix kernels only
       for (int nw = 1; nw <= (impl == 3 ? 8 : 4); nw += (nw >= 4 ? 4 : 1)) {
-        const int kmax = impl >= 2 ? 4 : 8;
+        const int kmax = nf == 8 ? 2 : impl >= 2 ? 4 : 8;
         int k = 1;
         while (k <= kmax && 64 * nw * k < p->npair) k *= 2;
         if (k > kmax) continue;
