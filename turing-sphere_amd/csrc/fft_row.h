@@ -18,9 +18,9 @@
 //   analysis   z[m] = x[2m] + i x[2m+1], Z = FFT_H^{-}(z),
 //              X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2pi i k/N} (-i)(Z[k] - conj Z[H-k])/2
 // with X[0] = Re F_0, X[m] = F_m (m <= mmax), 0 above.  Twiddles come from the
-// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N); each thread reads
-// the few its radix-4 butterflies need into registers when its kernel starts,
-// from per-pass copies of trig's entries that follow the table on the device.
+// plan's f64-built table trig[k] = (cos, sin)(2 pi k / N); the radix-4
+// passes take theirs from per-pass copies of trig's entries stored after the
+// table on the device, held per thread in registers or per workgroup in LDS.
 #pragma once
 
 namespace tsfft {
