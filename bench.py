@@ -158,7 +158,8 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
 
     cfg = ShtConfig(LMAX, LMAX, NLAT, NPHI)
     npts, nlm = cfg.npts, cfg.nlm
-    log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={legendre}")
+    tag = legendre + (" (plain: vector-ALU Legendre sums)" if plain else "")
+    log(f"[bench] plan lmax {LMAX} {NLAT}x{NPHI} legendre={tag}")
     sht = ShtPlan.create(cfg, legendre=legendre, max_fields=2)
     if plain:
         for st in ("synth", "analys"):
@@ -166,7 +167,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
         tuned = {st: sht.get_tile(st, 2) for st in ("synth", "analys")}
     else:
         tuned = sht.tune(2, reps=5)
-    log(f"[bench] {legendre} tuned tiles: {tuned}")
+    log(f"[bench] {tag} tuned tiles: {tuned}")
     src = open(os.path.join(ROOT, "turing-sphere_amd", "mfiles", "brusselator.m")).read()
     gm = GpuModel.create(sht, cfg, src, ["A", "B", "D1", "D2", "dt"], ["U", "V"], ["u", "v"],
                          spectral=not plain, fold=not plain)
@@ -180,7 +181,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     # initial state through this plan must come back before anything is timed
     rt = float(np.sqrt(np.sum((sht.analys(sht.synth(U0)) - U0.astype(np.float64)) ** 2) /
                        np.sum(U0.astype(np.float64) ** 2)))
-    log(f"[bench] {legendre} round-trip check: relL2 {rt:.2e}")
+    log(f"[bench] {tag} round-trip check: relL2 {rt:.2e}")
     if not rt < 1e-3:
         raise SanityError(f"{legendre}: round trip relL2 {rt:.3e} >= 1e-3; not timing")
     gm.upload("U", U0)
@@ -196,7 +197,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
     elapsed = time.perf_counter() - t0
     U = gm.read("U")
     finite = bool(np.all(np.isfinite(U)))
-    log(f"[bench] {legendre}: {steps} steps ({steps // gm.graph_steps} replays of a "
+    log(f"[bench] {tag}: {steps} steps ({steps // gm.graph_steps} replays of a "
         f"{gm.graph_steps}-step graph) in {elapsed * 1e3:.1f} ms -> {steps / elapsed:.1f} steps/s")
     latency = None
     if latency_steps > 0:
@@ -214,7 +215,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
                    "medianMs": round(float(np.median(samples)), 4),
                    "p90Ms": round(float(np.percentile(samples, 90)), 4),
                    "minMs": round(float(samples.min()), 4)}
-        log(f"[bench] {legendre} one-step latency: {latency}")
+        log(f"[bench] {tag} one-step latency: {latency}")
 
     L = _lib.load()
     s = gm.stream
@@ -252,7 +253,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
         spc = (C.c_int * 2)(0, 1)
         kms["leg_spectral"] = time_stage(lambda: _lib.check(L.tsphere_leg_spectral_imex(
             h, 2, fm.ptr, ptrs, lam.ptr, s1, s2, spc, s.handle)), s, 20)
-    log(f"[bench] {legendre} kernel ms: {kms}")
+    log(f"[bench] {tag} kernel ms: {kms}")
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "graph_steps": graph_size(steps, graph_steps), "round_trip_rel_l2": rt,
            "kernel_ms": kms, "finite": finite,
@@ -267,7 +268,7 @@ def run_variant(legendre: str, steps: int, warmup: int, graph_steps: int, err: b
         out["synth_rel_err_vs_f64"] = O.rel_l2(spat, ref.synth_fast(U0.astype(np.float64)))
         q = sht.analys(u0)
         out["analys_rel_err_vs_f64"] = O.rel_l2(q, ref.analys_fast(u0.astype(np.float64)))
-        log(f"[bench] {legendre} errors vs f64: synth {out['synth_rel_err_vs_f64']:.2e} "
+        log(f"[bench] {tag} errors vs f64: synth {out['synth_rel_err_vs_f64']:.2e} "
             f"analys {out['analys_rel_err_vs_f64']:.2e}")
     gm.destroy()
     sht.destroy()
