@@ -443,6 +443,8 @@ def main() -> int:
     ap.add_argument("--ensemble-only", type=int, default=0, metavar="STEPS",
                     help="only run the 256-run mixed-lmax Allen-Cahn ensemble for STEPS steps "
                          "(a short workload to trace its graph's kernels) and print its figures")
+    ap.add_argument("--sht-only", type=int, default=0, metavar="LMAX",
+                    help="time only the batched round trips of 64 fields at LMAX")
     ap.add_argument("--hbm-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (the known-size "
                          "workload that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
@@ -451,6 +453,12 @@ def main() -> int:
         return hbm_calibration()
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(args.ensemble_only), default=float), flush=True)
+        return 0
+    if args.sht_only:
+        # the batched round trips alone (64 fields at this lmax), e.g. for a
+        # kernel trace of the batched transform's kernels
+        print(json.dumps({leg: batched_sht(args.sht_only, 64, leg)
+                          for leg in ("fp32", "f64rec", "bf16x3")}, default=float), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the step runs on a "

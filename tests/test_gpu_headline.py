@@ -30,7 +30,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MFILES = os.path.join(ROOT, "turing-sphere_amd", "mfiles")
 BRUSS = ["A", "B", "D1", "D2", "dt"]
-C2 = {"A": 4.5, "B": 6.96, "D1": 2e-4, "D2": 8 * 2e-4, "dt": 0.002}
+BRUSSELATOR_PARAMS = {"A": 4.5, "B": 6.96, "D1": 2e-4, "D2": 8 * 2e-4, "dt": 0.002}
 LMAX, NLAT, NPHI = 1023, 1024, 2048
 
 # After 10 steps, against the f64 trajectory from the same f32 initial state:
@@ -52,13 +52,13 @@ TOLS = {"fp32": (STATE_TOL, INCR_TOL), "f64rec": (STATE_TOL, INCR_TOL), "bf16x3"
 _INIT = {}
 
 
-def _c2_initial(lmax=LMAX, nlat=NLAT, nphi=NPHI):
+def _brusselator_initial(lmax=LMAX, nlat=NLAT, nphi=NPHI):
     key = (lmax, nlat, nphi)
     if key not in _INIT:
         npts = nlat * nphi
         r = O.mulberry32(0, 2 * npts)
-        u0 = C2["A"] + (0.02 * r[:npts] - 0.01)
-        v0 = C2["B"] / C2["A"] + (0.02 * r[npts:] - 0.01)
+        u0 = BRUSSELATOR_PARAMS["A"] + (0.02 * r[:npts] - 0.01)
+        v0 = BRUSSELATOR_PARAMS["B"] / BRUSSELATOR_PARAMS["A"] + (0.02 * r[npts:] - 0.01)
         ref = O.ShtReference(O.Cfg(lmax, lmax, nlat, nphi))
         U0 = ref.analys_fast(u0.astype(np.float32).astype(np.float64)).astype(np.float32)
         V0 = ref.analys_fast(v0.astype(np.float32).astype(np.float64)).astype(np.float32)
@@ -66,7 +66,7 @@ def _c2_initial(lmax=LMAX, nlat=NLAT, nphi=NPHI):
     return _INIT[key]
 
 
-def _make(lmax, nlat, nphi, legendre, spectral, fold, src="brusselator", params=C2):
+def _make(lmax, nlat, nphi, legendre, spectral, fold, src="brusselator", params=BRUSSELATOR_PARAMS):
     import tsphere
     from tsphere.model import GpuModel
 
@@ -80,7 +80,7 @@ def _make(lmax, nlat, nphi, legendre, spectral, fold, src="brusselator", params=
     return sht, gm
 
 
-def _run_plans(lmax, nlat, nphi, legendre, U0, V0, src="brusselator", params=C2,
+def _run_plans(lmax, nlat, nphi, legendre, U0, V0, src="brusselator", params=BRUSSELATOR_PARAMS,
                forms=((True, False), (False, True), (False, False)), expect=None):
     outs = []
     for spectral, fold in forms:
@@ -113,7 +113,7 @@ def _run_plans(lmax, nlat, nphi, legendre, U0, V0, src="brusselator", params=C2,
 @pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
 def test_headline_step_plans_bit_exact(gpu, legendre):
     emit(f"headline step plans, Brusselator lmax {LMAX} {NLAT}x{NPHI} {legendre}")
-    U0, V0, _ = _c2_initial()
+    U0, V0, _ = _brusselator_initial()
     expect = {(True, False): "spectral", (False, True): "fold", (False, False): "separate"}
     _run_plans(LMAX, NLAT, NPHI, legendre, U0, V0, expect=expect)
 
@@ -122,7 +122,7 @@ def test_headline_step_plans_bit_exact(gpu, legendre):
 def test_headline_step_vs_f64_trajectory(gpu, legendre):
     steps = 10
     emit(f"headline step vs f64 trajectory, {steps} steps, {legendre}")
-    U0, V0, ref = _c2_initial()
+    U0, V0, ref = _brusselator_initial()
     sht, gm = _make(LMAX, NLAT, NPHI, legendre, True, False)
     assert "spectral" in [l.split()[0] for l in gm.describe()["step"]]
     gm.upload("U", U0)
@@ -132,7 +132,7 @@ def test_headline_step_vs_f64_trajectory(gpu, legendre):
     gm.destroy()
     sht.destroy()
     emit("  f64 trajectory ...")
-    Ur, Vr = O.imex_trajectory(ref.cfg, O.BRUSSELATOR, C2, U0, V0, steps, ref)
+    Ur, Vr = O.imex_trajectory(ref.cfg, O.BRUSSELATOR, BRUSSELATOR_PARAMS, U0, V0, steps, ref)
     d64 = [Ur - U0.astype(np.float64), Vr - V0.astype(np.float64)]
     eu = O.rel_l2(U.astype(np.float64) - U0, d64[0])
     ev = O.rel_l2(V.astype(np.float64) - V0, d64[1])
@@ -148,7 +148,7 @@ def test_two_chain_spectral_op_plans_bit_exact(gpu, legendre):
     """768 latitude pairs: the spectral op's lanes carry two 64-pair chains."""
     lmax, nlat, nphi = 767, 1536, 2048
     emit(f"two-chain spectral op plans lmax {lmax} {nlat}x{nphi} {legendre}")
-    U0, V0, _ = _c2_initial(lmax, nlat, nphi)
+    U0, V0, _ = _brusselator_initial(lmax, nlat, nphi)
     expect = {(True, False): "spectral", (False, True): "fold", (False, False): "separate"}
     _run_plans(lmax, nlat, nphi, legendre, U0, V0, expect=expect)
 
@@ -159,7 +159,7 @@ def test_beyond_spectral_limit_takes_fold(gpu):
     which matches separate launches bit for bit."""
     lmax, nlat, nphi = 1023, 2050, 2048
     emit(f"beyond the spectral op's limit: lmax {lmax} {nlat}x{nphi}")
-    U0, V0, _ = _c2_initial(lmax, nlat, nphi)
+    U0, V0, _ = _brusselator_initial(lmax, nlat, nphi)
     expect = {(True, False): "fold", (False, False): "separate"}
     _run_plans(lmax, nlat, nphi, "fp32", U0, V0, forms=((True, False), (False, False)),
                expect=expect)
@@ -171,7 +171,7 @@ def test_split_bf16_beyond_block_limit_separate(gpu):
     column-tile kernels, and a spectral request plans exactly that."""
     lmax, nlat, nphi = 767, 1536, 2048
     emit(f"split-BF16 beyond the block kernels' limit: lmax {lmax} {nlat}x{nphi}")
-    U0, V0, _ = _c2_initial(lmax, nlat, nphi)
+    U0, V0, _ = _brusselator_initial(lmax, nlat, nphi)
     expect = {(True, False): "separate", (False, False): "separate"}
     _run_plans(lmax, nlat, nphi, "bf16x3", U0, V0, forms=((True, False), (False, False)),
                expect=expect)

@@ -1593,7 +1593,7 @@ void k_leg_spectral_m4(typename P::Tables T, int nfield, float2* fm, Fields St,
       // the span's alpha in batches read ahead of the tile stores: all 16 of
       // the f32 form at once, the f64 form's four at a time (16 doubles would
       // take 32 VGPRs of a kernel held to 128; with 4 it spills 28 bytes per
-      // lane instead of 120, and its step measured 152 -> 140 us at C2)
+      // lane instead of 120, and its step measured 152 -> 140 us at lmax 1023 with two fields)
       constexpr int AB = sizeof(typename P::ab_t) == 8 ? M4_SPAN / 4 : M4_SPAN;
 #pragma unroll
       for (int pb = 0; pb < M4_SPAN; pb += AB) {

@@ -708,7 +708,7 @@ def _grid_groups_kernel(name, label, log2h, nlat, mmax, synth, ext, gins, eouts,
     # rows of up to 1024 complex points take G x 16 KiB of LDS, which leaves
     # room for 8 waves per SIMD: registers are capped to match (64 VGPRs;
     # without the cap the pass twiddles held through the element-wise part
-    # take the C2 kernel to 70 and a CU holds 3 workgroups instead of 4)
+    # take the two-field 1024 x 2048 kernel to 70 and a CU holds 3 workgroups instead of 4)
     occ = " __attribute__((amdgpu_waves_per_eu(8)))" if log2h <= 10 else ""
     ap(f'extern "C" __global__ __launch_bounds__({256 * G}){occ} void {name}({", ".join(sig)}) {{')
     ap(f"  using S = tsfft::FftShape<{log2h}>;")
