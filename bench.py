@@ -296,6 +296,107 @@ def batched_sht(lmax: int, nf: int, legendre: str, reps: int = 10):
             "tiles": {k: tiles[k] for k in ("synth", "analys")}}
 
 
+def reference_sht_bench(lmax: int = 63, steps: int = 2000, warmup: int = 100, batch: int = 16,
+                        latency_steps: int = 200):
+    """The reference's transform benchmark (scripts/bench-sht.ts:187-360) on
+    this library: one field at lmax 63 on the grid its default run uses
+    (gridForLmax(63, 3) = 128 x 256, src/sht/layout.ts:53-59), a round trip
+    = one synthesis and one analysis, alternating two spectral buffers.
+    Throughput as batches of `batch` round trips per submission awaited once
+    each (one captured HIP graph per submission here), latency as single
+    round trips awaited each; a first round trip must come back within 1e-3
+    relative L2 before anything is timed.  Published for this command
+    (README.md:350-356, RTX PRO 6000 Blackwell): 0.084 ms per round trip
+    through WebGPU (11,848/s), 0.021 ms through upstream SHTNS's CUDA path
+    (48,009/s)."""
+    import tsphere
+    from tsphere import Event, _lib
+
+    nlat = 2 * -(-max(lmax + 1, (4 * lmax + 1) / 2) // 2)
+    nlat = int(nlat)
+    nphi = 1
+    while nphi < 4 * lmax + 1:
+        nphi *= 2
+    cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
+    p = tsphere.ShtPlan.create(cfg, max_fields=1)
+    s = p.stream
+    rng = np.random.default_rng(1)
+    nlm = cfg.nlm
+    lsig = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
+    q0 = (rng.normal(size=(nlm, 2)) / (lsig[:, None] + 1.0)).astype(np.float32)
+    q0[:lmax + 1, 1] = 0.0  # m = 0 coefficients are real
+    qb = [tsphere.DeviceBuffer(2 * nlm), tsphere.DeviceBuffer(2 * nlm)]
+    sp = tsphere.DeviceBuffer(cfg.npts)
+    syn = [p.create_synth_binding(qb[i], sp, 1, 2 * nlm, cfg.npts) for i in (0, 1)]
+    ana = [p.create_analys_binding(sp, qb[1 - i], 1, cfg.npts, 2 * nlm) for i in (0, 1)]
+    cur = [0]
+
+    def encode(n):
+        for _ in range(n):
+            p.encode_synth_into(s, syn[cur[0]])
+            p.encode_analys_into(s, ana[cur[0]])
+            cur[0] ^= 1
+
+    qb[0].upload(q0.reshape(-1), s)
+    encode(1)
+    s.sync()
+    back = qb[1].download(s).astype(np.float64)
+    ref = q0.reshape(-1).astype(np.float64)
+    rt = float(np.sqrt(np.sum((back - ref) ** 2) / np.sum(ref ** 2)))
+    if not rt < 1e-3:
+        raise SanityError(f"reference transform benchmark: round trip relL2 {rt:.3e} >= 1e-3")
+    qb[0].upload(q0.reshape(-1), s)
+    cur[0] = 0
+    encode(warmup)
+    s.sync()
+    # one submission = one graph of `batch` round trips (an even count, so
+    # every replay starts from the same buffer)
+    import ctypes as C_
+
+    _lib.call("tsphere_graph_begin", s.handle)
+    try:
+        encode(batch)
+    finally:
+        h = C_.c_void_p()
+        rc = _lib.load().tsphere_graph_end(s.handle, C_.byref(h))
+    _lib.check(rc, "graph capture")
+    nb = max(1, -(-steps // batch))
+    t0 = time.perf_counter()
+    for _ in range(nb):
+        _lib.call("tsphere_graph_launch", h.value, s.handle)
+        s.sync()
+    thr_ms = (time.perf_counter() - t0) * 1e3 / (nb * batch)
+    lat = np.empty(latency_steps)
+    for i in range(latency_steps):
+        t1 = time.perf_counter()
+        encode(1)
+        s.sync()
+        lat[i] = (time.perf_counter() - t1) * 1e3
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    for _ in range(nb):
+        _lib.call("tsphere_graph_launch", h.value, s.handle)
+    e1.record(s)
+    gpu_ms = e0.elapsed_ms(e1) / (nb * batch)
+    _lib.call("tsphere_graph_destroy", h.value)
+    finite = bool(np.all(np.isfinite(qb[0].download(s))))
+    p.destroy()
+    return {"workload": f"1 field, lmax {lmax}, {nlat}x{nphi} (the reference's bench:sht default), "
+                        f"round trip = synth + analys",
+            "first_round_trip_rel_l2": rt,
+            "round_trips": nb * batch, "batch": batch,
+            "ms_per_round_trip": round(thr_ms, 5),
+            "round_trips_per_s": round(1e3 / thr_ms, 1),
+            "gpu_ms_per_round_trip": round(gpu_ms, 5),
+            "latency": {"samples": latency_steps, "meanMs": round(float(lat.mean()), 4),
+                        "medianMs": round(float(np.median(lat)), 4),
+                        "minMs": round(float(lat.min()), 4)},
+            "published": {"webgpu_rtx_pro_6000_ms": 0.084, "shtns_cuda_rtx_pro_6000_ms": 0.021},
+            "speedup_vs_published": {"webgpu_rtx_pro_6000": round(0.084 / thr_ms, 2),
+                                     "shtns_cuda_rtx_pro_6000": round(0.021 / thr_ms, 2)},
+            "finite": finite}
+
+
 def ensemble_bench(steps: int = 200, legendre: str = "fp32"):
     """256 Allen-Cahn runs, 64 each at lmax 63/127/255/511 on (L+1) x 2(L+1)
     Gauss grids, eps = U[0.01, 0.05] (mulberry32 seed 0, its own stream),
@@ -512,6 +613,8 @@ def main() -> int:
         for leg in ("fp32", "f64rec", "bf16x3"):
             extras[f"sht_64fields_l511_{leg}"] = batched_sht(511, 64, leg)
             log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
+        extras["reference_sht_bench"] = reference_sht_bench()
+        log(f"[bench] reference transform benchmark: {extras['reference_sht_bench']}")
         extras["reference_default_run"] = default_run()
         log(f"[bench] default run: {extras['reference_default_run']}")
         extras["ensemble_allencahn_256runs"] = ensemble_bench()
