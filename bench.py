@@ -615,6 +615,18 @@ def main() -> int:
             log(f"[bench] 64-field lmax 511 {leg}: {extras[f'sht_64fields_l511_{leg}']}")
         extras["reference_sht_bench"] = reference_sht_bench()
         log(f"[bench] reference transform benchmark: {extras['reference_sht_bench']}")
+        # the same at the grids of src/sht/sht.ts:72-76's published WGSL round
+        # trips (RTX PRO 6000; lmax 63 / 127 / 255 on 128x256 / 256x512 /
+        # 512x1024: 37.8 / 66.6 / 131.9 us)
+        extras["reference_sht_sizes"] = {}
+        for lm, pub in ((63, 0.0378), (127, 0.0666), (255, 0.1319)):
+            r = reference_sht_bench(lm, steps=400, warmup=20, latency_steps=20)
+            extras["reference_sht_sizes"][f"lmax{lm}"] = {
+                "ms_per_round_trip": r["ms_per_round_trip"],
+                "gpu_ms_per_round_trip": r["gpu_ms_per_round_trip"],
+                "published_webgpu_rtx_pro_6000_ms": pub,
+                "speedup": round(pub / r["ms_per_round_trip"], 2)}
+        log(f"[bench] reference transform sizes: {extras['reference_sht_sizes']}")
         extras["reference_default_run"] = default_run()
         log(f"[bench] default run: {extras['reference_default_run']}")
         extras["ensemble_allencahn_256runs"] = ensemble_bench()
