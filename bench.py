@@ -311,12 +311,9 @@ def reference_sht_bench(lmax: int = 63, steps: int = 2000, warmup: int = 100, ba
     (48,009/s)."""
     import tsphere
     from tsphere import Event, _lib
+    from tsphere.layout import grid_for_lmax
 
-    nlat = 2 * -(-max(lmax + 1, (4 * lmax + 1) / 2) // 2)
-    nlat = int(nlat)
-    nphi = 1
-    while nphi < 4 * lmax + 1:
-        nphi *= 2
+    nlat, nphi = grid_for_lmax(lmax, 3)  # the default preset's cubic reaction
     cfg = tsphere.ShtConfig(lmax, lmax, nlat, nphi)
     p = tsphere.ShtPlan.create(cfg, max_fields=1)
     s = p.stream
