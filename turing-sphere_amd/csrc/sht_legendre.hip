@@ -1900,6 +1900,343 @@ hipError_t analys_m4_nf8(const typename P::Tables& T, int nfield, const LegTile&
   }
 }
 
+// ---------------------------- per-parity matrix products (tile impl 5)
+// For launches of many fields.  Per m-row and parity of l - m both
+// contractions are plain matrix products on v_mfma_f32_16x16x4_f32 (operand
+// layout as for the 16x16x4 kernels above), with 16-column blocks that are
+// (field, re/im) of 8 fields:
+//   synthesis  D_par[pair][col] = sum_{l of par} z_l[pair] (c_l Q_l)[col]
+//   analysis   D_par[l][col]    = sum_pairs z_l[pair] W_par[pair][col]
+// Every column is live and every product adds a nonzero term (the 16x16x4
+// kernels above carry both parities in their columns, so half their products
+// are zero), and one recurrence value serves 8 CB fields (the 4x4 block
+// kernels: 2 or 4).  One workgroup per m-row and group of NF = 8 CB fields;
+// wave w runs the recurrence of pairs 64 w .. 64 w + 63 (lane = pair) and
+// passes each span's values through a wave-private LDS tile that holds
+// (z_l, z_{l+1}) of a pair side by side, so one 8-byte load gives a lane its
+// operands of both parities.  Spans start at the row's first significant l,
+// which is m + 4i, so span offset p has the parity of l - m.
+constexpr int G16_SSPAN = 16;        // synthesis: l per span (two 4-l steps per parity)
+constexpr int G16_ASPAN = 32;        // analysis: l per span (16 rows per parity)
+constexpr int G16_STROW = 128 + 32;  // synthesis tile: floats per row (l-pair) of 64 pairs x 2
+constexpr int G16_ATROW = 128 + 4;   // analysis tile: floats per row (l-pair)
+// staged c_l Q_l of the synthesis: floats per row (l-pair) of 16 CB columns x
+// 2 parities, padded so that the two rows a half-wave reads fall in distinct banks
+__host__ __device__ constexpr int g16_qrow(int cb) { return cb == 1 ? 32 : 96; }
+
+// The row's alpha shifted by two (sal[i] = alpha at l - m = i + 2), zero past
+// the row: a span's alphas then start 16-byte aligned.
+template <class P>
+__device__ __forceinline__ void g16_stage_alpha(const typename P::Tables& T, int m,
+                                                typename P::ab_t* sal) {
+  const int n = T.lmax - m + 1;
+  const long long base = lm_base(T.lmax, m);
+  for (int i = threadIdx.x; i < analys_ab_len(T.lmax); i += blockDim.x)
+    sal[i] = i + 2 < n ? T.ab[base + i + 2] : P::zero_ab();
+}
+
+// One span of SP values of this lane's chain from ls into its tile row
+// (p / 2: (z_{ls+p}, z_{ls+p+1})), the seed taken at its position.
+template <int SP, int TROW, class P>
+__device__ __forceinline__ void g16_recur(ChainT<P>& c, int ls, int lz, int m,
+                                          const typename P::ab_t* sal, float* tile) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int pb = 0; pb < SP; pb += 8) {
+    typename P::ab_t al[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) al[q] = sal[ls - m + pb + q];
+#pragma unroll
+    for (int p = pb; p < pb + 8; p += 2) {
+      if (p % kTile == 0 && ls <= lz && c.ls == ls + p) {
+        c.y0 = c.s0;
+        c.y1 = c.s1;
+      }
+      *reinterpret_cast<float2*>(tile + (p / 2) * TROW + 2 * lane) =
+          make_float2((float)c.y0, (float)c.y1);
+      P::advance(al[p - pb], al[p - pb + 1], c.ct, c.y0, c.y1);
+    }
+  }
+}
+
+template <int CB, class P>
+__global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int nfield, Fields Fq,
+                                                       float2* __restrict__ fm) {
+  constexpr int NF = 8 * CB, SP = G16_SSPAN, QROW = g16_qrow(CB), TROW = G16_STROW;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nthr = blockDim.x;
+  const int nwaves = nthr >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  auto* sal = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  float* const sq = reinterpret_cast<float*>(sal + analys_ab_len(L));  // [2][SP / 2][QROW]
+  float* const tile = sq + 2 * (SP / 2) * QROW + wv * ((SP / 2) * TROW);
+  const int f0 = blockIdx.y * NF;
+  const int m = row_m(0, 1, M, false);
+  const long long base = lm_base(L, m);
+  g16_stage_alpha<P>(T, m, sal);
+
+  ChainT<P> c;
+  load_chain_t<P>(T, m, 64 * wv + lane, c);
+  const int lk = wave_min(c.ls), lz = last_seed(c.ls, L);
+  if (lane == 0) smin[wv] = lk;
+  __syncthreads();
+  int lblk = smin[0];
+  for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+
+  // c_l Q_l of span [ls, ls + SP): row i holds l = ls + 2 i (even l - m) and
+  // l + 1 (odd), column c = (field, re/im) at floats 2 c + parity, so a
+  // field's four values of a row are one 16-byte store; zero past lmax
+  auto stage = [&](int ls, float* d) {
+    for (int it = tid; it < NF * (SP / 2); it += nthr) {
+      const int f = it / (SP / 2), i = it % (SP / 2);
+      const int l = ls + 2 * i;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (f0 + f < nfield && l <= L) {
+        const float2* src = reinterpret_cast<const float2*>(Fq.at(f0 + f)) + base + (l - m);
+        const float2 a = src[0];
+        const float ca = T.cz[base + (l - m)];
+        float2 b = make_float2(0.f, 0.f);
+        float cb = 0.f;
+        if (l + 1 <= L) {
+          b = src[1];
+          cb = T.cz[base + (l - m) + 1];
+        }
+        v = make_float4(a.x * ca, b.x * cb, a.y * ca, b.y * cb);
+      }
+      *reinterpret_cast<float4*>(d + i * QROW + 4 * f) = v;
+    }
+  };
+
+  f4v acc[2][4][CB];  // [parity][16-pair block][column block]
+#pragma unroll
+  for (int pr = 0; pr < 2; ++pr)
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[pr][pb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  if (lblk <= L) stage(lblk, sq);
+  int buf = 0;
+  for (int ls = lblk; ls <= L; ls += SP) {
+    __syncthreads();  // span ls staged; the other buffer's readers are done
+    if (ls + SP <= L) stage(ls + SP, sq + (buf ^ 1) * ((SP / 2) * QROW));
+    if (lk < ls + SP) {  // a lane of this wave is significant by the span's end
+      g16_recur<SP, TROW, P>(c, ls, lz, m, sal, tile);
+      wave_sync_lds();
+      const float* d = sq + buf * ((SP / 2) * QROW);
+#pragma unroll
+      for (int ks = 0; ks < SP / 8; ++ks) {
+        // k = g: l = ls + 8 ks + 2 g (+1 for the odd parity)
+        float2 b[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          b[cb] = *reinterpret_cast<const float2*>(d + (4 * ks + g) * QROW + 32 * cb + 2 * r16);
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) {
+          const float2 a =
+              *reinterpret_cast<const float2*>(tile + (4 * ks + g) * TROW + 32 * pb + 2 * r16);
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb) {
+            acc[0][pb][cb] = mf4(a.x, b[cb].x, acc[0][pb][cb]);
+            acc[1][pb][cb] = mf4(a.y, b[cb].y, acc[1][pb][cb]);
+          }
+        }
+      }
+      wave_sync_lds();  // the tile is rewritten in the next span
+    }
+    buf ^= 1;
+  }
+
+  // north = E + O, south = E - O; D row 4 g + r is pair 64 w + 16 pb + 4 g + r
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const int f = f0 + 8 * cb + (r16 >> 1);
+      if (f >= nfield) continue;
+      float* out = reinterpret_cast<float*>(fm + (long long)f * nlat * (M + 1) + m) + (r16 & 1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int jp = 64 * wv + 16 * pb + 4 * g + r;
+        if (jp >= T.npair) continue;
+        const float e = acc[0][pb][cb][r], o = acc[1][pb][cb][r];
+        out[2LL * jp * (M + 1)] = e + o;
+        out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+      }
+    }
+}
+
+template <int CB, class P>
+__global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, int nfield,
+                                                        const float2* __restrict__ fm, Fields Fq) {
+  constexpr int NF = 8 * CB, SP = G16_ASPAN, TROW = G16_ATROW;
+  constexpr int NO = 2 * CB * 256;  // a wave's sums of one span
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ int smin[8];
+  const int L = T.lmax, M = T.mmax, nlat = T.nlat;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nthr = blockDim.x;
+  const int nwaves = nthr >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  auto* sal = reinterpret_cast<typename P::ab_t*>(lds_raw);
+  float* const tiles = reinterpret_cast<float*>(sal + analys_ab_len(L));
+  float* const tile = tiles + wv * ((SP / 2) * TROW);
+  // the waves' sums of a span, [nwaves][NO]: one buffer, so that three
+  // 4-wave workgroups of 16 fields fit a CU (two buffers and one barrier per
+  // span measured 232 against 202 us for 64 fields at lmax 511)
+  float* const red = tiles + nwaves * ((SP / 2) * TROW);
+  const int f0 = blockIdx.y * NF;
+  const int m = row_m(0, 1, M, false);
+  const long long base = lm_base(L, m);
+  g16_stage_alpha<P>(T, m, sal);
+
+  ChainT<P> c;
+  load_chain_t<P>(T, m, 64 * wv + lane, c);
+  const int lk = wave_min(c.ls), lz = last_seed(c.ls, L);
+  if (lane == 0) smin[wv] = lk;
+
+  // B operands of the whole row: W_par of pair 64 w + 4 kp + g, column
+  // 16 cb + r16 (W_E = (G_N + G_S) w, W_O = (G_N - G_S) w, leg.ts:207-219)
+  float W[2][16][CB];
+#pragma unroll
+  for (int kp = 0; kp < 16; ++kp)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const int jp = 64 * wv + 4 * kp + g;
+      const int f = f0 + 8 * cb + (r16 >> 1);
+      float e = 0.f, o = 0.f;
+      if (jp < T.npair && f < nfield) {
+        const float* gp =
+            reinterpret_cast<const float*>(fm + (long long)f * nlat * (M + 1) + m) + (r16 & 1);
+        const int js = nlat - 1 - jp;
+        const float w = T.w[jp];
+        const float gN = gp[2LL * jp * (M + 1)], gS = gp[2LL * js * (M + 1)];
+        if (js == jp) {  // equator of an odd grid pairs with itself
+          e = gN * w;
+        } else {
+          e = (gN + gS) * w;
+          o = (gN - gS) * w;
+        }
+      }
+      W[0][kp][cb] = e;
+      W[1][kp][cb] = o;
+    }
+  __syncthreads();  // smin and the staged alpha
+  int lblk = smin[0];
+  for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+  lblk = min(lblk, L + 1);
+
+  // coefficients below every lane's significance point are exactly 0
+  const int nz = lblk - m;
+  for (int o = tid; o < NF * nz; o += nthr) {
+    const int f = o / nz, i = o - f * nz;
+    if (f0 + f < nfield)
+      reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + i] =
+          make_float2(0.f, 0.f);
+  }
+
+  for (int ls = lblk; ls <= L; ls += SP) {
+    f4v acc[2][CB];
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[pr][cb] = f4v{0.f, 0.f, 0.f, 0.f};
+    if (lk < ls + SP) {  // a lane of this wave is significant by the span's end
+      g16_recur<SP, TROW, P>(c, ls, lz, m, sal, tile);
+      wave_sync_lds();
+#pragma unroll
+      for (int kp = 0; kp < 16; ++kp) {
+        // A[row r16][k g]: z at l = ls + 2 r16 (+1) of pair 4 kp + g
+        const float2 a = *reinterpret_cast<const float2*>(tile + r16 * TROW + 2 * (4 * kp + g));
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          acc[0][cb] = mf4(a.x, W[0][kp][cb], acc[0][cb]);
+          acc[1][cb] = mf4(a.y, W[1][kp][cb], acc[1][cb]);
+        }
+      }
+      wave_sync_lds();  // the tile is rewritten in the next span
+    }
+    // this wave's sums of the span: D row 4 g + r is l = ls + 2 (4 g + r) +
+    // parity.  red holds, per wave, [cb][row][field slot][re/im][parity]:
+    // one field's four values of a row (l, l + 1; re, im) are 16 contiguous
+    // bytes, its slot the field's index XOR the row's low bits, so that the
+    // combine's 16-byte reads of consecutive rows fall in distinct banks
+    float* rw = red + wv * NO;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * g + r;
+        *reinterpret_cast<float2*>(
+            rw + ((cb * 16 + row) * 8 + ((r16 >> 1) ^ (row & 7))) * 4 + 2 * (r16 & 1)) =
+            make_float2(acc[0][cb][r], acc[1][cb][r]);
+      }
+    __syncthreads();
+    // the waves' sums in wave order: thread (field, row) takes l = ls + 2 row
+    // and l + 1, re and im; consecutive threads on consecutive l of a field
+    for (int o = tid; o < NF * 16; o += nthr) {
+      const int f = o >> 4, row = o & 15;
+      const int l = ls + 2 * row;
+      if (l > L || f0 + f >= nfield) continue;
+      const int idx = (((f >> 3) * 16 + row) * 8 + ((f & 7) ^ (row & 7))) * 4;
+      float4 v = *reinterpret_cast<const float4*>(red + idx);
+      for (int w = 1; w < nwaves; ++w) {
+        const float4 u = *reinterpret_cast<const float4*>(red + w * NO + idx);
+        v.x += u.x;
+        v.y += u.y;
+        v.z += u.z;
+        v.w += u.w;
+      }
+      float2* dst = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f))) + base + (l - m);
+      const float c0 = T.cz[base + (l - m)];
+      dst[0] = make_float2(v.x * c0, v.z * c0);  // (re, im) of l: parity 0
+      if (l + 1 <= L) {
+        const float c1 = T.cz[base + (l - m) + 1];
+        dst[1] = make_float2(v.y * c1, v.w * c1);
+      }
+    }
+    __syncthreads();  // red is rewritten in the next span
+  }
+}
+
+template <int CB, class P>
+hipError_t synth_g16(const typename P::Tables& T, int nfield, int nwaves, const Fields& q,
+                     float2* fm, hipStream_t s) {
+  if (nwaves < 1 || nwaves > 8 || 64 * nwaves < T.npair) return hipErrorInvalidValue;
+  dim3 grid(T.mmax + 1, (nfield + 8 * CB - 1) / (8 * CB));
+  const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
+                     sizeof(float) * (2 * (G16_SSPAN / 2) * g16_qrow(CB) +
+                                      (size_t)nwaves * (G16_SSPAN / 2) * G16_STROW);
+  hipLaunchKernelGGL((k_leg_synth_g16<CB, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm);
+  return hipGetLastError();
+}
+
+template <int CB, class P>
+hipError_t analys_g16(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
+                      const Fields& q, hipStream_t s) {
+  if (nwaves < 1 || nwaves > 8 || 64 * nwaves < T.npair) return hipErrorInvalidValue;
+  dim3 grid(T.mmax + 1, (nfield + 8 * CB - 1) / (8 * CB));
+  const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
+                     sizeof(float) * (size_t)nwaves * ((G16_ASPAN / 2) * G16_ATROW + 2 * CB * 256);
+  hipLaunchKernelGGL((k_leg_analys_g16<CB, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm, q);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t synth_g16_nf(const typename P::Tables& T, int nfield, const LegTile& t,
+                        const Fields& q, float2* fm, hipStream_t s) {
+  return t.nf == 8 ? synth_g16<1, P>(T, nfield, t.nwaves, q, fm, s)
+                   : synth_g16<2, P>(T, nfield, t.nwaves, q, fm, s);
+}
+
+template <class P>
+hipError_t analys_g16_nf(const typename P::Tables& T, int nfield, const LegTile& t,
+                         const float2* fm, const Fields& q, hipStream_t s) {
+  return t.nf == 8 ? analys_g16<1, P>(T, nfield, t.nwaves, fm, q, s)
+                   : analys_g16<2, P>(T, nfield, t.nwaves, fm, q, s);
+}
+
 // ------------------------------ split-BF16 x3 on 4x4x4 matrix blocks (b4)
 // v_mfma_f32_4x4x4_16b_bf16 keeps the block and output layout of the 4x4x1
 // FP32 product above (A_b[r][k] = the k-th bf16 of lane 4b + r, B_b[k][j] =
@@ -2560,6 +2897,17 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
     return t;
   }
   if (t.impl == 0) t.impl = kDefaultImpl;
+  // per-parity matrix products: one m-row per workgroup, 8 or 16 fields, one
+  // chain per lane, a wave per 64 pairs (up to 512 pairs)
+  if (t.impl == 5 && npair > 64 * 8) t = LegTile{0, 0, 0, 0, 3, 0};
+  if (t.impl == 5) {
+    if (t.nf == 0) t.nf = nfield >= 16 ? 16 : 8;
+    if (t.nwaves == 0) t.nwaves = (npair + 63) / 64;
+    if (t.k == 0) t.k = 1;
+    if (t.rows == 0) t.rows = 1;
+    t.ctc = 0;
+    return t;
+  }
   // one row per workgroup when a launch has one or two fields (its workgroups
   // are few and long); the 16x16x4 matrix-core kernels always pair rows
   if (t.rows == 0) t.rows = nfield <= 2 ? 1 : 2;
@@ -2580,6 +2928,9 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
 namespace {
 
 bool fp32_tile_ok(const LegTile& t, int npair) {
+  if (t.impl == 5)
+    return (t.nf == 8 || t.nf == 16) && t.k == 1 && t.rows == 1 && t.nwaves >= 1 &&
+           t.nwaves <= 8 && 64 * t.nwaves >= npair;
   const bool kok = t.impl >= 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
   const int wmax = t.impl == 3 ? 8 : 4;
   // eight fields per workgroup: block-matrix kernels with one or two chains
@@ -2595,6 +2946,7 @@ hipError_t launch_leg_synth(const LegTables& T, int nfield, const Fields& q, flo
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 0, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return synth_g16_nf<RecF>(T, nfield, t, q, fm, s);
   if (t.impl == 2) switch (t.nf) {
       case 1: return synth_mf_nf<1, RecF>(T, nfield, t, q, fm, s);
       case 2: return synth_mf_nf<2, RecF>(T, nfield, t, q, fm, s);
@@ -2618,6 +2970,7 @@ hipError_t launch_leg_analys(const LegTables& T, int nfield, const float2* fm, c
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return analys_g16_nf<RecF>(T, nfield, t, fm, q, s);
   if (t.impl == 2) switch (t.nf) {
       case 1: return analys_mf_nf<1, RecF>(T, nfield, t, fm, q, s);
       case 2: return analys_mf_nf<2, RecF>(T, nfield, t, fm, q, s);
@@ -2641,6 +2994,7 @@ hipError_t launch_leg_synth_f64r(const LegTablesZ64& T, int nfield, const Fields
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 0, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return synth_g16_nf<RecD>(T, nfield, t, q, fm, s);
   if (t.impl == 2) switch (t.nf) {
       case 1: return synth_mf_nf<1, RecD>(T, nfield, t, q, fm, s);
       case 2: return synth_mf_nf<2, RecD>(T, nfield, t, q, fm, s);
@@ -2664,6 +3018,7 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
   if (nfield <= 0) return hipSuccess;
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return analys_g16_nf<RecD>(T, nfield, t, fm, q, s);
   if (t.impl == 2) switch (t.nf) {
       case 1: return analys_mf_nf<1, RecD>(T, nfield, t, fm, q, s);
       case 2: return analys_mf_nf<2, RecD>(T, nfield, t, fm, q, s);

@@ -1046,10 +1046,10 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
   if (!p) return fail(TSPHERE_ERR_INVALID, "plan is NULL");
   if (stage != 0 && stage != 1) return fail(TSPHERE_ERR_INVALID, "stage must be 0 or 1");
   const LegTile t = tile ? to_tile(*tile) : LegTile{};
-  const int impl_max = p->cfg.legendre == TSPHERE_LEG_BF16X3 ? 4 : 3;
-  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > impl_max ||
-      t.rows < 0 || t.rows > 2 || (p->cfg.legendre == TSPHERE_LEG_BF16X3 && t.impl != 0 && t.impl != 4))
-    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0..3, split-BF16 0 or 4; "
+  const bool bf = p->cfg.legendre == TSPHERE_LEG_BF16X3;
+  if (t.nf < 0 || t.nwaves < 0 || t.k < 0 || t.ctc < 0 || t.impl < 0 || t.impl > 5 ||
+      t.rows < 0 || t.rows > 2 || (bf && t.impl != 0 && t.impl != 4) || (!bf && t.impl == 4))
+    return fail(TSPHERE_ERR_INVALID, "tile fields must be >= 0 (impl 0..3 or 5, split-BF16 0 or 4; "
                                      "rows 0, 1 or 2)");
   // validate against a one-field and a max-fields launch
   for (int nf : {1, p->cfg.max_fields}) {
@@ -1063,6 +1063,13 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
     } else if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
       if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
         return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
+    } else if (r.impl == 5) {
+      if (!(r.nf == 8 || r.nf == 16) || r.k != 1 || r.rows != 1 || r.nwaves < 1 ||
+          r.nwaves > 8 || 64LL * r.nwaves < p->npair)
+        return fail(TSPHERE_ERR_INVALID,
+                    "per-parity product tile nf=%d nwaves=%d k=%d rows=%d: need nf 8 or 16, k 1, "
+                    "rows 1, 64*nwaves >= %d latitude pairs (<= 8 waves)",
+                    r.nf, r.nwaves, r.k, r.rows, p->npair);
     } else {
       const bool kok = r.impl >= 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
                                    : (r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8);
@@ -1090,8 +1097,9 @@ int tsphere_plan_get_tile(tsphere_plan p, int stage, int nfield, tsphere_leg_til
 
 // Candidate tiles of one stage: FP32 -- every fields-per-block choice (eight
 // only for the block-matrix kernels, with at most two chains per lane) with
-// every wave count at the smallest k that covers the pairs; MFMA -- the
-// column tiles per block.
+// every wave count at the smallest k that covers the pairs, and the
+// per-parity products with 8 and 16 fields; MFMA -- the column tiles per
+// block.
 static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield) {
   std::vector<LegTile> c;
   if (p->cfg.legendre == TSPHERE_LEG_BF16X3) {
@@ -1118,6 +1126,10 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
         if (impl != 2) c.push_back(LegTile{nf, nw, k, 0, impl, 1});
       }
     }
+  // per-parity matrix products: 8 or 16 fields per workgroup, a wave per 64 pairs
+  if (p->npair <= 64 * 8)
+    for (int nf : {8, 16})
+      if (nf <= nfield) c.push_back(LegTile{nf, (p->npair + 63) / 64, 1, 0, 5, 1});
   return c;
 }
 

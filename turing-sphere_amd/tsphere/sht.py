@@ -116,8 +116,11 @@ class ShtPlan:
         """Fix the Legendre workgroup shape of one stage (0 = built-in); for
         the FP32 and FP64-recurrence forms impl 1 = VALU contraction,
         2 = FP32 matrix cores as 16x16x4 tiles, 3 = FP32 matrix cores as
-        16 4x4 blocks per wave; nf = fields per workgroup (1, 2, 4, and 8
-        for impl 3 with k 1 or 2).  Shapes the library does not have raise."""
+        16 4x4 blocks per wave, 5 = FP32 matrix cores as per-parity
+        products whose 16 columns are (field, re/im) of 8 fields (batched
+        transforms: nf 8 or 16, k 1, rows 1, up to 512 latitude pairs);
+        nf = fields per workgroup (1, 2, 4, and 8 for impl 3 with k 1 or 2).
+        Shapes the library does not have raise."""
         t = _lib.LegTileC(nf, nwaves, k, ctc, impl, rows)
         _lib.call("tsphere_plan_set_tile", self._h, self.STAGES[stage], C.byref(t))
 

@@ -25,7 +25,7 @@ def cache_path() -> str:
 
 def _key(plan, nfield: int) -> str:
     c = plan.cfg
-    return f"gfx950/v4/{plan.legendre}/L{c.lmax}/M{c.mmax}/nlat{c.nlat}/nphi{c.nphi}/nf{nfield}"
+    return f"gfx950/v5/{plan.legendre}/L{c.lmax}/M{c.mmax}/nlat{c.nlat}/nphi{c.nphi}/nf{nfield}"
 
 
 def _load() -> dict:
