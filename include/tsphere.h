@@ -214,6 +214,18 @@ int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float *fm
 int tsphere_leg_imex_synth(tsphere_plan p, int nfield, const float *const *analysed,
                            float *const *state, const float *lam, const float *s1,
                            const float *s2, float *fm, tsphere_stream s);
+/* tsphere_leg_imex_synth for a batch (an ensemble's runs): field f = pointer
+ * f / per, copy f % per at qlm_stride floats (as tsphere_leg_synth_fields),
+ * the same layout for analysed and state; coef is a DEVICE array of 2 nptr
+ * per floats, (s1, s2) of field f at coef[2 f], coef[2 f + 1], so a graph
+ * that captured this call takes new scalars from an upload.  Any field
+ * count up to max_fields; split-BF16 plans up to 512 latitude pairs.
+ * No reference counterpart of its own (it fuses the implicit-solve dispatch
+ * and encodeSynthInto of plan.ts:436-478 for a batch of runs). */
+int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
+                                  const float *const *analysed, float *const *state,
+                                  long long qlm_stride, const float *lam, const float *coef,
+                                  float *fm, tsphere_stream s);
 /* The spectral op: per m-row, the Legendre analysis of the scratch fm
  * (nfield fields, as tsphere_leg_analys_multi), the diagonal implicit step
  *   X_new = (X + s1[f] A_f) / (1 + s2[f] lam)     per real coefficient

@@ -245,3 +245,50 @@ def test_ensemble_grid_fusion_bit_exact(gpu):
         assert np.all(np.isfinite(b))
         emit(f"  max diff {float(np.max(np.abs(a - b))):.1e}")
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
+@pytest.mark.parametrize("model,R", [("schnakenberg", 3), ("allencahn", 9)])
+def test_ensemble_fold_bit_exact(gpu, legendre, model, R):
+    """An ensemble step folds the implicit solve into the next step's
+    synthesis (tsphere_leg_imex_synth_fields, each run's s1 and s2 from a
+    device array): equal to the ensemble plan with the separate generated
+    solve kernel bit for bit, over graph replays, an eager remainder and a
+    per-run parameter change between calls (an upload, no re-capture).
+    Nine single-species runs put a partial group of fields in the last
+    workgroup."""
+    import tsphere
+    from tsphere.model import GpuModel
+
+    cfg = tsphere.ShtConfig(31, 31, 32, 64)
+    if model == "schnakenberg":
+        src, names, state, view = SCH_SRC, ["a", "b", "D1", "D2", "dt"], ["U", "V"], ["u", "v"]
+        p1 = {"a": np.array([0.1, 0.12, 0.08]), "b": 0.9, "D1": 4e-4,
+              "D2": np.array([8e-3, 6e-3, 1e-2]), "dt": 0.05}
+        p2 = dict(p1, D2=np.array([1e-2, 8e-3, 6e-3]), dt=0.04)
+    else:
+        src, names, state, view = AC_SRC, ["eps2", "dt"], ["U"], ["u"]
+        p1 = {"eps2": np.linspace(1e-4, 2.5e-3, R), "dt": 2e-3}
+        p2 = {"eps2": np.linspace(2.5e-3, 1e-4, R), "dt": 3e-3}
+    emit(f"ensemble fold {model} {legendre} lmax31 runs {R}")
+    outs = []
+    for fold in (False, True):
+        sht = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=len(state) * R)
+        gm = GpuModel.create(sht, cfg, src, names, state, view, runs=R, fold=fold)
+        assert ("imex" in [l.split()[0] for l in gm.describe()["step"]]) == fold
+        gm.set_params(p1)
+        rng = np.random.default_rng(11)
+        for k, n in enumerate(state):
+            x = rng.normal(size=2 * sht.nlm * R).astype(np.float32) * 0.01
+            gm.upload(n, x + (1.0 - 0.1 * k) * np.tile(np.eye(1, 2 * sht.nlm, 0, np.float32)[0], R))
+        gm.graph_steps = 5
+        gm.step(12)
+        gm.set_params(p2)
+        gm.step(7)
+        outs.append([gm.read(n) for n in state + view])
+        gm.destroy()
+        sht.destroy()
+    for a, b in zip(*outs):
+        assert np.all(np.isfinite(b))
+        emit(f"  max diff {float(np.max(np.abs(a - b))):.1e}")
+        assert np.array_equal(a, b)

@@ -211,3 +211,67 @@ console.log('ok');
     for f in range(nfield):
         assert np.array_equal(np.fromfile(files[f"ox{f}"], np.float32), xA[f].download(s))
     plan.destroy()
+
+
+@pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
+def test_batched_fold_entry_matches_per_run_calls(gpu, legendre):
+    """tsphere_leg_imex_synth_fields over 2 species x 5 runs (field f =
+    species * 5 + run, the scalars of field f at coef[2 f], coef[2 f + 1] in
+    device memory) against tsphere_leg_imex_synth called once per run with
+    that run's scalars: the implicit step is element-wise and the synthesis
+    sums each field over l in one order, so the state and the scratch agree
+    bit for bit."""
+    import tsphere
+    from tsphere import _lib
+    from tsphere.device import DeviceBuffer
+
+    lmax, nlat, nphi, nsp, R = 31, 32, 64, 2, 5
+    emit(f"batched fold entry {legendre} lmax {lmax} species {nsp} runs {R}")
+    L = _lib.load()
+    plan = tsphere.ShtPlan.create(tsphere.ShtConfig(lmax, lmax, nlat, nphi), legendre=legendre,
+                                  max_fields=nsp * R)
+    s, nlm, M = plan.stream, plan.nlm, lmax
+    rng = np.random.default_rng(7)
+    x0 = (0.1 * rng.normal(size=(nsp, R, 2 * nlm))).astype(np.float32)
+    a0 = (0.1 * rng.normal(size=(nsp, R, 2 * nlm))).astype(np.float32)
+    ell = np.repeat(np.concatenate([np.arange(m, lmax + 1) for m in range(M + 1)]), 2)
+    lamb = DeviceBuffer(2 * nlm, "lam")
+    lamb.upload((ell * (ell + 1)).astype(np.float32), s)
+    s1 = np.float32(rng.uniform(1e-3, 5e-3, size=(nsp, R)))
+    s2 = np.float32(rng.uniform(1e-6, 1e-4, size=(nsp, R)))
+    fsz = 2 * (M + 1) * nlat  # floats per field of the scratch
+
+    def buffers(tag):
+        xs = [DeviceBuffer(R * 2 * nlm, f"x{tag}{k}") for k in range(nsp)]
+        an = [DeviceBuffer(R * 2 * nlm, f"a{tag}{k}") for k in range(nsp)]
+        for k in range(nsp):
+            xs[k].upload(x0[k].reshape(-1), s)
+            an[k].upload(a0[k].reshape(-1), s)
+        return xs, an
+
+    xs, an = buffers("B")
+    coef = DeviceBuffer(2 * nsp * R, "coef")
+    coef.upload(np.stack([s1, s2], axis=-1).reshape(-1), s)
+    fmB = DeviceBuffer(nsp * R * fsz, "fmB")
+    _lib.check(L.tsphere_leg_imex_synth_fields(
+        plan.handle, nsp, R, (C.c_void_p * nsp)(*[b.ptr for b in an]),
+        (C.c_void_p * nsp)(*[b.ptr for b in xs]), 2 * nlm, lamb.ptr, coef.ptr, fmB.ptr,
+        s.handle), "imex_synth_fields")
+    got_x = np.stack([b.download(s).reshape(R, -1) for b in xs])
+    got_fm = fmB.download(s).reshape(nsp, R, fsz)
+
+    xr, ar = buffers("R")
+    fmR = DeviceBuffer(nsp * fsz, "fmR")
+    for r in range(R):
+        off = 4 * r * 2 * nlm
+        _lib.check(L.tsphere_leg_imex_synth(
+            plan.handle, nsp, (C.c_void_p * nsp)(*[b.ptr + off for b in ar]),
+            (C.c_void_p * nsp)(*[b.ptr + off for b in xr]), lamb.ptr,
+            (C.c_float * nsp)(*s1[:, r].tolist()), (C.c_float * nsp)(*s2[:, r].tolist()),
+            fmR.ptr, s.handle), "imex_synth")
+        want_fm = fmR.download(s).reshape(nsp, fsz)
+        assert np.array_equal(got_fm[:, r], want_fm)
+    want_x = np.stack([b.download(s).reshape(R, -1) for b in xr])
+    assert np.all(np.isfinite(got_fm)) and np.array_equal(got_x, want_x)
+    assert not np.array_equal(got_x, x0)
+    plan.destroy()

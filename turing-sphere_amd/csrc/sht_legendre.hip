@@ -229,6 +229,18 @@ __device__ __forceinline__ float imex_step(float x, float s1, float a, float s2,
   return (x + (s1 * a)) / (1.0f + (s2 * lam));
 }
 
+// The implicit step's scalars of field fld: from the device array of a
+// batched step, else from the launch arguments (fld < 4 there).
+__device__ __forceinline__ void imex_coefs(const ImexCoef& ic, int fld, float& s1, float& s2) {
+  if (ic.dev) {
+    s1 = ic.dev[2 * fld];
+    s2 = ic.dev[2 * fld + 1];
+  } else {
+    s1 = ic.s1[fld & 3];
+    s2 = ic.s2[fld & 3];
+  }
+}
+
 // stage_row_t with the implicit step in front: field f's state X is replaced
 // by X_new = (X + s1 A) / (1 + s2 lam) (A: its analysed values), written back
 // to the state, and c_l X_new staged as the synthesis coefficients.
@@ -252,8 +264,10 @@ __device__ __forceinline__ void stage_row_imex(const typename P::Tables& T, int 
         float2* X = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(fld))) + base + i;
         const float2 x = *X;
         const float2 a = reinterpret_cast<const float2*>(Fa.at(fld))[base + i];
-        xn.x = imex_step(x.x, ic.s1[fld], a.x, ic.s2[fld], lm2.x);
-        xn.y = imex_step(x.y, ic.s1[fld], a.y, ic.s2[fld], lm2.y);
+        float c1, c2;
+        imex_coefs(ic, fld, c1, c2);
+        xn.x = imex_step(x.x, c1, a.x, c2, lm2.x);
+        xn.y = imex_step(x.y, c1, a.y, c2, lm2.y);
         *X = xn;
       }
       sq[i * NF + f] = make_float2(xn.x * cz, xn.y * cz);
@@ -2322,8 +2336,10 @@ __device__ __forceinline__ void stage_row_b4(const LegTablesZ64& T, int m, int f
           const float2 a = reinterpret_cast<const float2*>(Fa.at(fld))[base + i];
           const float2 lm2 = reinterpret_cast<const float2*>(lam)[base + i];
           float2 xn;
-          xn.x = imex_step(x.x, ic.s1[fld], a.x, ic.s2[fld], lm2.x);
-          xn.y = imex_step(x.y, ic.s1[fld], a.y, ic.s2[fld], lm2.y);
+          float c1, c2;
+          imex_coefs(ic, fld, c1, c2);
+          xn.x = imex_step(x.x, c1, a.x, c2, lm2.x);
+          xn.y = imex_step(x.y, c1, a.y, c2, lm2.y);
           *X = xn;
           v = make_float2(xn.x * cz, xn.y * cz);
         } else {

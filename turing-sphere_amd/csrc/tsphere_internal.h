@@ -200,6 +200,9 @@ hipError_t launch_leg_analys_mfma(const LegTables64& T, int nfield, const float2
 struct ImexCoef {
   float s1[4], s2[4];  // per analysed field
   int spc[4];          // the state field (species) each analysed field updates
+  // batched implicit step (fold only): (s1, s2) of field f at dev[2 f],
+  // dev[2 f + 1] in device memory, for any field count; null: s1, s2 above
+  const float* dev;
 };
 // The implicit step (s1[f], s2[f] per field; spc unused) of every state
 // field q[f] from its analysed values a[f], in front of the Legendre

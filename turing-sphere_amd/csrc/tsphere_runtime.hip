@@ -975,6 +975,35 @@ int tsphere_leg_analys_fields(tsphere_plan p, int nptr, int per, const float* fm
   return ok();
 }
 
+int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
+                                  const float* const* analysed, float* const* state,
+                                  long long qlm_stride, const float* lam, const float* coef,
+                                  float* fm, tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)analysed, (const void* const*)state))
+    return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  if (!lam || !coef || !fm) return fail(TSPHERE_ERR_INVALID, "NULL argument");
+  if (p->cfg.legendre == TSPHERE_LEG_BF16X3 && p->npair > kB4MaxPairs)
+    return fail(TSPHERE_ERR_INVALID, "the fused implicit step of the split-BF16 form covers up "
+                                     "to %d latitude pairs", kB4MaxPairs);
+  const int nf = nptr * per;
+  const Fields q = grouped((const void* const*)state, nptr, per, qlm_stride);
+  const Fields a = grouped((const void* const*)analysed, nptr, per, qlm_stride);
+  ImexCoef ic{};
+  ic.dev = coef;
+  float2* f2 = reinterpret_cast<float2*>(fm);
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC)
+    TS_HIP(launch_leg_imex_synth_f64r(p->tablesz64(), nf, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
+  else if (p->cfg.legendre == TSPHERE_LEG_BF16X3)
+    TS_HIP(launch_leg_imex_synth_b4(p->tablesz64(), nf, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
+  else
+    TS_HIP(launch_leg_imex_synth(p->tables(), nf, q, a, lam, ic, f2, p->tile[0], S(s)),
+           "leg_imex_synth");
+  return ok();
+}
+
 // ------------------------------------------------------- transposes
 // synth^T: G = 2^{[m>=1]} * FourierAnalysis(g), q = sum_i P_lm(theta_i) G_m(theta_i)
 // analys^T: F = LegendreSynth(q), g_ij = w_i * FourierSynth(F with m>=1 halved)

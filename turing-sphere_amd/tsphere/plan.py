@@ -248,6 +248,7 @@ class ModelPlan:
         self._ptr_cache = {}
         self.runs = 1
         self._prm = None       # ensembles: device copy of the per-run scalars
+        self._coef = None      # ensembles with a folded implicit solve: (s1, s2) per field
         self._fm = None        # grid ops: Fourier scratch between the stages
         self._trig = None      # grid ops: the SHT plan's device twiddle table
 
@@ -274,9 +275,11 @@ class ModelPlan:
         ``spectral`` (single runs, with ``grid``): a step made of one grid op
         and the implicit solve of its species becomes a SpectralOp (the
         analysis, the solve and the next step's synthesis in one kernel).
-        ``fold`` (single runs, with ``grid``, when not ``spectral``): the same
-        step keeps the analysis as its own launch and folds the implicit
-        solve into the next step's synthesis launch."""
+        ``fold`` (with ``grid``, when not ``spectral``; ensembles too): the
+        same step keeps the analysis as its own launch and folds the implicit
+        solve into the next step's synthesis launch -- for an ensemble with
+        each run's s1, s2 in a device array that a parameter change
+        re-uploads."""
         self = ModelPlan()
         self._sht = sht
         self.runs = runs = int(runs)
@@ -387,10 +390,10 @@ class ModelPlan:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
             if grid:
                 ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
-                if (spectral or fold) and runs == 1:
+                mode = "spectral" if spectral and runs == 1 else "fold" if fold else None
+                if mode:
                     ops = self._fuse_spectral(ops, {o.c_name for o in fn.outputs}, param_slots,
-                                              derived, host_of,
-                                              "spectral" if spectral else "fold")
+                                              derived, host_of, mode)
         self._param_slots = param_slots
         self._derived = derived
 
@@ -431,7 +434,23 @@ class ModelPlan:
         self._host = host
         if runs > 1 and host.allocate:
             self._prm = DeviceBuffer(self._params.size, f"mgpu-{label}-params")
+            spec = [op for op in ops if isinstance(op, SpectralOp)]
+            if spec:
+                self._coef = DeviceBuffer(2 * len(spec[0].species) * runs, f"mgpu-{label}-imex")
+                self._coef.upload(self._imex_coefs(spec[0]), None)
         return self
+
+    def _imex_coefs(self, op: "SpectralOp") -> np.ndarray:
+        """An ensemble's folded implicit solve: (s1, s2) of field
+        species * runs + run, each evaluated from that run's scalars as the
+        generated kernel rounds them."""
+        by = {sp: f for f, sp in enumerate(op.spc)}
+        out = np.zeros((len(op.species), self.runs, 2), np.float32)
+        for sp in range(len(op.species)):
+            for r in range(self.runs):
+                out[sp, r, 0] = self._scalar(op.s1[by[sp]], r)
+                out[sp, r, 1] = self._scalar(op.s2[by[sp]], r)
+        return out.reshape(-1)
 
     def _fuse(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c) -> list:
         """Merge each run of consecutive element-wise kernels over the same
@@ -562,18 +581,20 @@ class ModelPlan:
         lbl = f"{G.label} => {E.label}"
         return [SpectralOp("spectral", lbl, G, E, list(G.srcs), spc, s1, s2, lam, mode)]
 
-    def _scalar(self, e) -> np.float32:
+    def _scalar(self, e, run: int | None = None) -> np.float32:
         """A scalar IR expression of params and literals in f32, rounded per
-        operation as the generated kernels compute it."""
+        operation as the generated kernels compute it (an ensemble: with run
+        `run`'s scalars)."""
         if e.kind == "NumLit":
             return np.float32(e.value)
         if e.kind == "Var":
             if e.c_name in self._param_slots:
-                return np.float32(self._params[self._param_slots[e.c_name]])
-            return self._scalar(self._derived[e.c_name][1])
+                row = self._params if run is None else self._params[run]
+                return np.float32(row[self._param_slots[e.c_name]])
+            return self._scalar(self._derived[e.c_name][1], run)
         if e.kind == "Unary" and e.builtin in ("uminus",):
-            return np.float32(-self._scalar(e.operand))
-        a, b = self._scalar(e.left), self._scalar(e.right)
+            return np.float32(-self._scalar(e.operand, run))
+        a, b = self._scalar(e.left, run), self._scalar(e.right, run)
         with np.errstate(all="ignore"):
             if e.builtin == "plus":
                 return np.float32(a + b)
@@ -628,6 +649,9 @@ class ModelPlan:
             self._params[:, i] = np.where(np.isfinite(a), a, 0.0)
         if self._prm is not None:
             self._prm.upload(self._params, stream)
+        if self._coef is not None:
+            spec = [op for op in self.ops if isinstance(op, SpectralOp)]
+            self._coef.upload(self._imex_coefs(spec[0]), stream)
 
     def invalidate(self) -> None:
         for g in self._graphs.values():
@@ -668,10 +692,30 @@ class ModelPlan:
             # the grid op's own synthesis once; then each step's spectral
             # kernel leaves the next step's synthesis in the scratch
             G = spec[0].grid
-            _lib.check(L.tsphere_leg_synth_multi(self._sht.handle, len(G.srcs),
-                                                 self._ptrs(G.srcs), self._fm.ptr, sh), G.label)
+            if ens:
+                rc = L.tsphere_leg_synth_fields(self._sht.handle, len(G.srcs), self.runs,
+                                                self._ptrs(G.srcs), nq, self._fm.ptr, sh)
+            else:
+                rc = L.tsphere_leg_synth_multi(self._sht.handle, len(G.srcs), self._ptrs(G.srcs),
+                                               self._fm.ptr, sh)
+            _lib.check(rc, G.label)
         for _ in range(steps):
             for op in self.ops:
+                if isinstance(op, SpectralOp) and ens:
+                    # fold, every run at once: the analysis of all runs, then
+                    # the implicit solve (scalars per run from the device
+                    # array) in front of the next step's synthesis
+                    self.launch_grid(op.grid, stream)
+                    G, h = op.grid, self._sht.handle
+                    _lib.check(L.tsphere_leg_analys_fields(h, len(G.dsts), self.runs,
+                                                           self._fm.ptr, self._ptrs(G.dsts), nq,
+                                                           sh), G.label)
+                    by = {sp: f for f, sp in enumerate(op.spc)}
+                    ana = [G.dsts[by[sp]] for sp in range(len(op.species))]
+                    _lib.check(L.tsphere_leg_imex_synth_fields(
+                        h, len(op.species), self.runs, self._ptrs(ana), self._ptrs(op.species),
+                        nq, self._bufs[op.lam].ptr, self._coef.ptr, self._fm.ptr, sh), op.label)
+                    continue
                 if isinstance(op, SpectralOp):
                     self.launch_grid(op.grid, stream)
                     n = len(op.species)
@@ -858,6 +902,9 @@ class ModelPlan:
         if self._prm is not None:
             self._prm.free()
             self._prm = None
+        if self._coef is not None:
+            self._coef.free()
+            self._coef = None
         if self._fm is not None:
             self._fm.free()
             self._fm = None
