@@ -322,8 +322,10 @@ int tsphere_accumulate(float *dst, const float *src, long long n, tsphere_stream
  *               and 3 (2: the pair (m, mmax - m), balanced; 1: one row, twice
  *               the workgroups).  impl 5 = per-parity matrix products on
  *               16x16x4 tiles whose columns are (field, re/im) of 8 fields,
- *               for batched transforms: nf 8 or 16, k 1, rows 1, a wave per
- *               64 latitude pairs (up to 512 pairs; beyond, impl 3).
+ *               for batched transforms: nf 8 or 16, k 1, rows 1 (the
+ *               8-field synthesis also 2: consecutive rows m, m + 1 per
+ *               workgroup), a wave per 64 latitude pairs (up to 512 pairs;
+ *               beyond, impl 3).
  *   Split-BF16 form:  impl 4 = 16 4x4x4 blocks per wave
  *               (v_mfma_f32_4x4x4_16b_bf16: two fields' re/im fill the 4
  *               columns; the built-in choice up to 512 latitude pairs) with

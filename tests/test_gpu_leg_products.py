@@ -86,3 +86,36 @@ def test_parity_products_above_512_pairs_take_the_block_kernels(gpu):
     plan.set_tile("synth", impl=5)
     assert plan.get_tile("synth", 4)["impl"] == 3
     plan.destroy()
+
+
+@pytest.mark.parametrize("lmax,nlat,nphi,nf", [(15, 17, 32, 11), (12, 13, 26, 9),
+                                                (63, 64, 128, 8), (101, 102, 204, 19)])
+def test_two_row_synthesis(gpu, lmax, nlat, nphi, nf):
+    """The 8-field synthesis with two consecutive m-rows per workgroup (lane
+    pairs write (re, im) of m and m + 1 as 16 contiguous bytes) equals the
+    one-row form and the VALU form bit for bit: every row's sums run over l
+    in the same order.  An odd row count (lmax 12) leaves the last workgroup
+    one row; partial groups of 8 fields leave idle columns."""
+    import tsphere
+    from tsphere import _lib
+
+    emit(f"two-row synthesis lmax{lmax} {nlat}x{nphi} fields {nf}")
+    L = _lib.load()
+    plan, qs, _ = _plan_and_fields(lmax, nlat, nphi, nf, "fp32", 70)
+    s = plan.stream
+    q = tsphere.DeviceBuffer(nf * 2 * plan.nlm)
+    q.upload(qs, s)
+    fm = tsphere.DeviceBuffer(nf * 2 * (lmax + 1) * nlat)
+    outs = []
+    for tile in (dict(impl=5, nf=8, rows=2), dict(impl=5, nf=8, rows=1), dict(impl=1)):
+        plan.set_tile("synth", **tile)
+        fm.fill(np.nan, s)
+        _lib.check(L.tsphere_leg_synth(plan.handle, nf, q.ptr, 2 * plan.nlm, fm.ptr, s.handle))
+        outs.append(fm.download(s))
+    assert np.all(np.isfinite(outs[0]))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    with pytest.raises(Exception, match="tile"):
+        plan.set_tile("analys", impl=5, nf=8, rows=2)
+    with pytest.raises(Exception, match="tile"):
+        plan.set_tile("synth", impl=5, nf=16, rows=2)
+    plan.destroy()

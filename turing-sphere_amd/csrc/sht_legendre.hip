@@ -1973,9 +1973,14 @@ __device__ __forceinline__ void g16_recur(ChainT<P>& c, int ls, int lz, int m,
   }
 }
 
-template <int CB, class P>
+// ROWS = 2 (8 fields): the workgroup takes the consecutive rows m0 = 2 q and
+// m0 + 1 one after the other, keeps both rows' sums, and neighbouring lanes
+// (re, im of one field) swap values so that each writes one (re, im) pair:
+// a lane pair writes 16 contiguous bytes of (field, latitude), m0 and m0 + 1.
+template <int CB, int ROWS, class P>
 __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int nfield, Fields Fq,
                                                        float2* __restrict__ fm) {
+  static_assert(ROWS == 1 || (ROWS == 2 && CB == 1), "two rows with 8 fields per workgroup");
   constexpr int NF = 8 * CB, SP = G16_SSPAN, QROW = g16_qrow(CB), TROW = G16_STROW;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ int smin[8];
@@ -1987,7 +1992,23 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
   float* const sq = reinterpret_cast<float*>(sal + analys_ab_len(L));  // [2][SP / 2][QROW]
   float* const tile = sq + 2 * (SP / 2) * QROW + wv * ((SP / 2) * TROW);
   const int f0 = blockIdx.y * NF;
-  const int m = row_m(0, 1, M, false);
+  const int m0 = ROWS * row_m(0, 1, M, false);
+
+  f4v acc[ROWS][2][4][CB];  // [row][parity][16-pair block][column block]
+#pragma unroll
+  for (int rr = 0; rr < ROWS; ++rr)
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr)
+#pragma unroll
+      for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) acc[rr][pr][pb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int rr = 0; rr < ROWS; ++rr) {
+  const int m = m0 + rr;
+  if (m > M) break;
+  if (rr > 0) __syncthreads();  // the previous row's readers of alpha, c_l Q_l and tiles are done
   const long long base = lm_base(L, m);
   g16_stage_alpha<P>(T, m, sal);
 
@@ -2023,14 +2044,6 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
     }
   };
 
-  f4v acc[2][4][CB];  // [parity][16-pair block][column block]
-#pragma unroll
-  for (int pr = 0; pr < 2; ++pr)
-#pragma unroll
-    for (int pb = 0; pb < 4; ++pb)
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb) acc[pr][pb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
-
   if (lblk <= L) stage(lblk, sq);
   int buf = 0;
   for (int ls = lblk; ls <= L; ls += SP) {
@@ -2053,8 +2066,8 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
               *reinterpret_cast<const float2*>(tile + (4 * ks + g) * TROW + 32 * pb + 2 * r16);
 #pragma unroll
           for (int cb = 0; cb < CB; ++cb) {
-            acc[0][pb][cb] = mf4(a.x, b[cb].x, acc[0][pb][cb]);
-            acc[1][pb][cb] = mf4(a.y, b[cb].y, acc[1][pb][cb]);
+            acc[rr][0][pb][cb] = mf4(a.x, b[cb].x, acc[rr][0][pb][cb]);
+            acc[rr][1][pb][cb] = mf4(a.y, b[cb].y, acc[rr][1][pb][cb]);
           }
         }
       }
@@ -2062,24 +2075,47 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
     }
     buf ^= 1;
   }
+  }
 
   // north = E + O, south = E - O; D row 4 g + r is pair 64 w + 16 pb + 4 g + r
+  if constexpr (ROWS == 1) {
 #pragma unroll
-  for (int pb = 0; pb < 4; ++pb)
+    for (int pb = 0; pb < 4; ++pb)
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      const int f = f0 + 8 * cb + (r16 >> 1);
-      if (f >= nfield) continue;
-      float* out = reinterpret_cast<float*>(fm + (long long)f * nlat * (M + 1) + m) + (r16 & 1);
+      for (int cb = 0; cb < CB; ++cb) {
+        const int f = f0 + 8 * cb + (r16 >> 1);
+        if (f >= nfield) continue;
+        float* out = reinterpret_cast<float*>(fm + (long long)f * nlat * (M + 1) + m0) + (r16 & 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int jp = 64 * wv + 16 * pb + 4 * g + r;
+          if (jp >= T.npair) continue;
+          const float e = acc[0][0][pb][cb][r], o = acc[0][1][pb][cb][r];
+          out[2LL * jp * (M + 1)] = e + o;
+          out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+        }
+      }
+  } else {
+    // lane 2 i holds re, lane 2 i + 1 im of one field for both rows: the
+    // even lane takes (re, im) of row m0, the odd lane of row m0 + 1
+    const int comp = r16 & 1;
+    const int f = f0 + (r16 >> 1);
+    float2* out = fm + (long long)min(f, nfield - 1) * nlat * (M + 1) + m0 + comp;
+    const bool live = f < nfield && (comp == 0 || m0 + 1 <= M);
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int jp = 64 * wv + 16 * pb + 4 * g + r;
-        if (jp >= T.npair) continue;
-        const float e = acc[0][pb][cb][r], o = acc[1][pb][cb][r];
-        out[2LL * jp * (M + 1)] = e + o;
-        out[2LL * (nlat - 1 - jp) * (M + 1)] = e - o;
+        const float e0 = acc[0][0][pb][0][r], o0 = acc[0][1][pb][0][r];
+        const float e1 = acc[1][0][pb][0][r], o1 = acc[1][1][pb][0][r];
+        const float nA = e0 + o0, sA = e0 - o0, nB = e1 + o1, sB = e1 - o1;
+        const float gn = __shfl_xor(comp ? nA : nB, 1), gs = __shfl_xor(comp ? sA : sB, 1);
+        if (!live || jp >= T.npair) continue;
+        out[(long long)jp * (M + 1)] = comp ? make_float2(gn, nB) : make_float2(nA, gn);
+        out[(long long)(nlat - 1 - jp) * (M + 1)] = comp ? make_float2(gs, sB) : make_float2(sA, gs);
       }
-    }
+  }
 }
 
 template <int CB, class P>
@@ -2214,15 +2250,16 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
   }
 }
 
-template <int CB, class P>
+template <int CB, int ROWS, class P>
 hipError_t synth_g16(const typename P::Tables& T, int nfield, int nwaves, const Fields& q,
                      float2* fm, hipStream_t s) {
   if (nwaves < 1 || nwaves > 8 || 64 * nwaves < T.npair) return hipErrorInvalidValue;
-  dim3 grid(T.mmax + 1, (nfield + 8 * CB - 1) / (8 * CB));
+  dim3 grid((T.mmax + ROWS) / ROWS, (nfield + 8 * CB - 1) / (8 * CB));
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (2 * (G16_SSPAN / 2) * g16_qrow(CB) +
                                       (size_t)nwaves * (G16_SSPAN / 2) * G16_STROW);
-  hipLaunchKernelGGL((k_leg_synth_g16<CB, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, q, fm);
+  hipLaunchKernelGGL((k_leg_synth_g16<CB, ROWS, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, q,
+                     fm);
   return hipGetLastError();
 }
 
@@ -2240,13 +2277,16 @@ hipError_t analys_g16(const typename P::Tables& T, int nfield, int nwaves, const
 template <class P>
 hipError_t synth_g16_nf(const typename P::Tables& T, int nfield, const LegTile& t,
                         const Fields& q, float2* fm, hipStream_t s) {
-  return t.nf == 8 ? synth_g16<1, P>(T, nfield, t.nwaves, q, fm, s)
-                   : synth_g16<2, P>(T, nfield, t.nwaves, q, fm, s);
+  if (t.nf == 8)
+    return t.rows == 2 ? synth_g16<1, 2, P>(T, nfield, t.nwaves, q, fm, s)
+                       : synth_g16<1, 1, P>(T, nfield, t.nwaves, q, fm, s);
+  return synth_g16<2, 1, P>(T, nfield, t.nwaves, q, fm, s);
 }
 
 template <class P>
 hipError_t analys_g16_nf(const typename P::Tables& T, int nfield, const LegTile& t,
                          const float2* fm, const Fields& q, hipStream_t s) {
+  if (t.rows != 1) return hipErrorInvalidValue;  // one row per analysis workgroup
   return t.nf == 8 ? analys_g16<1, P>(T, nfield, t.nwaves, fm, q, s)
                    : analys_g16<2, P>(T, nfield, t.nwaves, fm, q, s);
 }
@@ -2944,9 +2984,9 @@ LegTile resolve_leg_tile(int legendre, int stage, int npair, int nfield, LegTile
 namespace {
 
 bool fp32_tile_ok(const LegTile& t, int npair) {
-  if (t.impl == 5)
-    return (t.nf == 8 || t.nf == 16) && t.k == 1 && t.rows == 1 && t.nwaves >= 1 &&
-           t.nwaves <= 8 && 64 * t.nwaves >= npair;
+  if (t.impl == 5)  // two rows per workgroup: the 8-field synthesis
+    return (t.nf == 8 || t.nf == 16) && t.k == 1 && (t.rows == 1 || (t.rows == 2 && t.nf == 8)) &&
+           t.nwaves >= 1 && t.nwaves <= 8 && 64 * t.nwaves >= npair;
   const bool kok = t.impl >= 2 ? (t.k == 1 || t.k == 2 || t.k == 4) : round_k(t.k) == t.k;
   const int wmax = t.impl == 3 ? 8 : 4;
   // eight fields per workgroup: block-matrix kernels with one or two chains

@@ -83,7 +83,8 @@ struct LegTablesZ64 {
 //  cover the pairs).
 //  rows (VALU kernels): m-rows per workgroup, 2 = the pair (m, M - m), 1 = one.
 //  impl 5 = per-parity matrix products (16x16x4, columns (field, re/im) of 8
-//  fields): nf 8 or 16, k 1, rows 1, nwaves * 64 >= pairs (<= 512 pairs).
+//  fields): nf 8 or 16, k 1, rows 1 (the 8-field synthesis also 2: the
+//  consecutive rows 2q, 2q + 1), nwaves * 64 >= pairs (<= 512 pairs).
 //  Split-BF16 form: impl 4 = 4x4x4 matrix blocks (nf, nwaves, rows as the
 //  FP32 block kernels, k = 1), else ctc column tiles (16 columns each) of the
 //  16x16x32 kernels per workgroup.

@@ -1093,12 +1093,13 @@ int tsphere_plan_set_tile(tsphere_plan p, int stage, const tsphere_leg_tile* til
       if (!(r.ctc == 1 || r.ctc == 2 || (stage == 0 && r.ctc == 4)))
         return fail(TSPHERE_ERR_INVALID, "MFMA tile: ctc must be 1, 2 (or 4 for synthesis)");
     } else if (r.impl == 5) {
-      if (!(r.nf == 8 || r.nf == 16) || r.k != 1 || r.rows != 1 || r.nwaves < 1 ||
+      const bool rows_ok = r.rows == 1 || (r.rows == 2 && r.nf == 8 && stage == 0);
+      if (!(r.nf == 8 || r.nf == 16) || r.k != 1 || !rows_ok || r.nwaves < 1 ||
           r.nwaves > 8 || 64LL * r.nwaves < p->npair)
         return fail(TSPHERE_ERR_INVALID,
                     "per-parity product tile nf=%d nwaves=%d k=%d rows=%d: need nf 8 or 16, k 1, "
-                    "rows 1, 64*nwaves >= %d latitude pairs (<= 8 waves)",
-                    r.nf, r.nwaves, r.k, r.rows, p->npair);
+                    "rows 1 (2 for the 8-field synthesis), 64*nwaves >= %d latitude pairs "
+                    "(<= 8 waves)", r.nf, r.nwaves, r.k, r.rows, p->npair);
     } else {
       const bool kok = r.impl >= 2 ? (r.k == 1 || r.k == 2 || r.k == 4)
                                    : (r.k == 1 || r.k == 2 || r.k == 4 || r.k == 8);
@@ -1158,7 +1159,11 @@ static std::vector<LegTile> tile_candidates(const Plan* p, int stage, int nfield
   // per-parity matrix products: 8 or 16 fields per workgroup, a wave per 64 pairs
   if (p->npair <= 64 * 8)
     for (int nf : {8, 16})
-      if (nf <= nfield) c.push_back(LegTile{nf, (p->npair + 63) / 64, 1, 0, 5, 1});
+      if (nf <= nfield) {
+        c.push_back(LegTile{nf, (p->npair + 63) / 64, 1, 0, 5, 1});
+        // two consecutive rows per workgroup: the 8-field synthesis
+        if (nf == 8 && stage == 0) c.push_back(LegTile{8, (p->npair + 63) / 64, 1, 0, 5, 2});
+      }
   return c;
 }
 

@@ -118,7 +118,8 @@ class ShtPlan:
         2 = FP32 matrix cores as 16x16x4 tiles, 3 = FP32 matrix cores as
         16 4x4 blocks per wave, 5 = FP32 matrix cores as per-parity
         products whose 16 columns are (field, re/im) of 8 fields (batched
-        transforms: nf 8 or 16, k 1, rows 1, up to 512 latitude pairs);
+        transforms: nf 8 or 16, k 1, rows 1 -- 2 for the 8-field synthesis,
+        two consecutive m-rows per workgroup -- up to 512 latitude pairs);
         nf = fields per workgroup (1, 2, 4, and 8 for impl 3 with k 1 or 2).
         Shapes the library does not have raise."""
         t = _lib.LegTileC(nf, nwaves, k, ctc, impl, rows)
