@@ -2122,7 +2122,8 @@ template <int CB, class P>
 __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, int nfield,
                                                         const float2* __restrict__ fm, Fields Fq) {
   constexpr int NF = 8 * CB, SP = G16_ASPAN, TROW = G16_ATROW;
-  constexpr int NO = 2 * CB * 256;  // a wave's sums of one span
+  constexpr int RS = (SP / 2) * TROW;  // floats of a wave's tile
+  static_assert(2 * CB * 256 <= RS, "a wave's sums of a span fit its tile");
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ int smin[8];
   const int L = T.lmax, M = T.mmax, nlat = T.nlat;
@@ -2131,11 +2132,12 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
   const int r16 = lane & 15, g = lane >> 4;
   auto* sal = reinterpret_cast<typename P::ab_t*>(lds_raw);
   float* const tiles = reinterpret_cast<float*>(sal + analys_ab_len(L));
-  float* const tile = tiles + wv * ((SP / 2) * TROW);
-  // the waves' sums of a span, [nwaves][NO]: one buffer, so that three
-  // 4-wave workgroups of 16 fields fit a CU (two buffers and one barrier per
-  // span measured 232 against 202 us for 64 fields at lmax 511)
-  float* const red = tiles + nwaves * ((SP / 2) * TROW);
+  float* const tile = tiles + wv * RS;
+  // a wave's sums of a span go to its own tile once its products have read
+  // it, and the workgroup combines them from the tiles: no LDS of their own,
+  // so that four 4-wave workgroups of 16 fields fit a CU (a separate buffer:
+  // three per CU, 185 us for 64 fields at lmax 511; two buffers, one barrier
+  // per span: two per CU, 232 us)
   const int f0 = blockIdx.y * NF;
   const int m = row_m(0, 1, M, false);
   const long long base = lm_base(L, m);
@@ -2146,8 +2148,24 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
   const int lk = wave_min(c.ls), lz = last_seed(c.ls, L);
   if (lane == 0) smin[wv] = lk;
 
+  __syncthreads();  // smin and the staged alpha
+  int lblk = smin[0];
+  for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
+  lblk = min(lblk, L + 1);
+
+  // coefficients below every lane's significance point are exactly 0
+  const int nz = lblk - m;
+  for (int o = tid; o < NF * nz; o += nthr) {
+    const int f = o / nz, i = o - f * nz;
+    if (f0 + f < nfield)
+      reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + i] =
+          make_float2(0.f, 0.f);
+  }
+
   // B operands of the whole row: W_par of pair 64 w + 4 kp + g, column
-  // 16 cb + r16 (W_E = (G_N + G_S) w, W_O = (G_N - G_S) w, leg.ts:207-219)
+  // 16 cb + r16 (W_E = (G_N + G_S) w, W_O = (G_N - G_S) w, leg.ts:207-219);
+  // loaded after the row's barrier, so that their latency overlaps the first
+  // span's recurrence instead of holding the barrier
   float W[2][16][CB];
 #pragma unroll
   for (int kp = 0; kp < 16; ++kp)
@@ -2172,19 +2190,6 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
       W[0][kp][cb] = e;
       W[1][kp][cb] = o;
     }
-  __syncthreads();  // smin and the staged alpha
-  int lblk = smin[0];
-  for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
-  lblk = min(lblk, L + 1);
-
-  // coefficients below every lane's significance point are exactly 0
-  const int nz = lblk - m;
-  for (int o = tid; o < NF * nz; o += nthr) {
-    const int f = o / nz, i = o - f * nz;
-    if (f0 + f < nfield)
-      reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + i] =
-          make_float2(0.f, 0.f);
-  }
 
   for (int ls = lblk; ls <= L; ls += SP) {
     f4v acc[2][CB];
@@ -2208,11 +2213,11 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
       wave_sync_lds();  // the tile is rewritten in the next span
     }
     // this wave's sums of the span: D row 4 g + r is l = ls + 2 (4 g + r) +
-    // parity.  red holds, per wave, [cb][row][field slot][re/im][parity]:
+    // parity.  The wave's tile then holds [cb][row][field slot][re/im][parity]:
     // one field's four values of a row (l, l + 1; re, im) are 16 contiguous
     // bytes, its slot the field's index XOR the row's low bits, so that the
     // combine's 16-byte reads of consecutive rows fall in distinct banks
-    float* rw = red + wv * NO;
+    float* rw = tile;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
@@ -2230,9 +2235,9 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
       const int l = ls + 2 * row;
       if (l > L || f0 + f >= nfield) continue;
       const int idx = (((f >> 3) * 16 + row) * 8 + ((f & 7) ^ (row & 7))) * 4;
-      float4 v = *reinterpret_cast<const float4*>(red + idx);
+      float4 v = *reinterpret_cast<const float4*>(tiles + idx);
       for (int w = 1; w < nwaves; ++w) {
-        const float4 u = *reinterpret_cast<const float4*>(red + w * NO + idx);
+        const float4 u = *reinterpret_cast<const float4*>(tiles + w * RS + idx);
         v.x += u.x;
         v.y += u.y;
         v.z += u.z;
@@ -2246,7 +2251,7 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
         dst[1] = make_float2(v.y * c1, v.w * c1);
       }
     }
-    __syncthreads();  // red is rewritten in the next span
+    __syncthreads();  // the tiles are rewritten in the next span
   }
 }
 
@@ -2269,7 +2274,7 @@ hipError_t analys_g16(const typename P::Tables& T, int nfield, int nwaves, const
   if (nwaves < 1 || nwaves > 8 || 64 * nwaves < T.npair) return hipErrorInvalidValue;
   dim3 grid(T.mmax + 1, (nfield + 8 * CB - 1) / (8 * CB));
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
-                     sizeof(float) * (size_t)nwaves * ((G16_ASPAN / 2) * G16_ATROW + 2 * CB * 256);
+                     sizeof(float) * (size_t)nwaves * (G16_ASPAN / 2) * G16_ATROW;
   hipLaunchKernelGGL((k_leg_analys_g16<CB, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm, q);
   return hipGetLastError();
 }
