@@ -2191,6 +2191,10 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
       W[1][kp][cb] = o;
     }
 
+  const int cf = tid >> 4, crow = tid & 15;  // this thread's first combine item
+  const int cidx = (((cf >> 3) * 16 + crow) * 8 + ((cf & 7) ^ (crow & 7))) * 4;
+  float2* const cq =
+      reinterpret_cast<float2*>(const_cast<char*>(Fq.at(min(f0 + cf, nfield - 1)))) + base;
   for (int ls = lblk; ls <= L; ls += SP) {
     f4v acc[2][CB];
 #pragma unroll
@@ -2230,11 +2234,14 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
     __syncthreads();
     // the waves' sums in wave order: thread (field, row) takes l = ls + 2 row
     // and l + 1, re and im; consecutive threads on consecutive l of a field
+    // (a thread's first item is the same in every span: its field's row
+    // pointer and sum index are formed before the span loop)
     for (int o = tid; o < NF * 16; o += nthr) {
-      const int f = o >> 4, row = o & 15;
+      const bool first = o == tid;
+      const int f = first ? cf : o >> 4, row = first ? crow : o & 15;
       const int l = ls + 2 * row;
       if (l > L || f0 + f >= nfield) continue;
-      const int idx = (((f >> 3) * 16 + row) * 8 + ((f & 7) ^ (row & 7))) * 4;
+      const int idx = first ? cidx : (((f >> 3) * 16 + row) * 8 + ((f & 7) ^ (row & 7))) * 4;
       float4 v = *reinterpret_cast<const float4*>(tiles + idx);
       for (int w = 1; w < nwaves; ++w) {
         const float4 u = *reinterpret_cast<const float4*>(tiles + w * RS + idx);
@@ -2243,11 +2250,14 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
         v.z += u.z;
         v.w += u.w;
       }
-      float2* dst = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f))) + base + (l - m);
-      const float c0 = T.cz[base + (l - m)];
+      float2* const q =
+          first ? cq : reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f))) + base;
+      float2* dst = q + (l - m);
+      const float* cz = T.cz + base + (l - m);
+      const float c0 = cz[0];
       dst[0] = make_float2(v.x * c0, v.z * c0);  // (re, im) of l: parity 0
       if (l + 1 <= L) {
-        const float c1 = T.cz[base + (l - m) + 1];
+        const float c1 = cz[1];
         dst[1] = make_float2(v.y * c1, v.w * c1);
       }
     }
