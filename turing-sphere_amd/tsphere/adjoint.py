@@ -281,13 +281,20 @@ class AdjointModel:
             self.out_of[s] = fn.outputs[i].c_name
         in_c = {p.name: p.c_name for p in fn.params}
         self.in_of = {s: in_c[s] for s in self.state}
-        # cotangent buffers for every tensor value except host constants
+        # cotangent buffers for every tensor value except host constants, in
+        # one arena: zeroed by one fill, with the state outputs' and the state
+        # inputs' cotangents each contiguous in state order, so that the
+        # carried cotangent moves in and out with one copy each
         const_c = {in_c[n] for n in ("lam", "noise") if n in in_c}
+        names = [c for c in tape._bufs if c not in const_c]
+        first = [self.out_of[s] for s in self.state] + [self.in_of[s] for s in self.state]
+        order = [c for c in first if c in names] + [c for c in names if c not in first]
+        self._bar_arena = DeviceBuffer(max(1, sum(tape._bufs[c].count for c in order)), "bars")
         self._bar = {}
-        for c, b in tape._bufs.items():
-            if c in const_c:
-                continue
-            self._bar[c] = DeviceBuffer(b.count, f"bar-{c}")
+        off = 0
+        for c in order:
+            self._bar[c] = DeviceBuffer.view(self._bar_arena, off, tape._bufs[c].count, f"bar-{c}")
+            off += tape._bufs[c].count
         self._tmp = {}
         # adjoint kernels, in reverse op order
         self._adj = []
@@ -357,22 +364,37 @@ class AdjointModel:
     def _state_buf(self, s):
         return self._host.get(s)
 
+    def _contiguous(self, bufs) -> bool:
+        """Whether the per-species buffers follow one another in memory."""
+        n = 2 * self.nlm
+        return all(b.ptr == bufs[0].ptr + 4 * i * n and b.count == n for i, b in enumerate(bufs))
+
+    def _copy_species(self, dsts, srcs) -> None:
+        """Copy one state vector's species: one copy when both sides are
+        contiguous (the state arena, the cotangent arena, the carry)."""
+        L = _lib.load()
+        n = 2 * self.nlm
+        if self._contiguous(dsts) and self._contiguous(srcs):
+            _lib.check(L.tsphere_memcpy_dtod(dsts[0].ptr, srcs[0].ptr, 4 * n * len(dsts),
+                                             self.stream.handle), "copy state")
+            return
+        for d, s_ in zip(dsts, srcs):
+            _lib.check(L.tsphere_memcpy_dtod(d.ptr, s_.ptr, 4 * n, self.stream.handle),
+                       "copy state")
+
+    def _at(self, buf: DeviceBuffer, off: int):
+        """The species slices of one state vector in `buf` at float offset off."""
+        n = 2 * self.nlm
+        return [DeviceBuffer.view(buf, off + i * n, n) for i in range(len(self.state))]
+
     def _load_state(self, src: DeviceBuffer, off: int = 0) -> None:
         """Copy one state vector (all species) from `src` at float offset off."""
-        L = _lib.load()
-        n = 2 * self.nlm
-        for i, s in enumerate(self.state):
-            b = self._state_buf(s)
-            _lib.check(L.tsphere_memcpy_dtod(b.ptr, _off(src, off + i * n), 4 * n,
-                                             self.stream.handle), "load state")
+        self._copy_species([self._state_buf(s) for s in self.state], self._at(src, off))
 
     def _store_state(self, dst: DeviceBuffer, off: int, from_outputs: bool) -> None:
-        L = _lib.load()
-        n = 2 * self.nlm
-        for i, s in enumerate(self.state):
-            b = self.tape._bufs[self.out_of[s]] if from_outputs else self._state_buf(s)
-            _lib.check(L.tsphere_memcpy_dtod(_off(dst, off + i * n), b.ptr, 4 * n,
-                                             self.stream.handle), "store state")
+        srcs = [self.tape._bufs[self.out_of[s]] if from_outputs else self._state_buf(s)
+                for s in self.state]
+        self._copy_species(self._at(dst, off), srcs)
 
     def _step_once(self) -> None:
         self.tape.encode_steps(self.stream, 1)
@@ -387,8 +409,7 @@ class AdjointModel:
 
     # ------------------------------------------------------------ backward
     def _zero_bars(self):
-        for b in self._bar.values():
-            b.fill(0.0, self.stream)
+        self._bar_arena.fill(0.0, self.stream)
 
     def _backward_step(self) -> None:
         """With the step's intermediates on the device and the cotangents of
@@ -450,8 +471,8 @@ class AdjointModel:
             for a in self._adj:
                 if a[0] == "kernel":
                     a[4].fill(0.0, self.stream)
-            L = _lib.load()
             nq = 2 * self.nlm
+
             def keep_and_advance(j):
                 self._store_state(seg, j * n1, from_outputs=False)
                 self.fwd.encode_steps(self.stream, 1)
@@ -460,24 +481,26 @@ class AdjointModel:
                 self._load_state(seg, j * n1)
                 self._step_once()          # regenerate this step's intermediates
                 self._zero_bars()
-                for i, s in enumerate(self.state):
-                    _lib.check(L.tsphere_memcpy_dtod(
-                        self._bar[self.out_of[s]].ptr, _off(carry, i * nq), 4 * nq,
-                        self.stream.handle), "cotangent in")
+                self._copy_species([self._bar[self.out_of[s]] for s in self.state],
+                                   self._at(carry, 0))      # cotangent in
                 self._backward_step()
-                for i, s in enumerate(self.state):
-                    _lib.check(L.tsphere_memcpy_dtod(
-                        _off(carry, i * nq), self._bar[self.in_of[s]].ptr, 4 * nq,
-                        self.stream.handle), "cotangent out")
+                self._copy_species(self._at(carry, 0),
+                                   [self._bar[self.in_of[s]] for s in self.state])  # out
+
+            def segment(n):
+                # from the segment's checkpoint (already in the state): keep
+                # each step's input while advancing, then differentiate the
+                # steps in reverse
+                for j in range(n - 1):
+                    keep_and_advance(j)
+                self._store_state(seg, (n - 1) * n1, from_outputs=False)
+                for j in reversed(range(n)):
+                    differentiate(j)
 
             for g in reversed(range(nseg)):
                 n = min(steps, g * C_ + C_) - g * C_
                 self._load_state(chk, g * n1)
-                for j in range(n - 1):
-                    self._replay(("keep", C_, j), lambda j=j: keep_and_advance(j))
-                self._store_state(seg, (n - 1) * n1, from_outputs=False)
-                for j in reversed(range(n)):
-                    self._replay(("diff", C_, j), lambda j=j: differentiate(j))
+                self._replay(("segment", C_, n), lambda n=n: segment(n))
             out = carry.download(self.stream)
             grads = {s: out[i * nq:(i + 1) * nq].copy() for i, s in enumerate(self.state)}
             pg = np.zeros(len(self.param_slots))
@@ -518,15 +541,13 @@ class AdjointModel:
         if self._module:
             _lib.call("tsphere_module_destroy", self._module)
             self._module = None
-        for b in list(self._bar.values()) + list(self._tmp.values()) + self._owned:
+        for b in list(self._tmp.values()) + self._owned:
             b.free()
+        self._bar_arena.free()
         self._bar, self._tmp, self._owned = {}, {}, []
         self.tape.destroy()
         self._host.destroy()
 
-
-def _off(buf: DeviceBuffer, floats: int):
-    return C.c_void_p(buf.ptr + 4 * floats)
 
 
 def _dedupe_helpers(srcs):
