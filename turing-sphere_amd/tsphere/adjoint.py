@@ -39,7 +39,7 @@ from .ir import is_numeric, is_tensor
 from .layout import ShtConfig
 from .mfront import CompiledModel, model_bindings
 from .model import eigenvalues
-from .plan import HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
+from .plan import FusedOp, HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
 
 WG = 256
 EPT = 4  # elements per thread (tsphere_kernel_launch launches count / 4 threads)
@@ -265,8 +265,11 @@ class AdjointModel:
         host.ensure_group(self.state, 2 * nlm)
         host.ensure("lam", 2 * nlm)
         self._host = host
+        # the step tape: every value stored for the reverse pass, consecutive
+        # element-wise statements in one kernel each (fewer graph nodes per
+        # regenerated step), the Fourier stages separate
         tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape",
-                                fuse=False)
+                                grid=False, keep_all=True)
         self.tape = tape
         # state <- step(state) only: the fused plan, writing the state in place
         self.fwd = ModelPlan.create(sht, PlanSpec(fn, list(self.state) +
@@ -301,7 +304,10 @@ class AdjointModel:
         srcs = []
         self.param_slots = list(tape.param_names)
         npar = len(self.param_slots)
-        for idx, op in enumerate(reversed(tape.ops)):
+        rev = []
+        for op in reversed(tape.ops):  # statements of a fused kernel: their own adjoints
+            rev.extend(reversed(op.members) if isinstance(op, FusedOp) else [op])
+        for idx, op in enumerate(rev):
             if isinstance(op, KernelOp):
                 gi = [c for c in op.ins if c in self._bar]
                 name = f"adj{idx}"

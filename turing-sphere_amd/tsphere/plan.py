@@ -257,7 +257,7 @@ class ModelPlan:
     def create(sht, spec: PlanSpec, host: HostBuffers, batch: bool = True,
                label: str = "step", runs: int = 1, shared=(), fuse: bool = True,
                grid: bool = True, spectral: bool = False,
-               fold: bool = False) -> "ModelPlan":
+               fold: bool = False, keep_all: bool = False) -> "ModelPlan":
         """ModelPlan.create (plan.ts:206-407).
 
         ``runs`` > 1 compiles an ensemble: every value gets `runs` copies back
@@ -275,6 +275,9 @@ class ModelPlan:
         ``spectral`` (single runs, with ``grid``): a step made of one grid op
         and the implicit solve of its species becomes a SpectralOp (the
         analysis, the solve and the next step's synthesis in one kernel).
+        ``keep_all`` (with ``fuse``): a fused element-wise kernel writes every
+        value it computes to its buffer, not only those read after it (a
+        step tape whose values a reverse pass reads).
         ``fold`` (with ``grid``, when not ``spectral``; ensembles too): the
         same step keeps the analysis as its own launch and folds the implicit
         solve into the next step's synthesis launch -- for an ensemble with
@@ -387,7 +390,7 @@ class ModelPlan:
             ops = _schedule(ops, max_batch=max(1, min(8, cap)))
         ops = self._fold_feedback(ops, host, host_of)
         if batch and fuse:
-            ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
+            ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c, keep_all)
             if grid:
                 ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
                 mode = "spectral" if spectral and runs == 1 else "fold" if fold else None
@@ -452,7 +455,8 @@ class ModelPlan:
                 out[sp, r, 1] = self._scalar(op.s2[by[sp]], r)
         return out.reshape(-1)
 
-    def _fuse(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c) -> list:
+    def _fuse(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c,
+              keep_all: bool = False) -> list:
         """Merge each run of consecutive element-wise kernels over the same
         element count into one kernel; values used only inside a run stay in
         registers."""
@@ -474,7 +478,7 @@ class ModelPlan:
             for o in ops[:i] + ops[j:]:
                 later |= _reads(o)
             stores = [g.out for g in group
-                      if g.out in outputs or g.out in later or
+                      if keep_all or g.out in outputs or g.out in later or
                       id(self._bufs.get(g.out)) in host_bufs]
             if not stores:
                 stores = [group[-1].out]
