@@ -272,9 +272,11 @@ class AdjointModel:
                                 grid=False, keep_all=True)
         self.tape = tape
         # state <- step(state) only: the fused plan, writing the state in place
+        # (the spectral op where the step has its form, else the fold: the
+        # same states as the separate launches, bit for bit)
         self.fwd = ModelPlan.create(sht, PlanSpec(fn, list(self.state) +
                                                   [None] * (len(fn.outputs) - len(self.state))),
-                                    host, True, "forward")
+                                    host, True, "forward", spectral=True, fold=True)
         self._graphs = {}
         self._seg = self._carry = None
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
