@@ -323,9 +323,19 @@ class AdjointModel:
                 for s_ in op.srcs:
                     if s_ not in self._tmp:
                         self._tmp[s_] = DeviceBuffer(tape._bufs[s_].count, f"tmp-{s_}")
-                self._adj.append((op.kind, op))
+                self._adj.append((op.kind, op, set()))
             else:
                 raise UnsupportedOnGpu("unexpected op in the step tape")
+        # a transpose whose result is the first contribution to a cotangent
+        # (zeroed, nothing added before it in the reverse pass) writes it
+        # directly: no temporary, no accumulate launch
+        touched = {self.out_of[s] for s in self.state}  # the carried cotangent
+        for a in self._adj:
+            if a[0] == "kernel":
+                touched |= set(a[3])
+            else:
+                a[2].update(s_ for s_ in a[1].srcs if s_ in self._bar and s_ not in touched)
+                touched |= {s_ for s_ in a[1].srcs if s_ in self._bar}
         pre = ("__device__ __forceinline__ float ts_sign(float x) "
                "{ return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }\n")
         self.adjoint_source = pre + "\n".join(_dedupe_helpers(srcs))
@@ -436,17 +446,18 @@ class AdjointModel:
                 _lib.check(L.tsphere_kernel_launch(h, op.count, len(ptrs), arr, npar, pv, sh),
                            f"adjoint of {op.label}")
                 continue
-            kind, op = a
+            kind, op, direct = a
             n = len(op.srcs)
             obar = (C.c_void_p * n)(*[self._bar[d].ptr for d in op.dsts])
-            tmp = (C.c_void_p * n)(*[self._tmp[s].ptr for s in op.srcs])
+            tmp = (C.c_void_p * n)(*[(self._bar if s in direct else self._tmp)[s].ptr
+                                     for s in op.srcs])
             if kind == "synth":   # x = S y: y_bar += S^T x_bar
                 rc = L.tsphere_synth_adjoint(self.sht.handle, n, 1, obar, 0, tmp, 0, sh)
             else:                 # x = A y: y_bar += A^T x_bar
                 rc = L.tsphere_analys_adjoint(self.sht.handle, n, 1, obar, 0, tmp, 0, sh)
             _lib.check(rc, f"transpose of {op.label}")
             for s_ in op.srcs:
-                if s_ in self._bar:
+                if s_ in self._bar and s_ not in direct:
                     _lib.check(L.tsphere_accumulate(self._bar[s_].ptr, self._tmp[s_].ptr,
                                                     self._tmp[s_].count, sh), "accumulate")
 
