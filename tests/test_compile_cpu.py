@@ -412,3 +412,32 @@ def test_grid_fused_plan_external_input():
     g = step.ops[0]
     assert isinstance(g, GridOp) and len(g.ins) == 1 and len(g.dsts) == 2
     assert "const float* gin0" in step.kernel_source()
+
+
+@pytest.mark.parametrize("key", sorted(EXPECTED_KERNELS))
+def test_keep_all_fused_kernels_store_every_value(key):
+    """The reverse pass's step tape (ModelPlan.create(keep_all=True), no
+    feedback): consecutive element-wise statements fuse into one kernel
+    that writes every value it computes -- each statement's result keeps a
+    buffer the reverse pass reads -- while the default fused plan writes
+    only what is read after it."""
+    from tsphere.plan import FusedOp, HostBuffers, KernelOp, ModelPlan, PlanSpec
+
+    _, step_fn, state, view = compile_fns(key)
+    plans = {}
+    for keep in (False, True):
+        host = HostBuffers(allocate=False)
+        host.ensure_group(state, 2 * 2080)
+        plans[keep] = ModelPlan.create(None, PlanSpec(step_fn, [None] * len(step_fn.outputs)),
+                                       host, True, "tape", grid=False, keep_all=keep)
+    tape = plans[True]
+    members = [m for op in tape.ops for m in (op.members if isinstance(op, FusedOp) else
+                                              [op] if isinstance(op, KernelOp) else [])]
+    assert len(members) == EXPECTED_KERNELS[key]
+    stored = {c for op in tape.ops if isinstance(op, FusedOp) for c in op.outs}
+    stored |= {op.out for op in tape.ops if isinstance(op, KernelOp)}
+    assert stored == {m.out for m in members}
+    # the same fusion as the default plan: only the stores differ
+    assert tape.generated_kernels() == plans[False].generated_kernels()
+    lean = {c for op in plans[False].ops if isinstance(op, FusedOp) for c in op.outs}
+    assert lean <= stored
