@@ -298,27 +298,38 @@ def test_ensemble_plan_kernels():
 
 @pytest.mark.parametrize("key", ["schnakenberg", "brusselator", "allencahn"])
 def test_adjoint_kernels_compile(key):
-    """Reverse mode: every element-wise statement of the step gets a
-    generated adjoint kernel (symbolic derivatives of its IR tree) that
-    compiles for gfx950; derivatives of the reaction terms appear."""
+    """Reverse mode: every element-wise statement of the step gets
+    generated adjoint code (symbolic derivatives of its IR tree) that
+    compiles for gfx950; derivatives of the reaction terms appear.  One
+    kernel per statement, and one per fused kernel of the step tape (its
+    statements in reverse order, each in its own scope)."""
+    import re
+
     from tsphere import _lib
     from tsphere.adjoint import build_adjoint_kernel
-    from tsphere.plan import HostBuffers, KernelOp, ModelPlan, PlanSpec
+    from tsphere.plan import FusedOp, HostBuffers, KernelOp, ModelPlan, PlanSpec
 
     _, step_fn, state, view = compile_fns(key)
-    host = HostBuffers(allocate=False)
-    host.ensure_group(state, 2 * 2080)
-    tape = ModelPlan.create(None, PlanSpec(step_fn, [None] * len(step_fn.outputs)), host, True,
-                            "tape", fuse=False)
-    srcs = []
-    for i, op in enumerate(tape.ops):
-        if isinstance(op, KernelOp):
-            gi = [c for c in op.ins if c in tape._bufs and not c.startswith("lam")]
-            srcs.append(build_adjoint_kernel(op, f"adj{i}", gi, len(tape.param_names)))
-    src = ("__device__ __forceinline__ float ts_sign(float x) { return x; }\n" +
-           "\n".join(srcs))
-    assert "bar0[i] += ob *" in src and "partials[" in src
-    assert _lib.load().tsphere_rtc_check(src.encode(), None) == _lib.OK, _lib.last_error()
+    for fuse in (False, True):
+        host = HostBuffers(allocate=False)
+        host.ensure_group(state, 2 * 2080)
+        tape = ModelPlan.create(None, PlanSpec(step_fn, [None] * len(step_fn.outputs)), host,
+                                True, "tape", fuse=fuse, grid=False, keep_all=True)
+        srcs, nstmt = [], 0
+        for i, op in enumerate(tape.ops):
+            if isinstance(op, (KernelOp, FusedOp)):
+                members = list(reversed(op.members)) if isinstance(op, FusedOp) else [op]
+                ops_gis = [(m, [c for c in m.ins if c in tape._bufs and not c.startswith("lam")])
+                           for m in members]
+                code, ins, bars = build_adjoint_kernel(ops_gis, f"adj{i}", len(tape.param_names))
+                assert code.count("// adjoint of:") == len(members)
+                nstmt += len(members)
+                srcs.append(code)
+        assert nstmt == EXPECTED_KERNELS[key]
+        src = ("__device__ __forceinline__ float ts_sign(float x) { return x; }\n" +
+               "\n".join(srcs))
+        assert re.search(r"bar\d+\[i\] \+= ob \*", src) and "partials[" in src
+        assert _lib.load().tsphere_rtc_check(src.encode(), None) == _lib.OK, _lib.last_error()
 
 
 @pytest.mark.parametrize("key", ["brusselator", "schnakenberg", "allencahn"])

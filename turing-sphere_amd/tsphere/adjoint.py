@@ -171,34 +171,61 @@ def derivative(e, wrt: str, ctx: _Ctx):
     raise UnsupportedOnGpu(f"cannot differentiate '{k}'", getattr(e, "span", None))
 
 
-def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: int):
-    """HIP source of the adjoint of one element-wise statement.
+def build_adjoint_kernel(ops_gis, name: str, nparams: int):
+    """HIP source of the adjoint of consecutive element-wise statements (the
+    statements of one fused kernel, or one statement), taken per element in
+    the order given -- the reverse of the step's.
 
-    Signature: (long long n, const float* obar, const float* in0.., float*
-    bar_<k>.. (one per entry of `grad_inputs`, in that order), float*
-    partials, float p0..): bar_k[i] += obar[i] * d out / d in_k, and
-    partials[block * nparams + j] += sum over the block of obar * d out / d p_j.
-    """
-    io = op.io
-    ctx = _Ctx(io)
-    _emit(op.stmt.expr, ctx)  # binds derived scalars used by the value (prologue)
-    dins = []
-    for c in grad_inputs:
-        d = derivative(op.stmt.expr, c, ctx)
-        dins.append(d)
-    pnames = sorted(io.params, key=lambda c: io.params[c])
-    dps = [derivative(op.stmt.expr, c, ctx) for c in pnames]
-    nin = len(io.tensors)
-    loads = "\n".join(f"      const float a{i} = in{i}[i];" for i in range(nin))
-    pro = "\n".join("      " + line.strip() for line in ctx.prologue)
-    kin = "".join(f", const float* __restrict__ in{i}" for i in range(nin))
-    kbar = "".join(f", float* __restrict__ bar{j}" for j in range(len(grad_inputs)))
-    kpar = "".join(f", float p{i}" for i in range(nparams))
-    upd = "\n".join(f"      bar{j}[i] += ob * {d};" for j, d in enumerate(dins) if d is not None)
-    acc = "\n".join(f"      c{j} += ob * {d};" for j, d in enumerate(dps) if d is not None)
+    ``ops_gis``: [(KernelOp, grad_inputs)], grad_inputs the c_names whose
+    cotangents the statement adds to.  Signature: (long long n, const
+    float* in0.. (``ins``), float* bar0.. (``bars``), float* partials, float
+    p0..).  For each element i and statement k: ob = bar[out_k][i];
+    bar[c][i] += ob * d out_k / d c for c in its grad inputs;
+    partials[block * nparams + j] += the block's sum of ob * d out_k / d p_j.
+    A statement's output cotangent may have been added to by a later
+    statement of the step, processed before it here for the same element,
+    so the cotangent pointers carry no __restrict__.
+    Returns (code, ins, bars)."""
+    ins, bars, blocks = [], [], []
+    dps_all = {}  # parameter slot -> used by some statement
+    helpers_src = ""
+    for k, (op, gi) in enumerate(ops_gis):
+        io = op.io
+        ctx = _Ctx(io)
+        _emit(op.stmt.expr, ctx)  # binds derived scalars used by the value (prologue)
+        dins = [derivative(op.stmt.expr, c, ctx) for c in gi]
+        dps = {io.params[c]: derivative(op.stmt.expr, c, ctx) for c in io.params}
+        ren = lambda t, k=k: re.sub(r"\ba(\d+)\b", rf"s{k}a\1", t)
+        for c in sorted(io.tensors, key=lambda c: io.tensors[c]):
+            if c not in ins:
+                ins.append(c)
+        for c in [op.out] + list(gi):
+            if c not in bars:
+                bars.append(c)
+        loads = "\n".join(f"        const float s{k}a{io.tensors[c]} = in{ins.index(c)}[i];"
+                          for c in sorted(io.tensors, key=lambda c: io.tensors[c]))
+        pro = "\n".join("        " + ren(line.strip()) for line in ctx.prologue)
+        upd = "\n".join(f"        bar{bars.index(c)}[i] += ob * {ren(d)};"
+                        for c, d in zip(gi, dins) if d is not None)
+        acc = "\n".join(f"        c{j} += ob * {ren(d)};"
+                        for j, d in sorted(dps.items()) if d is not None)
+        for j, d in dps.items():
+            if d is not None:
+                dps_all[j] = True
+        helpers_src += upd + acc + pro
+        blocks.append(f"""      {{  // adjoint of: {op.label}
+        const float ob = bar{bars.index(op.out)}[i];
+{loads}
+{pro}
+{upd}
+{acc}
+      }}""")
     np_ = max(1, nparams)
+    kin = "".join(f", const float* __restrict__ in{i}" for i in range(len(ins)))
+    kbar = "".join(f", float* bar{j}" for j in range(len(bars)))
+    kpar = "".join(f", float p{i}" for i in range(nparams))
     cdecl = " ".join(f"float c{j} = 0.0f;" for j in range(nparams))
-    live = [j for j, d in enumerate(dps) if d is not None]  # parameters this statement uses
+    live = sorted(dps_all)  # parameters some statement uses
     red = ""
     for j in live:
         red += f"""  for (int o = 32; o > 0; o >>= 1) c{j} += __shfl_xor(c{j}, o);
@@ -207,23 +234,19 @@ def build_adjoint_kernel(op: KernelOp, name: str, grad_inputs: list, nparams: in
     fin = "\n".join(
         f"    partials[(long long)blockIdx.x * {np_} + {j}] += "
         f"(red[{j}][0] + red[{j}][1]) + (red[{j}][2] + red[{j}][3]);" for j in live)
-    used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", upd + acc + pro)}
+    used = {int(m) for m in re.findall(r"\bpow_i(\d+)\(", helpers_src)}
     helpers = "\n".join(
         f"__device__ __forceinline__ float pow_i{k}(float x) {{ return {'x' + ' * x' * (k - 1)}; }}"
         for k in sorted(used))
+    body = "\n".join(blocks)
     code = f"""{helpers}
-// adjoint of: {op.label}
-extern "C" __global__ __launch_bounds__({WG}) void {name}(long long n, const float* __restrict__ obar{kin}{kbar}, float* __restrict__ partials{kpar}) {{
+extern "C" __global__ __launch_bounds__({WG}) void {name}(long long n{kin}{kbar}, float* __restrict__ partials{kpar}) {{
   __shared__ float red[{np_}][4];
   // {EPT} consecutive elements per thread: the launch contract of tsphere_kernel_launch
   const long long i0 = ((long long)blockIdx.x * {WG} + threadIdx.x) * {EPT};
   {cdecl}
   for (long long i = i0; i < n && i < i0 + {EPT}; ++i) {{
-      const float ob = obar[i];
-{loads}
-{pro}
-{upd}
-{acc}
+{body}
   }}
 {red}{"  __syncthreads();" if live else ""}
   if ({"threadIdx.x == 0" if live else "false"}) {{
@@ -231,7 +254,7 @@ extern "C" __global__ __launch_bounds__({WG}) void {name}(long long n, const flo
   }}
 }}
 """
-    return code
+    return code, ins, bars
 
 
 # ------------------------------------------------------------- the engine
@@ -306,18 +329,19 @@ class AdjointModel:
         srcs = []
         self.param_slots = list(tape.param_names)
         npar = len(self.param_slots)
-        rev = []
-        for op in reversed(tape.ops):  # statements of a fused kernel: their own adjoints
-            rev.extend(reversed(op.members) if isinstance(op, FusedOp) else [op])
-        for idx, op in enumerate(rev):
-            if isinstance(op, KernelOp):
-                gi = [c for c in op.ins if c in self._bar]
+        for idx, op in enumerate(reversed(tape.ops)):
+            if isinstance(op, (KernelOp, FusedOp)):
+                # a fused kernel's statements: one adjoint kernel, reversed
+                members = list(reversed(op.members)) if isinstance(op, FusedOp) else [op]
+                ops_gis = [(m, [c for c in m.ins if c in self._bar]) for m in members]
                 name = f"adj{idx}"
-                code = build_adjoint_kernel(op, name, gi, npar)
-                nblk = (op.count + WG * EPT - 1) // (WG * EPT)
+                code, ins, bars = build_adjoint_kernel(ops_gis, name, npar)
+                count = members[0].count
+                nblk = (count + WG * EPT - 1) // (WG * EPT)
                 part = DeviceBuffer(nblk * max(1, npar), f"partials-{name}")
                 self._owned.append(part)
-                self._adj.append(("kernel", op, name, gi, part, nblk))
+                gi = sorted({c for _, g in ops_gis for c in g})
+                self._adj.append(("kernel", (ins, bars, count), name, gi, part, nblk))
                 srcs.append(code)
             elif isinstance(op, XformOp):
                 for s_ in op.srcs:
@@ -439,12 +463,12 @@ class AdjointModel:
         nq, ns = 2 * self.nlm, self.npts
         for a in self._adj:
             if a[0] == "kernel":
-                _, op, name, gi, part, nblk, h = a
-                ptrs = ([self._bar[op.out].ptr] + [self.tape._bufs[c].ptr for c in op.ins] +
-                        [self._bar[c].ptr for c in gi] + [part.ptr])
+                _, (ins, bars, count), name, gi, part, nblk, h = a
+                ptrs = ([self.tape._bufs[c].ptr for c in ins] + [self._bar[c].ptr for c in bars] +
+                        [part.ptr])
                 arr = (C.c_void_p * len(ptrs))(*ptrs)
-                _lib.check(L.tsphere_kernel_launch(h, op.count, len(ptrs), arr, npar, pv, sh),
-                           f"adjoint of {op.label}")
+                _lib.check(L.tsphere_kernel_launch(h, count, len(ptrs), arr, npar, pv, sh),
+                           f"adjoint kernel {name}")
                 continue
             kind, op, direct = a
             n = len(op.srcs)
