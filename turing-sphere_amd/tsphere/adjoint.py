@@ -39,7 +39,7 @@ from .ir import is_numeric, is_tensor
 from .layout import ShtConfig
 from .mfront import CompiledModel, model_bindings
 from .model import eigenvalues
-from .plan import FusedOp, HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
+from .plan import FusedOp, GridOp, HostBuffers, KernelOp, ModelPlan, PlanSpec, XformOp, _compile
 
 WG = 256
 EPT = 4  # elements per thread (tsphere_kernel_launch launches count / 4 threads)
@@ -289,10 +289,11 @@ class AdjointModel:
         host.ensure("lam", 2 * nlm)
         self._host = host
         # the step tape: every value stored for the reverse pass, consecutive
-        # element-wise statements in one kernel each (fewer graph nodes per
-        # regenerated step), the Fourier stages separate
+        # element-wise statements in one kernel each and a synth ->
+        # element-wise -> analys run in one grid kernel (fewer graph nodes per
+        # regenerated step)
         tape = ModelPlan.create(sht, PlanSpec(fn, [None] * len(fn.outputs)), host, True, "tape",
-                                grid=False, keep_all=True)
+                                keep_all=True)
         self.tape = tape
         # state <- step(state) only: the fused plan, writing the state in place
         # (the spectral op where the step has its form, else the fold: the
@@ -329,7 +330,10 @@ class AdjointModel:
         srcs = []
         self.param_slots = list(tape.param_names)
         npar = len(self.param_slots)
-        for idx, op in enumerate(reversed(tape.ops)):
+        rev = []
+        for op in reversed(tape.ops):  # a grid op: its Legendre/element-wise/Legendre ops
+            rev.extend(reversed(op.members) if isinstance(op, GridOp) else [op])
+        for idx, op in enumerate(rev):
             if isinstance(op, (KernelOp, FusedOp)):
                 # a fused kernel's statements: one adjoint kernel, reversed
                 members = list(reversed(op.members)) if isinstance(op, FusedOp) else [op]

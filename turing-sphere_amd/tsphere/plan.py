@@ -275,9 +275,10 @@ class ModelPlan:
         ``spectral`` (single runs, with ``grid``): a step made of one grid op
         and the implicit solve of its species becomes a SpectralOp (the
         analysis, the solve and the next step's synthesis in one kernel).
-        ``keep_all`` (with ``fuse``): a fused element-wise kernel writes every
-        value it computes to its buffer, not only those read after it (a
-        step tape whose values a reverse pass reads).
+        ``keep_all`` (with ``fuse``): a fused element-wise kernel, or a grid
+        kernel, writes every value it computes to its buffer (the synthesised
+        grid values included), not only those read after it (a step tape
+        whose values a reverse pass reads).
         ``fold`` (with ``grid``, when not ``spectral``; ensembles too): the
         same step keeps the analysis as its own launch and folds the implicit
         solve into the next step's synthesis launch -- for an ensemble with
@@ -392,7 +393,8 @@ class ModelPlan:
         if batch and fuse:
             ops = self._fuse(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c, keep_all)
             if grid:
-                ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c)
+                ops = self._fuse_grid(ops, {o.c_name for o in fn.outputs}, host, runs, shared_c,
+                                      keep_all)
                 mode = "spectral" if spectral and runs == 1 else "fold" if fold else None
                 if mode:
                     ops = self._fuse_spectral(ops, {o.c_name for o in fn.outputs}, param_slots,
@@ -494,7 +496,8 @@ class ModelPlan:
             i = j
         return out
 
-    def _fuse_grid(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c) -> list:
+    def _fuse_grid(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c,
+                   keep_all: bool = False) -> list:
         """Turn each synth -> element-wise -> analys run into a GridOp: the
         element-wise statements run between the row FFTs of one generated
         kernel, so the synthesised grid values are written only if something
@@ -530,7 +533,8 @@ class ModelPlan:
             for o in ops[:i] + ops[i + 3:]:
                 later |= _reads(o)
             stores = [c for c in list(S.dsts) + produced
-                      if c in outputs or c in later or id(self._bufs.get(c)) in host_bufs]
+                      if keep_all or c in outputs or c in later or
+                      id(self._bufs.get(c)) in host_bufs]
             name = f"g{len(out)}_" + "_".join(_ident(st.name) for st in stmts)[:40]
             lbl = f"{S.label} => {E.label} => {A.label}"
             kern = build_grid_kernel(name, lbl, log2h, cfg.nlat, cfg.mmax, list(S.dsts), stmts,
