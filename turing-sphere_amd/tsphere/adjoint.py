@@ -348,6 +348,15 @@ class AdjointModel:
                 self._adj.append(("kernel", (ins, bars, count), name, gi, part, nblk))
                 srcs.append(code)
             elif isinstance(op, XformOp):
+                if not any(s_ in self._tmp for s_ in op.srcs):
+                    # the transpose's outputs side by side: one accumulate for all
+                    tot = sum(tape._bufs[s_].count for s_ in op.srcs)
+                    arena = DeviceBuffer(tot, f"tmp-{op.kind}{idx}")
+                    self._owned.append(arena)
+                    off = 0
+                    for s_ in op.srcs:
+                        self._tmp[s_] = DeviceBuffer.view(arena, off, tape._bufs[s_].count)
+                        off += tape._bufs[s_].count
                 for s_ in op.srcs:
                     if s_ not in self._tmp:
                         self._tmp[s_] = DeviceBuffer(tape._bufs[s_].count, f"tmp-{s_}")
@@ -484,10 +493,18 @@ class AdjointModel:
             else:                 # x = A y: y_bar += A^T x_bar
                 rc = L.tsphere_analys_adjoint(self.sht.handle, n, 1, obar, 0, tmp, 0, sh)
             _lib.check(rc, f"transpose of {op.label}")
-            for s_ in op.srcs:
-                if s_ in self._bar and s_ not in direct:
-                    _lib.check(L.tsphere_accumulate(self._bar[s_].ptr, self._tmp[s_].ptr,
-                                                    self._tmp[s_].count, sh), "accumulate")
+            acc = [s_ for s_ in op.srcs if s_ in self._bar and s_ not in direct]
+            if len(acc) > 1 and all(
+                    self._bar[c].ptr == self._bar[acc[0]].ptr + 4 * k * self._bar[c].count and
+                    self._tmp[c].ptr == self._tmp[acc[0]].ptr + 4 * k * self._tmp[c].count and
+                    self._tmp[c].count == self._tmp[acc[0]].count for k, c in enumerate(acc)):
+                _lib.check(L.tsphere_accumulate(self._bar[acc[0]].ptr, self._tmp[acc[0]].ptr,
+                                                len(acc) * self._tmp[acc[0]].count, sh),
+                           "accumulate")
+                continue
+            for s_ in acc:
+                _lib.check(L.tsphere_accumulate(self._bar[s_].ptr, self._tmp[s_].ptr,
+                                                self._tmp[s_].count, sh), "accumulate")
 
     def vjp(self, state0: dict, steps: int, cot: dict):
         """Cotangents of the initial state and of the parameters for a
