@@ -18,11 +18,12 @@ Trajectories are checkpointed: the forward pass keeps the state every
 ``checkpoint_every`` steps; the backward pass re-runs each segment from its
 checkpoint, keeping the segment's step inputs, and differentiates its steps
 in reverse, re-running each step once more to regenerate its intermediates.
-Steps that only advance the state run through the forward plan (fused, and
-grid-fused where it applies; bit-identical to the tape's step), the initial
-pass as HIP-graph replays; each position j of a segment has one captured
-graph for "keep input j, advance" and one for "regenerate step j,
-differentiate it", so a gradient is a short sequence of graph launches.
+Steps that only advance the state run through the forward plan (fused, with
+the spectral op where the step has its form; bit-identical to the tape's
+step), the initial pass as HIP-graph replays; each segment of the backward
+pass is one captured graph (keep every step's input while advancing, then
+regenerate and differentiate the steps in reverse), so a gradient is a short
+sequence of graph launches.
 """
 from __future__ import annotations
 
@@ -515,7 +516,7 @@ class AdjointModel:
         nseg = max(1, math.ceil(steps / C_))
         chk = DeviceBuffer(nseg * n1, "checkpoints")
         if self._seg is None or self._seg.count < C_ * n1:
-            # the per-position graphs hold the segment and carry buffers: keep
+            # the segment graphs hold the segment and carry buffers: keep
             # them for the model's life, re-capturing only when they grow
             self._drop_graphs()
             if self._seg is not None:
