@@ -226,6 +226,21 @@ int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
                                   const float *const *analysed, float *const *state,
                                   long long qlm_stride, const float *lam, const float *coef,
                                   float *fm, tsphere_stream s);
+/* The implicit step behind the analysis, for a batch: the Legendre analysis
+ * of the scratch fm (as tsphere_leg_analys_fields), then state field f
+ * replaced by X_new = (X + s1 A_f) / (1 + s2 lam) per real coefficient, A_f
+ * its analysed values rounded as written, (s1, s2) = coef[2 f], coef[2 f + 1]
+ * (device array, as tsphere_leg_imex_synth_fields); field f = pointer f / per,
+ * copy f % per at qlm_stride floats for analysed and state alike.  With the
+ * per-parity product analysis tile (impl 5) the step runs in the analysis
+ * kernel and analysed is not written; other tiles write analysed and run the
+ * step as a second kernel.  Any field count up to max_fields.  No reference
+ * counterpart of its own (it fuses encodeAnalysFrom and the implicit-solve
+ * dispatch of plan.ts:436-478 for a batch of runs). */
+int tsphere_leg_analys_imex_fields(tsphere_plan p, int nptr, int per, const float *fm,
+                                   float *const *analysed, float *const *state,
+                                   long long qlm_stride, const float *lam, const float *coef,
+                                   tsphere_stream s);
 /* The spectral op: per m-row, the Legendre analysis of the scratch fm
  * (nfield fields, as tsphere_leg_analys_multi), the diagonal implicit step
  *   X_new = (X + s1[f] A_f) / (1 + s2[f] lam)     per real coefficient

@@ -247,15 +247,17 @@ def test_ensemble_grid_fusion_bit_exact(gpu):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("legendre", ["fp32", "f64rec", "bf16x3"])
+@pytest.mark.parametrize("legendre,impl", [("fp32", None), ("fp32", 5), ("f64rec", None),
+                                           ("f64rec", 5), ("bf16x3", None)])
 @pytest.mark.parametrize("model,R", [("schnakenberg", 3), ("allencahn", 9)])
-def test_ensemble_fold_bit_exact(gpu, legendre, model, R):
-    """An ensemble step folds the implicit solve into the next step's
-    synthesis (tsphere_leg_imex_synth_fields, each run's s1 and s2 from a
-    device array): equal to the ensemble plan with the separate generated
-    solve kernel bit for bit, over graph replays, an eager remainder and a
-    per-run parameter change between calls (an upload, no re-capture).
-    Nine single-species runs put a partial group of fields in the last
+def test_ensemble_fold_bit_exact(gpu, legendre, impl, model, R):
+    """An ensemble step folds the implicit solve into the analysis
+    (tsphere_leg_analys_imex_fields, each run's s1 and s2 from a device
+    array; the per-parity product analysis tile, impl 5, applies it in its
+    combine): equal to the ensemble plan with the separate generated solve
+    kernel bit for bit, over graph replays, an eager remainder and a per-run
+    parameter change between calls (an upload, no re-capture).  Nine
+    single-species runs put a partial group of fields in the last
     workgroup."""
     import tsphere
     from tsphere.model import GpuModel
@@ -270,10 +272,12 @@ def test_ensemble_fold_bit_exact(gpu, legendre, model, R):
         src, names, state, view = AC_SRC, ["eps2", "dt"], ["U"], ["u"]
         p1 = {"eps2": np.linspace(1e-4, 2.5e-3, R), "dt": 2e-3}
         p2 = {"eps2": np.linspace(2.5e-3, 1e-4, R), "dt": 3e-3}
-    emit(f"ensemble fold {model} {legendre} lmax31 runs {R}")
+    emit(f"ensemble fold {model} {legendre} analysis tile {impl} lmax31 runs {R}")
     outs = []
     for fold in (False, True):
         sht = tsphere.ShtPlan.create(cfg, legendre=legendre, max_fields=len(state) * R)
+        if impl is not None:
+            sht.set_tile("analys", impl=impl)
         gm = GpuModel.create(sht, cfg, src, names, state, view, runs=R, fold=fold)
         assert ("imex" in [l.split()[0] for l in gm.describe()["step"]]) == fold
         gm.set_params(p1)

@@ -275,3 +275,77 @@ def test_batched_fold_entry_matches_per_run_calls(gpu, legendre):
     assert np.all(np.isfinite(got_fm)) and np.array_equal(got_x, want_x)
     assert not np.array_equal(got_x, x0)
     plan.destroy()
+
+
+@pytest.mark.parametrize("legendre,impl", [("fp32", None), ("fp32", 5), ("f64rec", None),
+                                           ("f64rec", 5), ("bf16x3", None)])
+@pytest.mark.parametrize("lmax,nlat,nphi,nsp,R", [(31, 32, 64, 2, 5), (63, 65, 128, 1, 19)])
+def test_batched_analysis_fold_entry(gpu, legendre, impl, lmax, nlat, nphi, nsp, R):
+    """tsphere_leg_analys_imex_fields (the analysis with the implicit step of
+    each field behind it, the scalars of field f at coef[2 f], coef[2 f + 1])
+    against tsphere_leg_analys_fields followed by the implicit step of
+    tsphere_leg_imex_synth_fields: the same analysed values, rounded as
+    written, enter the same step, so the state agrees bit for bit.  With
+    the per-parity product tile (impl 5) the step runs in the analysis
+    kernel's combine and the analysed buffers stay untouched; 19 runs leave
+    a partial group of 16 fields, an odd grid pairs the equator with
+    itself."""
+    import tsphere
+    from tsphere import _lib
+    from tsphere.device import DeviceBuffer
+
+    emit(f"analysis fold entry {legendre} tile {impl} lmax {lmax} {nlat}x{nphi} "
+         f"species {nsp} runs {R}")
+    L = _lib.load()
+    plan = tsphere.ShtPlan.create(tsphere.ShtConfig(lmax, lmax, nlat, nphi), legendre=legendre,
+                                  max_fields=nsp * R)
+    if impl is not None:
+        plan.set_tile("analys", impl=impl)
+    s, nlm, M = plan.stream, plan.nlm, lmax
+    rng = np.random.default_rng(17)
+    fsz = 2 * (M + 1) * nlat
+    fm = DeviceBuffer(nsp * R * fsz, "fm")
+    fm.upload(rng.normal(size=nsp * R * fsz).astype(np.float32), s)
+    x0 = (0.1 * rng.normal(size=(nsp, R, 2 * nlm))).astype(np.float32)
+    ell = np.repeat(np.concatenate([np.arange(m, lmax + 1) for m in range(M + 1)]), 2)
+    lamb = DeviceBuffer(2 * nlm, "lam")
+    lamb.upload((ell * (ell + 1)).astype(np.float32), s)
+    coef = DeviceBuffer(2 * nsp * R, "coef")
+    s1 = np.float32(rng.uniform(1e-3, 5e-3, size=(nsp, R)))
+    s2 = np.float32(rng.uniform(1e-6, 1e-4, size=(nsp, R)))
+    coef.upload(np.stack([s1, s2], axis=-1).reshape(-1), s)
+
+    def buffers(tag):
+        xs = [DeviceBuffer(R * 2 * nlm, f"x{tag}{k}") for k in range(nsp)]
+        an = [DeviceBuffer(R * 2 * nlm, f"a{tag}{k}") for k in range(nsp)]
+        for k in range(nsp):
+            xs[k].upload(x0[k].reshape(-1), s)
+            an[k].fill(np.nan, s)
+        return xs, an
+
+    def ptrs(bs):
+        return (C.c_void_p * nsp)(*[b.ptr for b in bs])
+
+    xb, ab = buffers("B")
+    _lib.check(L.tsphere_leg_analys_imex_fields(plan.handle, nsp, R, fm.ptr, ptrs(ab), ptrs(xb),
+                                                2 * nlm, lamb.ptr, coef.ptr, s.handle),
+               "analys_imex_fields")
+    got_x = np.stack([b.download(s) for b in xb])
+    got_a = np.stack([b.download(s) for b in ab])
+
+    xr, ar = buffers("R")
+    _lib.check(L.tsphere_leg_analys_fields(plan.handle, nsp, R, fm.ptr, ptrs(ar), 2 * nlm,
+                                           s.handle), "analys_fields")
+    want_a = np.stack([b.download(s) for b in ar])
+    junk = DeviceBuffer(nsp * R * fsz, "junk")
+    _lib.check(L.tsphere_leg_imex_synth_fields(plan.handle, nsp, R, ptrs(ar), ptrs(xr), 2 * nlm,
+                                               lamb.ptr, coef.ptr, junk.ptr, s.handle),
+               "imex_synth_fields")
+    want_x = np.stack([b.download(s) for b in xr])
+    assert np.all(np.isfinite(got_x)) and np.array_equal(got_x, want_x)
+    assert not np.array_equal(got_x, x0.reshape(nsp, -1))
+    if plan.get_tile("analys", nsp * R)["impl"] == 5:
+        assert np.all(np.isnan(got_a))
+    else:
+        assert np.array_equal(got_a, want_a)
+    plan.destroy()

@@ -2118,9 +2118,22 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
   }
 }
 
-template <int CB, class P>
+// X = (X + s1 a) / (1 + s2 lam) of one complex coefficient, (s1, s2) =
+// sc[0], sc[1]
+__device__ __forceinline__ void imex_apply(float2* X, float2 a, float2 lm, const float* sc) {
+  const float s1 = sc[0], s2 = sc[1];
+  const float2 x = *X;
+  *X = make_float2(imex_step(x.x, s1, a.x, s2, lm.x), imex_step(x.y, s1, a.y, s2, lm.y));
+}
+
+// IMEX: Fq is the state, and each analysed coefficient a of field f goes
+// straight into its implicit step (s1, s2 = coef[2 f], coef[2 f + 1]; lam
+// per coefficient) instead of to memory
+template <int CB, bool IMEX, class P>
 __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, int nfield,
-                                                        const float2* __restrict__ fm, Fields Fq) {
+                                                        const float2* __restrict__ fm, Fields Fq,
+                                                        const float* __restrict__ lam,
+                                                        const float* __restrict__ coef) {
   constexpr int NF = 8 * CB, SP = G16_ASPAN, TROW = G16_ATROW;
   constexpr int RS = (SP / 2) * TROW;  // floats of a wave's tile
   static_assert(2 * CB * 256 <= RS, "a wave's sums of a span fit its tile");
@@ -2153,13 +2166,18 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
   for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
   lblk = min(lblk, L + 1);
 
-  // coefficients below every lane's significance point are exactly 0
+  // coefficients below every lane's significance point are exactly 0 (the
+  // implicit step with a = 0 there)
   const int nz = lblk - m;
   for (int o = tid; o < NF * nz; o += nthr) {
     const int f = o / nz, i = o - f * nz;
-    if (f0 + f < nfield)
-      reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + i] =
-          make_float2(0.f, 0.f);
+    if (f0 + f >= nfield) continue;
+    float2* const X = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f))) + base + i;
+    if constexpr (IMEX)
+      imex_apply(X, make_float2(0.f, 0.f), reinterpret_cast<const float2*>(lam)[base + i],
+                 coef + 2 * (f0 + f));
+    else
+      *X = make_float2(0.f, 0.f);
   }
 
   // B operands of the whole row: W_par of pair 64 w + 4 kp + g, column
@@ -2255,10 +2273,20 @@ __global__ __launch_bounds__(512) void k_leg_analys_g16(typename P::Tables T, in
       float2* dst = q + (l - m);
       const float* cz = T.cz + base + (l - m);
       const float c0 = cz[0];
-      dst[0] = make_float2(v.x * c0, v.z * c0);  // (re, im) of l: parity 0
+      const float2 a0 = make_float2(v.x * c0, v.z * c0);  // (re, im) of l: parity 0
+      if constexpr (IMEX)
+        imex_apply(dst, a0, reinterpret_cast<const float2*>(lam)[base + (l - m)],
+                   coef + 2 * (f0 + f));
+      else
+        dst[0] = a0;
       if (l + 1 <= L) {
         const float c1 = cz[1];
-        dst[1] = make_float2(v.y * c1, v.w * c1);
+        const float2 a1 = make_float2(v.y * c1, v.w * c1);
+        if constexpr (IMEX)
+          imex_apply(dst + 1, a1, reinterpret_cast<const float2*>(lam)[base + (l - m) + 1],
+                     coef + 2 * (f0 + f));
+        else
+          dst[1] = a1;
       }
     }
     __syncthreads();  // the tiles are rewritten in the next span
@@ -2278,14 +2306,15 @@ hipError_t synth_g16(const typename P::Tables& T, int nfield, int nwaves, const 
   return hipGetLastError();
 }
 
-template <int CB, class P>
+template <int CB, bool IMEX, class P>
 hipError_t analys_g16(const typename P::Tables& T, int nfield, int nwaves, const float2* fm,
-                      const Fields& q, hipStream_t s) {
+                      const Fields& q, const float* lam, const float* coef, hipStream_t s) {
   if (nwaves < 1 || nwaves > 8 || 64 * nwaves < T.npair) return hipErrorInvalidValue;
   dim3 grid(T.mmax + 1, (nfield + 8 * CB - 1) / (8 * CB));
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (size_t)nwaves * (G16_ASPAN / 2) * G16_ATROW;
-  hipLaunchKernelGGL((k_leg_analys_g16<CB, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm, q);
+  hipLaunchKernelGGL((k_leg_analys_g16<CB, IMEX, P>), grid, dim3(64 * nwaves), lds, s, T, nfield,
+                     fm, q, lam, coef);
   return hipGetLastError();
 }
 
@@ -2298,12 +2327,13 @@ hipError_t synth_g16_nf(const typename P::Tables& T, int nfield, const LegTile& 
   return synth_g16<2, 1, P>(T, nfield, t.nwaves, q, fm, s);
 }
 
-template <class P>
+template <class P, bool IMEX = false>
 hipError_t analys_g16_nf(const typename P::Tables& T, int nfield, const LegTile& t,
-                         const float2* fm, const Fields& q, hipStream_t s) {
+                         const float2* fm, const Fields& q, hipStream_t s,
+                         const float* lam = nullptr, const float* coef = nullptr) {
   if (t.rows != 1) return hipErrorInvalidValue;  // one row per analysis workgroup
-  return t.nf == 8 ? analys_g16<1, P>(T, nfield, t.nwaves, fm, q, s)
-                   : analys_g16<2, P>(T, nfield, t.nwaves, fm, q, s);
+  return t.nf == 8 ? analys_g16<1, IMEX, P>(T, nfield, t.nwaves, fm, q, lam, coef, s)
+                   : analys_g16<2, IMEX, P>(T, nfield, t.nwaves, fm, q, lam, coef, s);
 }
 
 // ------------------------------ split-BF16 x3 on 4x4x4 matrix blocks (b4)
@@ -3106,6 +3136,52 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
     case 2: return analys_nf<2, RecD>(T, nfield, t, fm, q, s);
     default: return analys_nf<4, RecD>(T, nfield, t, fm, q, s);
   }
+}
+
+// ------------------------------- the analysis with the implicit step behind
+// X = (X + s1 A) / (1 + s2 lam) over the n2 floats of each field f (s1, s2 =
+// coef[2 f], coef[2 f + 1]): grid (x chunks, field)
+__global__ __launch_bounds__(256) void k_imex_fields(long long n2, const float* __restrict__ lam,
+                                                     const float* __restrict__ coef, Fields Fa,
+                                                     Fields Fq) {
+  const int f = blockIdx.y;
+  const float s1 = coef[2 * f], s2 = coef[2 * f + 1];
+  const float* A = reinterpret_cast<const float*>(Fa.at(f));
+  float* X = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f)));
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n2;
+       i += (long long)gridDim.x * 256)
+    X[i] = imex_step(X[i], s1, A[i], s2, lam[i]);
+}
+
+hipError_t launch_imex_fields(long long nlm, int nfield, const Fields& a, const Fields& q,
+                              const float* lam, const float* coef, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  const long long n2 = 2 * nlm;
+  dim3 grid((unsigned)std::min<long long>((n2 + 255) / 256, 1024), nfield);
+  hipLaunchKernelGGL(k_imex_fields, grid, dim3(256), 0, s, n2, lam, coef, a, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_leg_analys_imex(const LegTables& T, int nfield, const float2* fm,
+                                  const Fields& a, const Fields& q, const float* lam,
+                                  const float* coef, LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return analys_g16_nf<RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+  if (hipError_t e = launch_leg_analys(T, nfield, fm, a, tile, s)) return e;
+  return launch_imex_fields(nlm_calc(T.lmax, T.mmax), nfield, a, q, lam, coef, s);
+}
+
+hipError_t launch_leg_analys_imex_f64r(const LegTablesZ64& T, int nfield, const float2* fm,
+                                       const Fields& a, const Fields& q, const float* lam,
+                                       const float* coef, LegTile tile, hipStream_t s) {
+  if (nfield <= 0) return hipSuccess;
+  const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 1, T.npair, nfield, tile);
+  if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
+  if (t.impl == 5) return analys_g16_nf<RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+  if (hipError_t e = launch_leg_analys_f64r(T, nfield, fm, a, tile, s)) return e;
+  return launch_imex_fields(nlm_calc(T.lmax, T.mmax), nfield, a, q, lam, coef, s);
 }
 
 hipError_t launch_leg_imex_synth(const LegTables& T, int nfield, const Fields& q, const Fields& a,

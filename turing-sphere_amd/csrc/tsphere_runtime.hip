@@ -1004,6 +1004,31 @@ int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
   return ok();
 }
 
+int tsphere_leg_analys_imex_fields(tsphere_plan p, int nptr, int per, const float* fm,
+                                   float* const* analysed, float* const* state,
+                                   long long qlm_stride, const float* lam, const float* coef,
+                                   tsphere_stream s) {
+  if (int r = check_fields(p, nptr, per, (const void* const*)analysed, (const void* const*)state))
+    return r;
+  if (int r = check_stride(qlm_stride, 2 * p->nlm, per, "qlm_stride")) return r;
+  if (!fm || !lam || !coef) return fail(TSPHERE_ERR_INVALID, "NULL argument");
+  const int nf = nptr * per;
+  const Fields q = grouped((const void* const*)state, nptr, per, qlm_stride);
+  const Fields a = grouped((const void* const*)analysed, nptr, per, qlm_stride);
+  const float2* f2 = reinterpret_cast<const float2*>(fm);
+  if (p->cfg.legendre == TSPHERE_LEG_F64REC) {
+    TS_HIP(launch_leg_analys_imex_f64r(p->tablesz64(), nf, f2, a, q, lam, coef, p->tile[1], S(s)),
+           "leg_analys_imex");
+  } else if (p->cfg.legendre == TSPHERE_LEG_FP32) {
+    TS_HIP(launch_leg_analys_imex(p->tables(), nf, f2, a, q, lam, coef, p->tile[1], S(s)),
+           "leg_analys_imex");
+  } else {
+    TS_HIP(leg_analys(p, nf, f2, a, S(s)), "leg_analys");
+    TS_HIP(launch_imex_fields(p->nlm, nf, a, q, lam, coef, S(s)), "imex_fields");
+  }
+  return ok();
+}
+
 // ------------------------------------------------------- transposes
 // synth^T: G = 2^{[m>=1]} * FourierAnalysis(g), q = sum_i P_lm(theta_i) G_m(theta_i)
 // analys^T: F = LegendreSynth(q), g_ij = w_i * FourierSynth(F with m>=1 halved)

@@ -108,7 +108,9 @@ SIGNATURES = {
     "tsphere_leg_analys_fields": [_vp, _i, _i, _vp, C.POINTER(_vp), _ll, _vp],
     "tsphere_leg_imex_synth_fields": [_vp, _i, _i, C.POINTER(_vp), C.POINTER(_vp), _ll, _vp, _vp,
                                       _vp, _vp],
-    "tsphere_plan_fourier_table": [_vp, C.POINTER(_vp)],
+    "tsphere_leg_analys_imex_fields": [_vp, _i, _i, _vp, C.POINTER(_vp), C.POINTER(_vp), _ll, _vp,
+                                       _vp, _vp],
+    "tsphere_plan_fourier_table":[_vp, C.POINTER(_vp)],
     "tsphere_module_compile": [C.c_char_p, C.c_char_p, C.POINTER(_vp)],
     "tsphere_module_destroy": [_vp],
     "tsphere_module_kernel": [_vp, C.c_char_p, C.POINTER(_vp)],

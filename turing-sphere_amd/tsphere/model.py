@@ -45,7 +45,7 @@ class GpuModel:
     def create(sht, cfg: ShtConfig, source: str, param_names, state, view,
                stream: Stream | None = None, batch: bool = True,
                file_name: str = "model.m", runs: int = 1, grid: bool = True,
-               spectral: bool = True, fold: bool | None = None) -> "GpuModel":
+               spectral: bool = True, fold: bool = True) -> "GpuModel":
         """GpuModel.create (model.ts:103-163).  ``runs`` > 1: an ensemble of
         that many independent runs of the model (same grid), each value
         holding the runs back to back; the plan needs
@@ -58,11 +58,14 @@ class GpuModel:
         (bit-identical to the separate launches; at lmax 1023 129 against
         131-133 us per FP32 step, 140 against 163 with the FP64 recurrence);
         ``fold`` (used where the spectral op is off or does not apply, e.g.
-        above 1024 latitude pairs) folds the implicit solve into the next
-        step's synthesis launch; None = on for single runs, off for
-        ensembles, where it is available (each run's scalars in a device
-        array) but measured slower: at 64 runs of lmax 511 the synthesis
-        took 234 instead of 169 us to save a 36 us solve kernel."""
+        above 1024 latitude pairs, and for ensembles) folds the implicit
+        solve into another launch: a single run's into the next step's
+        synthesis, an ensemble's into the analysis (each run's scalars in a
+        device array; with the per-parity product analysis tile the solve
+        runs in the analysis kernel: 1680 against 1648 batched steps/s on
+        256 runs of lmax 63-511).  Folding an ensemble's solve into the
+        synthesis measured slower (at 64 runs of lmax 511 the synthesis took
+        234 instead of 169 us to save a 36 us solve kernel)."""
         self = GpuModel()
         self.runs = int(runs)
         npts = cfg.nlat * cfg.nphi
@@ -94,7 +97,7 @@ class GpuModel:
         self._step = ModelPlan.create(sht, PlanSpec(step_fn, feedback), host, batch, "step",
                                       runs=R, shared=shared, grid=grid,
                                       spectral=spectral and R == 1,
-                                      fold=(R == 1) if fold is None else fold)
+                                      fold=fold)
         host.upload("lam", eigenvalues(cfg, nlm), self.stream)
         self._stash = DeviceBuffer(max(1, len(state)) * 2 * nlm * R, "mgpu-state-stash")
         return self

@@ -182,8 +182,9 @@ class SpectralOp:
     s1: list                # scalar IR expressions per analysed field
     s2: list
     lam: str                # c_name of the eigenvalue array
-    mode: str = "spectral"  # or "fold": the grid op's analysis separate, the
-                            # implicit solve folded into the next synthesis
+    mode: str = "spectral"  # or "fold": the implicit solve folded into the
+                            # next synthesis (single runs) or into the
+                            # analysis (ensembles)
 
 
 @dataclass
@@ -279,11 +280,12 @@ class ModelPlan:
         kernel, writes every value it computes to its buffer (the synthesised
         grid values included), not only those read after it (a step tape
         whose values a reverse pass reads).
-        ``fold`` (with ``grid``, when not ``spectral``; ensembles too): the
-        same step keeps the analysis as its own launch and folds the implicit
-        solve into the next step's synthesis launch -- for an ensemble with
-        each run's s1, s2 in a device array that a parameter change
-        re-uploads."""
+        ``fold`` (with ``grid``, when not ``spectral``; ensembles too): a
+        single run keeps the analysis as its own launch and folds the
+        implicit solve into the next step's synthesis launch; an ensemble
+        folds it into the analysis launch (tsphere_leg_analys_imex_fields)
+        and synthesises the next step's state on its own, with each run's s1,
+        s2 in a device array that a parameter change re-uploads."""
         self = ModelPlan()
         self._sht = sht
         self.runs = runs = int(runs)
@@ -447,14 +449,13 @@ class ModelPlan:
 
     def _imex_coefs(self, op: "SpectralOp") -> np.ndarray:
         """An ensemble's folded implicit solve: (s1, s2) of field
-        species * runs + run, each evaluated from that run's scalars as the
-        generated kernel rounds them."""
-        by = {sp: f for f, sp in enumerate(op.spc)}
-        out = np.zeros((len(op.species), self.runs, 2), np.float32)
-        for sp in range(len(op.species)):
+        f * runs + run for analysed field f, each evaluated from that run's
+        scalars as the generated kernel rounds them."""
+        out = np.zeros((len(op.spc), self.runs, 2), np.float32)
+        for f in range(len(op.spc)):
             for r in range(self.runs):
-                out[sp, r, 0] = self._scalar(op.s1[by[sp]], r)
-                out[sp, r, 1] = self._scalar(op.s2[by[sp]], r)
+                out[f, r, 0] = self._scalar(op.s1[f], r)
+                out[f, r, 1] = self._scalar(op.s2[f], r)
         return out.reshape(-1)
 
     def _fuse(self, ops, outputs: set, host: HostBuffers, runs: int, shared_c,
@@ -707,22 +708,24 @@ class ModelPlan:
                 rc = L.tsphere_leg_synth_multi(self._sht.handle, len(G.srcs), self._ptrs(G.srcs),
                                                self._fm.ptr, sh)
             _lib.check(rc, G.label)
-        for _ in range(steps):
+        for k in range(steps):
             for op in self.ops:
                 if isinstance(op, SpectralOp) and ens:
-                    # fold, every run at once: the analysis of all runs, then
-                    # the implicit solve (scalars per run from the device
-                    # array) in front of the next step's synthesis
+                    # fold, every run at once: the analysis of all runs with
+                    # the implicit solve behind it (scalars per run from the
+                    # device array), then the next step's synthesis (the last
+                    # step's is left to the next call, which starts with its
+                    # own)
                     self.launch_grid(op.grid, stream)
                     G, h = op.grid, self._sht.handle
-                    _lib.check(L.tsphere_leg_analys_fields(h, len(G.dsts), self.runs,
-                                                           self._fm.ptr, self._ptrs(G.dsts), nq,
-                                                           sh), G.label)
-                    by = {sp: f for f, sp in enumerate(op.spc)}
-                    ana = [G.dsts[by[sp]] for sp in range(len(op.species))]
-                    _lib.check(L.tsphere_leg_imex_synth_fields(
-                        h, len(op.species), self.runs, self._ptrs(ana), self._ptrs(op.species),
-                        nq, self._bufs[op.lam].ptr, self._coef.ptr, self._fm.ptr, sh), op.label)
+                    st = [op.species[sp] for sp in op.spc]
+                    _lib.check(L.tsphere_leg_analys_imex_fields(
+                        h, len(G.dsts), self.runs, self._fm.ptr, self._ptrs(G.dsts),
+                        self._ptrs(st), nq, self._bufs[op.lam].ptr, self._coef.ptr, sh), op.label)
+                    if k < steps - 1:
+                        _lib.check(L.tsphere_leg_synth_fields(h, len(G.srcs), self.runs,
+                                                              self._ptrs(G.srcs), nq,
+                                                              self._fm.ptr, sh), G.label)
                     continue
                 if isinstance(op, SpectralOp):
                     self.launch_grid(op.grid, stream)
@@ -865,7 +868,7 @@ class ModelPlan:
             if isinstance(op, SpectralOp) and op.mode == "fold":
                 out.append(f"{op.grid.kind:<7} {op.grid.label}")
                 out.append(f"{'imex':<7} {op.label.split(' => ', 3)[-1]} (folded into the "
-                           f"next step's synthesis)")
+                           + ("analysis)" if self.runs > 1 else "next step's synthesis)"))
             elif isinstance(op, SpectralOp):
                 out.append(f"{op.grid.kind:<7} {op.grid.label}")
                 out.append(f"{op.kind:<7} {op.label.split(' => ', 3)[-1]} (with the "
