@@ -234,7 +234,8 @@ int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
  * copy f % per at qlm_stride floats for analysed and state alike.  With the
  * per-parity product analysis tile (impl 5) the step runs in the analysis
  * kernel and analysed is not written; other tiles write analysed and run the
- * step as a second kernel.  Any field count up to max_fields.  No reference
+ * step as a second kernel (analysed and state must not overlap).  Any field
+ * count up to max_fields.  No reference
  * counterpart of its own (it fuses encodeAnalysFrom and the implicit-solve
  * dispatch of plan.ts:436-478 for a batch of runs). */
 int tsphere_leg_analys_imex_fields(tsphere_plan p, int nptr, int per, const float *fm,

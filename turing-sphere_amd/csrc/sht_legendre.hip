@@ -3139,26 +3139,42 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
 }
 
 // ------------------------------- the analysis with the implicit step behind
-// X = (X + s1 A) / (1 + s2 lam) over the n2 floats of each field f (s1, s2 =
-// coef[2 f], coef[2 f + 1]): grid (x chunks, field)
-__global__ __launch_bounds__(256) void k_imex_fields(long long n2, const float* __restrict__ lam,
+// X = (X + s1 A) / (1 + s2 lam) over the nlm complex values of each field f
+// (s1, s2 = coef[2 f], coef[2 f + 1]; A and X distinct buffers): grid
+// (chunks of 1024 values, field), four values a thread, loads first
+__global__ __launch_bounds__(256) void k_imex_fields(long long nlm, const float2* __restrict__ lam,
                                                      const float* __restrict__ coef, Fields Fa,
                                                      Fields Fq) {
   const int f = blockIdx.y;
   const float s1 = coef[2 * f], s2 = coef[2 * f + 1];
-  const float* A = reinterpret_cast<const float*>(Fa.at(f));
-  float* X = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f)));
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n2;
-       i += (long long)gridDim.x * 256)
-    X[i] = imex_step(X[i], s1, A[i], s2, lam[i]);
+  const float2* __restrict__ A = reinterpret_cast<const float2*>(Fa.at(f));
+  float2* __restrict__ X = reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f)));
+  const long long i0 = (long long)blockIdx.x * 1024 + threadIdx.x;
+  float2 a[4], x[4], lm[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long long i = i0 + 256 * k;
+    if (i < nlm) {
+      a[k] = A[i];
+      x[k] = X[i];
+      lm[k] = lam[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long long i = i0 + 256 * k;
+    if (i < nlm)
+      X[i] = make_float2(imex_step(x[k].x, s1, a[k].x, s2, lm[k].x),
+                         imex_step(x[k].y, s1, a[k].y, s2, lm[k].y));
+  }
 }
 
 hipError_t launch_imex_fields(long long nlm, int nfield, const Fields& a, const Fields& q,
                               const float* lam, const float* coef, hipStream_t s) {
   if (nfield <= 0) return hipSuccess;
-  const long long n2 = 2 * nlm;
-  dim3 grid((unsigned)std::min<long long>((n2 + 255) / 256, 1024), nfield);
-  hipLaunchKernelGGL(k_imex_fields, grid, dim3(256), 0, s, n2, lam, coef, a, q);
+  dim3 grid((unsigned)((nlm + 1023) / 1024), nfield);
+  hipLaunchKernelGGL(k_imex_fields, grid, dim3(256), 0, s, nlm, reinterpret_cast<const float2*>(lam),
+                     coef, a, q);
   return hipGetLastError();
 }
 
