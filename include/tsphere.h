@@ -232,9 +232,10 @@ int tsphere_leg_imex_synth_fields(tsphere_plan p, int nptr, int per,
  * its analysed values rounded as written, (s1, s2) = coef[2 f], coef[2 f + 1]
  * (device array, as tsphere_leg_imex_synth_fields); field f = pointer f / per,
  * copy f % per at qlm_stride floats for analysed and state alike.  With the
- * per-parity product analysis tile (impl 5) the step runs in the analysis
- * kernel and analysed is not written; other tiles write analysed and run the
- * step as a second kernel (analysed and state must not overlap).  Any field
+ * 4x4-block or per-parity product analysis tiles (impl 3, 5) of the FP32 and
+ * FP64-recurrence forms the step runs in the analysis kernel and analysed is
+ * not written; other tiles write analysed and run the step as a second
+ * kernel (analysed and state must not overlap).  Any field
  * count up to max_fields.  No reference
  * counterpart of its own (it fuses encodeAnalysFrom and the implicit-solve
  * dispatch of plan.ts:436-478 for a batch of runs). */

@@ -247,14 +247,14 @@ def test_ensemble_grid_fusion_bit_exact(gpu):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("legendre,impl", [("fp32", None), ("fp32", 5), ("f64rec", None),
+@pytest.mark.parametrize("legendre,impl", [("fp32", 1), ("fp32", 3), ("fp32", 5), ("f64rec", 3),
                                            ("f64rec", 5), ("bf16x3", None)])
 @pytest.mark.parametrize("model,R", [("schnakenberg", 3), ("allencahn", 9)])
 def test_ensemble_fold_bit_exact(gpu, legendre, impl, model, R):
     """An ensemble step folds the implicit solve into the analysis
     (tsphere_leg_analys_imex_fields, each run's s1 and s2 from a device
-    array; the per-parity product analysis tile, impl 5, applies it in its
-    combine): equal to the ensemble plan with the separate generated solve
+    array; the 4x4-block and per-parity product analysis tiles, impl 3 and
+    5, apply it as they form each analysed value): equal to the ensemble plan with the separate generated solve
     kernel bit for bit, over graph replays, an eager remainder and a per-run
     parameter change between calls (an upload, no re-capture).  Nine
     single-species runs put a partial group of fields in the last

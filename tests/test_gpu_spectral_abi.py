@@ -277,8 +277,8 @@ def test_batched_fold_entry_matches_per_run_calls(gpu, legendre):
     plan.destroy()
 
 
-@pytest.mark.parametrize("legendre,impl", [("fp32", None), ("fp32", 5), ("f64rec", None),
-                                           ("f64rec", 5), ("bf16x3", None)])
+@pytest.mark.parametrize("legendre,impl", [("fp32", 1), ("fp32", 3), ("fp32", 5), ("f64rec", 1),
+                                           ("f64rec", 3), ("f64rec", 5), ("bf16x3", None)])
 @pytest.mark.parametrize("lmax,nlat,nphi,nsp,R", [(31, 32, 64, 2, 5), (63, 65, 128, 1, 19)])
 def test_batched_analysis_fold_entry(gpu, legendre, impl, lmax, nlat, nphi, nsp, R):
     """tsphere_leg_analys_imex_fields (the analysis with the implicit step of
@@ -286,10 +286,11 @@ def test_batched_analysis_fold_entry(gpu, legendre, impl, lmax, nlat, nphi, nsp,
     against tsphere_leg_analys_fields followed by the implicit step of
     tsphere_leg_imex_synth_fields: the same analysed values, rounded as
     written, enter the same step, so the state agrees bit for bit.  With
-    the per-parity product tile (impl 5) the step runs in the analysis
-    kernel's combine and the analysed buffers stay untouched; 19 runs leave
-    a partial group of 16 fields, an odd grid pairs the equator with
-    itself."""
+    the 4x4-block (impl 3) and per-parity product (impl 5) tiles the step
+    runs in the analysis kernel as it forms each analysed value and the
+    analysed buffers stay untouched; the VALU tile (impl 1) and the
+    split-BF16 form run it as a second kernel.  19 runs leave a partial
+    group of fields, an odd grid pairs the equator with itself."""
     import tsphere
     from tsphere import _lib
     from tsphere.device import DeviceBuffer
@@ -344,7 +345,7 @@ def test_batched_analysis_fold_entry(gpu, legendre, impl, lmax, nlat, nphi, nsp,
     want_x = np.stack([b.download(s) for b in xr])
     assert np.all(np.isfinite(got_x)) and np.array_equal(got_x, want_x)
     assert not np.array_equal(got_x, x0.reshape(nsp, -1))
-    if plan.get_tile("analys", nsp * R)["impl"] == 5:
+    if plan.get_tile("analys", nsp * R)["impl"] in (3, 5) and legendre != "bf16x3":
         assert np.all(np.isnan(got_a))
     else:
         assert np.array_equal(got_a, want_a)

@@ -241,6 +241,14 @@ __device__ __forceinline__ void imex_coefs(const ImexCoef& ic, int fld, float& s
   }
 }
 
+// X = (X + s1 a) / (1 + s2 lam) of one complex coefficient, (s1, s2) =
+// sc[0], sc[1]
+__device__ __forceinline__ void imex_apply(float2* X, float2 a, float2 lm, const float* sc) {
+  const float s1 = sc[0], s2 = sc[1];
+  const float2 x = *X;
+  *X = make_float2(imex_step(x.x, s1, a.x, s2, lm.x), imex_step(x.y, s1, a.y, s2, lm.y));
+}
+
 // stage_row_t with the implicit step in front: field f's state X is replaced
 // by X_new = (X + s1 A) / (1 + s2 lam) (A: its analysed values), written back
 // to the state, and c_l X_new staged as the synthesis coefficients.
@@ -1234,10 +1242,13 @@ constexpr int m4_analys_waves_per_eu() {
              : 1;
 }
 
-template <int NF, int K, class P>
+// IMEX: Fq is the state, and each analysed real coefficient a of field f
+// goes straight into its implicit step (s1, s2 = coef[2 f], coef[2 f + 1];
+// lam per real coefficient) instead of to memory
+template <int NF, int K, class P, bool IMEX = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(m4_analys_waves_per_eu<NF, K, P>())))
 void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict__ fm, Fields Fq,
-                     int rows) {
+                     int rows, const float* __restrict__ lam, const float* __restrict__ coef) {
   constexpr int FP = (NF + 1) / 2;
   constexpr int NRED = M4_NH * 64 * FP;  // a wave's sums between two barriers
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -1329,13 +1340,20 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
     for (int w = 1; w < nwaves; ++w) lblk = min(lblk, smin[w]);
     lblk = min(lblk, L + 1);
 
-    // coefficients below every lane's significance point are exactly 0
+    // coefficients below every lane's significance point are exactly 0 (the
+    // implicit step with a = 0 there)
     for (int i = m + tid; i < lblk; i += nthr)
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        if (f0 + f < nfield)
-          reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f)))[base + (i - m)] =
-              make_float2(0.f, 0.f);
+        if (f0 + f < nfield) {
+          float2* const X =
+              reinterpret_cast<float2*>(const_cast<char*>(Fq.at(f0 + f))) + base + (i - m);
+          if constexpr (IMEX)
+            imex_apply(X, make_float2(0.f, 0.f),
+                       reinterpret_cast<const float2*>(lam)[base + (i - m)], coef + 2 * (f0 + f));
+          else
+            *X = make_float2(0.f, 0.f);
+        }
 
     // Spans go M4_NH at a time between workgroup barriers: each wave reduces
     // its half-spans into red, then all threads combine the waves' sums.
@@ -1430,7 +1448,14 @@ void k_leg_analys_m4(typename P::Tables T, int nfield, const float2* __restrict_
                                     nwaves);
           float* dst = reinterpret_cast<float*>(const_cast<char*>(Fq.at(f0 + f))) +
                        2 * (base + (l - m));
-          dst[jj & 1] = s * T.cz[base + (l - m)];
+          const float a = s * T.cz[base + (l - m)];
+          if constexpr (IMEX) {
+            const float* sc = coef + 2 * (f0 + f);
+            dst[jj & 1] =
+                imex_step(dst[jj & 1], sc[0], a, sc[1], lam[2 * (base + (l - m)) + (jj & 1)]);
+          } else {
+            dst[jj & 1] = a;
+          }
         }
       }
       buf ^= 1;
@@ -1856,17 +1881,18 @@ hipError_t imex_synth_launch(const typename P::Tables& T, int nfield, const LegT
 #undef TS_IMEX_K
 }
 
-template <int NF, int K, class P>
+template <int NF, int K, class P, bool IMEX = false>
 hipError_t analys_m4(const typename P::Tables& T, int nfield, int nwaves, int rows,
-                     const float2* fm, const Fields& q, hipStream_t s) {
+                     const float2* fm, const Fields& q, hipStream_t s,
+                     const float* lam = nullptr, const float* coef = nullptr) {
   dim3 grid(rows == 1 ? T.mmax + 1 : T.mmax / 2 + 1, (nfield + NF - 1) / NF);
   // alpha, one tile per wave and chain, the waves' sums (two buffers of
   // 2 spans x 64 lanes x field pairs per wave)
   const size_t lds = sizeof(typename P::ab_t) * analys_ab_len(T.lmax) +
                      sizeof(float) * (size_t)nwaves * K * M4_SPAN * M4_TROW +
                      sizeof(float) * (size_t)2 * nwaves * M4_NH * 64 * ((NF + 1) / 2);
-  hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P>), grid, dim3(64 * nwaves), lds, s, T, nfield, fm,
-                     q, rows);
+  hipLaunchKernelGGL((k_leg_analys_m4<NF, K, P, IMEX>), grid, dim3(64 * nwaves), lds, s, T, nfield,
+                     fm, q, rows, lam, coef);
   return hipGetLastError();
 }
 
@@ -1881,13 +1907,14 @@ hipError_t synth_m4_nf(const typename P::Tables& T, int nfield, const LegTile& t
   }
 }
 
-template <int NF, class P>
+template <int NF, class P, bool IMEX = false>
 hipError_t analys_m4_nf(const typename P::Tables& T, int nfield, const LegTile& t,
-                        const float2* fm, const Fields& q, hipStream_t s) {
+                        const float2* fm, const Fields& q, hipStream_t s,
+                        const float* lam = nullptr, const float* coef = nullptr) {
   switch (t.k) {
-    case 1: return analys_m4<NF, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
-    case 2: return analys_m4<NF, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
-    case 4: return analys_m4<NF, 4, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 1: return analys_m4<NF, 1, P, IMEX>(T, nfield, t.nwaves, t.rows, fm, q, s, lam, coef);
+    case 2: return analys_m4<NF, 2, P, IMEX>(T, nfield, t.nwaves, t.rows, fm, q, s, lam, coef);
+    case 4: return analys_m4<NF, 4, P, IMEX>(T, nfield, t.nwaves, t.rows, fm, q, s, lam, coef);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1904,12 +1931,13 @@ hipError_t synth_m4_nf8(const typename P::Tables& T, int nfield, const LegTile& 
   }
 }
 
-template <class P>
+template <class P, bool IMEX = false>
 hipError_t analys_m4_nf8(const typename P::Tables& T, int nfield, const LegTile& t,
-                         const float2* fm, const Fields& q, hipStream_t s) {
+                         const float2* fm, const Fields& q, hipStream_t s,
+                         const float* lam = nullptr, const float* coef = nullptr) {
   switch (t.k) {
-    case 1: return analys_m4<8, 1, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
-    case 2: return analys_m4<8, 2, P>(T, nfield, t.nwaves, t.rows, fm, q, s);
+    case 1: return analys_m4<8, 1, P, IMEX>(T, nfield, t.nwaves, t.rows, fm, q, s, lam, coef);
+    case 2: return analys_m4<8, 2, P, IMEX>(T, nfield, t.nwaves, t.rows, fm, q, s, lam, coef);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2116,14 +2144,6 @@ __global__ __launch_bounds__(512) void k_leg_synth_g16(typename P::Tables T, int
         out[(long long)(nlat - 1 - jp) * (M + 1)] = comp ? make_float2(gs, sB) : make_float2(sA, gs);
       }
   }
-}
-
-// X = (X + s1 a) / (1 + s2 lam) of one complex coefficient, (s1, s2) =
-// sc[0], sc[1]
-__device__ __forceinline__ void imex_apply(float2* X, float2 a, float2 lm, const float* sc) {
-  const float s1 = sc[0], s2 = sc[1];
-  const float2 x = *X;
-  *X = make_float2(imex_step(x.x, s1, a.x, s2, lm.x), imex_step(x.y, s1, a.y, s2, lm.y));
 }
 
 // IMEX: Fq is the state, and each analysed coefficient a of field f goes
@@ -3185,6 +3205,12 @@ hipError_t launch_leg_analys_imex(const LegTables& T, int nfield, const float2* 
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_FP32, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
   if (t.impl == 5) return analys_g16_nf<RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return analys_m4_nf<1, RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+      case 2: return analys_m4_nf<2, RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+      case 8: return analys_m4_nf8<RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+      default: return analys_m4_nf<4, RecF, true>(T, nfield, t, fm, q, s, lam, coef);
+    }
   if (hipError_t e = launch_leg_analys(T, nfield, fm, a, tile, s)) return e;
   return launch_imex_fields(nlm_calc(T.lmax, T.mmax), nfield, a, q, lam, coef, s);
 }
@@ -3196,6 +3222,12 @@ hipError_t launch_leg_analys_imex_f64r(const LegTablesZ64& T, int nfield, const 
   const LegTile t = resolve_leg_tile(TSPHERE_LEG_F64REC, 1, T.npair, nfield, tile);
   if (!fp32_tile_ok(t, T.npair)) return hipErrorInvalidValue;
   if (t.impl == 5) return analys_g16_nf<RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+  if (t.impl == 3) switch (t.nf) {
+      case 1: return analys_m4_nf<1, RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+      case 2: return analys_m4_nf<2, RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+      case 8: return analys_m4_nf8<RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+      default: return analys_m4_nf<4, RecD, true>(T, nfield, t, fm, q, s, lam, coef);
+    }
   if (hipError_t e = launch_leg_analys_f64r(T, nfield, fm, a, tile, s)) return e;
   return launch_imex_fields(nlm_calc(T.lmax, T.mmax), nfield, a, q, lam, coef, s);
 }

@@ -241,9 +241,10 @@ hipError_t launch_leg_analys_f64r(const LegTablesZ64& T, int nfield, const float
                                   const Fields& q, LegTile tile, hipStream_t s);
 // The analysis of nfield fields with the implicit step of each behind it:
 // state q[f] = (q[f] + s1 A) / (1 + s2 lam), A its analysed values, (s1, s2)
-// = coef[2 f], coef[2 f + 1] in device memory.  The per-parity product tile
-// (impl 5) applies the step in its combine and leaves a[] untouched; other
-// tiles write A to a[] and run the step (launch_imex_fields) behind.
+// = coef[2 f], coef[2 f + 1] in device memory.  The 4x4-block and per-parity
+// product tiles (impl 3, 5) apply the step where they form each analysed
+// value and leave a[] untouched; other tiles write A to a[] and run the step
+// (launch_imex_fields) behind.
 hipError_t launch_imex_fields(long long nlm, int nfield, const Fields& a, const Fields& q,
                               const float* lam, const float* coef, hipStream_t s);
 hipError_t launch_leg_analys_imex(const LegTables& T, int nfield, const float2* fm,

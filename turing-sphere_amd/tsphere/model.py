@@ -61,9 +61,9 @@ class GpuModel:
         above 1024 latitude pairs, and for ensembles) folds the implicit
         solve into another launch: a single run's into the next step's
         synthesis, an ensemble's into the analysis (each run's scalars in a
-        device array; with the per-parity product analysis tile the solve
-        runs in the analysis kernel: 1702 against 1648 batched steps/s on
-        256 runs of lmax 63-511).  Folding an ensemble's solve into the
+        device array; with the 4x4-block and per-parity product analysis
+        tiles the solve runs in the analysis kernel: 1711 against 1648
+        batched steps/s on 256 runs of lmax 63-511).  Folding an ensemble's solve into the
         synthesis measured slower (at 64 runs of lmax 511 the synthesis took
         234 instead of 169 us to save a 36 us solve kernel)."""
         self = GpuModel()
