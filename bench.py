@@ -398,7 +398,7 @@ def ensemble_bench(steps: int = 200, legendre: str = "fp32"):
     """256 Allen-Cahn runs, 64 each at lmax 63/127/255/511 on (L+1) x 2(L+1)
     Gauss grids, eps = U[0.01, 0.05] (mulberry32 seed 0, its own stream),
     dt = 1e-3, u0 = U(-0.1, 0.1) (mulberry32 seed = run index), all runs
-    stepped by one HIP graph per 10 steps (groups as parallel branches)."""
+    stepped by one HIP graph per 20 steps (groups as parallel branches)."""
     import tsphere
     from tsphere.ensemble import MixedEnsemble
 
@@ -414,7 +414,7 @@ def ensemble_bench(steps: int = 200, legendre: str = "fp32"):
             noise.append((0.2 * mulberry_uniform(g.first_run + r, g.cfg.npts) - 0.1)
                          .astype(np.float32))
     ens.init(noise)
-    ens.graph_steps = 10
+    ens.graph_steps = 20
     ens.step(20)
     ens.sync()
     t0 = time.perf_counter()
